@@ -1,0 +1,16 @@
+/* Quantities: derived observables of an image.
+ * Replaces core/include/Spirit/Quantities.h (same names, argument meaning and error behaviour:
+ * errors are logged and a default value is returned). */
+#pragma once
+#ifndef SPIRIT_HIP_QUANTITIES_H
+#define SPIRIT_HIP_QUANTITIES_H
+#include "Spirit_Defines.h"
+
+/* Quantities.h:18 */
+SPIRIT_API void Quantity_Get_Average_Spin( State * state, float s[3], int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Quantities.h:21 */
+SPIRIT_API void Quantity_Get_Magnetization( State * state, float m[3], int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Quantities.h:24 (2D lattices: solid-angle sum over the lattice's triangles) */
+SPIRIT_API float Quantity_Get_Topological_Charge( State * state, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+#endif

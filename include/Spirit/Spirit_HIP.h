@@ -1,0 +1,42 @@
+/* Extensions of the C API that exist only in the HIP engine.  None of these
+ * has a counterpart in the reference; each comment says which callers use it. */
+#pragma once
+#ifndef SPIRIT_HIP_EXTENSIONS_H
+#define SPIRIT_HIP_EXTENSIONS_H
+#include "Spirit_Defines.h"
+
+/* For benchmarks and long single-shot loops: mode 1 (default) refreshes the host
+ * spin mirror after every Simulation_SingleShot / Simulation_N_Shot call; mode 0
+ * keeps the spins on the GPU until a System_* / Quantity_* getter, Simulation_Stop
+ * or the end of the run asks for them. */
+SPIRIT_API void Spirit_HIP_Set_Host_Sync( State * state, int mode ) SPIRIT_NOEXCEPT;
+
+/* For parity tests with a given noise sequence: the next n_steps LLG steps of the
+ * image use these standard normals (AoS, [n_steps][nos][3]) for the thermal field
+ * instead of the engine's counter-based Philox generator.  Call after
+ * Simulation_LLG_Start(..., singleshot=true).  NULL / 0 switches back. */
+SPIRIT_API void Spirit_HIP_Set_Thermal_Noise( State * state, const double * noise, int n_steps,
+                                              int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+/* For disordered samples and parameter fitting: per-site uniaxial anisotropy K_i [meV]
+ * (nos values, copied) along `normal`, added to the per-basis anisotropy.  NULL clears it. */
+SPIRIT_API void Spirit_HIP_Hamiltonian_Set_Site_Anisotropy( State * state, const double * K, const float normal[3],
+                                                            int idx_image SPIRIT_DEFAULT( -1 ),
+                                                            int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+/* For parity tests and tools: the full gradient dE/ds (AoS, nos*3, as the reference's
+ * Hamiltonian::Gradient_and_Energy) and energy of the image's current spins, evaluated
+ * on the GPU.  Returns 0 on success. */
+SPIRIT_API int Spirit_HIP_Gradient_and_Energy( State * state, double * gradient, double * energy,
+                                               int idx_image SPIRIT_DEFAULT( -1 ),
+                                               int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+/* For benchmarks: mean wall time [ms] of one step in the last Simulation_N_Shot /
+ * run batch of the image's LLG method (0 if none). */
+SPIRIT_API double Spirit_HIP_Get_Last_Step_Time( State * state, int idx_image SPIRIT_DEFAULT( -1 ),
+                                                 int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+/* Name of the GPU the engine runs on (empty if none is visible). */
+SPIRIT_API const char * Spirit_HIP_Device_Name( void ) SPIRIT_NOEXCEPT;
+
+#endif

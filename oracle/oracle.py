@@ -1,0 +1,347 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+``params_from_config`` restates the config-key semantics of the reference's
+parser (core/src/io/Configparser.cpp:271-567 geometry, 1256-1630 Heisenberg
+Hamiltonian) for the keys the workloads use, so that a test can build the
+oracle and the product from the same description.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+MAX_BASIS = 8
+MAX_PAIRS = 256
+
+MU_B = 0.057883817555
+MU_0 = 2.0133545 * 1e-28
+K_B = 0.08617330350
+GAMMA = 0.1760859644
+
+SOLVER_VP, SOLVER_SIB, SOLVER_DEPONDT, SOLVER_HEUN, SOLVER_RK4 = 0, 1, 2, 3, 4
+DDI_NONE, DDI_FFT, DDI_CUTOFF = 0, 1, 3
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int * 3),
+        ("n_cell_atoms", ctypes.c_int),
+        ("bravais", (ctypes.c_double * 3) * 3),
+        ("cell_atoms", (ctypes.c_double * 3) * MAX_BASIS),
+        ("lattice_constant", ctypes.c_double),
+        ("mu_s", ctypes.c_double * MAX_BASIS),
+        ("bc", ctypes.c_int * 3),
+        ("field_tesla", ctypes.c_double),
+        ("field_normal", ctypes.c_double * 3),
+        ("n_aniso", ctypes.c_int),
+        ("aniso_idx", ctypes.c_int * MAX_BASIS),
+        ("aniso_K", ctypes.c_double * MAX_BASIS),
+        ("aniso_normal", (ctypes.c_double * 3) * MAX_BASIS),
+        ("n_cubic", ctypes.c_int),
+        ("cubic_idx", ctypes.c_int * MAX_BASIS),
+        ("cubic_K", ctypes.c_double * MAX_BASIS),
+        ("site_aniso_K", ctypes.POINTER(ctypes.c_double)),
+        ("site_aniso_normal", ctypes.c_double * 3),
+        ("n_exch", ctypes.c_int),
+        ("exch", (ctypes.c_int * 5) * MAX_PAIRS),
+        ("exch_J", ctypes.c_double * MAX_PAIRS),
+        ("n_dmi", ctypes.c_int),
+        ("dmi", (ctypes.c_int * 5) * MAX_PAIRS),
+        ("dmi_D", ctypes.c_double * MAX_PAIRS),
+        ("dmi_normal", (ctypes.c_double * 3) * MAX_PAIRS),
+        ("ddi_method", ctypes.c_int),
+        ("ddi_n_periodic_images", ctypes.c_int * 3),
+        ("ddi_pb_zero_padding", ctypes.c_int),
+        ("ddi_radius", ctypes.c_double),
+    ]
+
+
+class OrcLLG(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("damping", ctypes.c_double),
+        ("temperature", ctypes.c_double),
+        ("direct_minimization", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+        ("noise", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.orc_shell_radii.argtypes = [P(OrcParams), i, P(d)]
+        L.orc_neighbours_in_shells.argtypes = [P(OrcParams), i, i, P(i), i]
+        L.orc_neighbours_in_shells.restype = i
+        L.orc_dmi_normal.argtypes = [P(OrcParams), P(i), i, P(d)]
+        L.orc_setup_shells.argtypes = [P(OrcParams), i, P(d), i, P(d), i]
+        L.orc_setup_shells.restype = i
+        L.orc_make_pairs_redundant.argtypes = [P(OrcParams)]
+        L.orc_idx_from_pair.argtypes = [i, P(i), P(i), i, P(i)]
+        L.orc_idx_from_pair.restype = i
+        L.orc_create.argtypes = [P(OrcParams)]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        L.orc_nos.argtypes = [vp]
+        L.orc_nos.restype = i
+        L.orc_positions.argtypes = [vp, P(d)]
+        L.orc_ddi_padded.argtypes = [vp, P(i)]
+        L.orc_ddi_dipole_matrices.argtypes = [vp, P(d)]
+        L.orc_gradient_and_energy.argtypes = [vp, P(d), P(d), P(d)]
+        L.orc_gradient.argtypes = [vp, P(d), P(d)]
+        L.orc_gradient_term.argtypes = [vp, i, P(d), P(d)]
+        L.orc_llg_iterate.argtypes = [vp, i, P(OrcLLG), P(d), i, ctypes.c_uint64, P(d), P(d), P(d)]
+        L.orc_normal3.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(d)]
+        L.orc_fft.argtypes = [P(d), i, i]
+        L.orc_rotate.argtypes = [P(d), P(d), d, P(d)]
+        L.orc_sib_transform.argtypes = [P(d), P(d), P(d)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# ---------------------------------------------------------------------------
+# config-dict -> oracle parameters
+# ---------------------------------------------------------------------------
+BRAVAIS = {
+    # core/src/data/Geometry.cpp:446-488
+    "sc": [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]],
+    "fcc": [[0.5, 0.0, 0.5], [0.5, 0.5, 0.0], [0.0, 0.5, 0.5]],
+    "bcc": [[0.5, 0.5, -0.5], [-0.5, 0.5, -0.5], [0.5, -0.5, -0.5]],
+    "hex2d": [[0.5 * math.sqrt(3), -0.5, 0.0], [0.5 * math.sqrt(3), 0.5, 0.0], [0.0, 0.0, 1.0]],
+    "hex2d60": [[0.5 * math.sqrt(3), -0.5, 0.0], [0.5 * math.sqrt(3), 0.5, 0.0], [0.0, 0.0, 1.0]],
+    "hex2d120": [[0.5, -0.5 * math.sqrt(3), 0.0], [0.5, 0.5 * math.sqrt(3), 0.0], [0.0, 0.0, 1.0]],
+}
+
+
+def _normalized(v):
+    v = [float(x) for x in v]
+    z = v[0] * v[0] + v[1] * v[1] + v[2] * v[2]
+    if z > 0:
+        n = math.sqrt(z)
+        v = [x / n for x in v]
+    return v
+
+
+def params_from_config(cfg, site_K=None):
+    """Build OrcParams from a config dict (same keys as the reference's input file).
+
+    Returns (params, keepalive) -- keepalive holds arrays referenced by pointer.
+    """
+    p = OrcParams()
+    bv = cfg.get("bravais_vectors")
+    if bv is None:
+        bv = BRAVAIS[cfg.get("bravais_lattice", "sc").lower()]
+    for k in range(3):
+        for c in range(3):
+            p.bravais[k][c] = float(bv[k][c])
+    basis = cfg.get("basis", [[0.0, 0.0, 0.0]])
+    p.n_cell_atoms = len(basis)
+    for a, pos in enumerate(basis):
+        for c in range(3):
+            p.cell_atoms[a][c] = float(pos[c])
+    nc = cfg.get("n_basis_cells", [100, 100, 1])
+    for c in range(3):
+        p.n_cells[c] = int(nc[c])
+    p.lattice_constant = float(cfg.get("lattice_constant", 1.0))
+    mu = cfg.get("mu_s", [1.0])
+    if not isinstance(mu, (list, tuple)):
+        mu = [mu]
+    for a in range(p.n_cell_atoms):
+        p.mu_s[a] = float(mu[a] if a < len(mu) else mu[0])
+    bc = cfg.get("boundary_conditions", [0, 0, 0])
+    for c in range(3):
+        p.bc[c] = 1 if int(bc[c]) != 0 else 0
+
+    p.field_tesla = float(cfg.get("external_field_magnitude", 0.0))
+    bn = _normalized(cfg.get("external_field_normal", [0, 0, 1]))
+    if math.sqrt(sum(x * x for x in bn)) < 1e-8:
+        bn = [0.0, 0.0, 1.0]
+    for c in range(3):
+        p.field_normal[c] = bn[c]
+
+    K = float(cfg.get("anisotropy_magnitude", 0.0))
+    kn = _normalized(cfg.get("anisotropy_normal", [0, 0, 1]))
+    if K != 0:
+        p.n_aniso = p.n_cell_atoms
+        for a in range(p.n_cell_atoms):
+            p.aniso_idx[a] = a
+            p.aniso_K[a] = K
+            for c in range(3):
+                p.aniso_normal[a][c] = kn[c]
+    K4 = float(cfg.get("cubic_anisotropy_magnitude", 0.0))
+    if K4 != 0:
+        p.n_cubic = p.n_cell_atoms
+        for a in range(p.n_cell_atoms):
+            p.cubic_idx[a] = a
+            p.cubic_K[a] = K4
+    keep = []
+    if site_K is not None:
+        arr = np.ascontiguousarray(site_K, dtype=np.float64)
+        keep.append(arr)
+        p.site_aniso_K = _dp(arr)
+        sn = _normalized(cfg.get("site_anisotropy_normal", [0, 0, 1]))
+        for c in range(3):
+            p.site_aniso_normal[c] = sn[c]
+
+    ham = cfg.get("hamiltonian", "heisenberg_neighbours")
+    if ham == "heisenberg_pairs":
+        ne = nd = 0
+        for row in cfg.get("pairs", []):
+            i, j, da, db, dc, Dx, Dy, Dz, J = row
+            dn = math.sqrt(Dx ** 2 + Dy ** 2 + Dz ** 2)
+            if dn != 0:
+                Dx, Dy, Dz = Dx / dn, Dy / dn, Dz / dn
+            if J != 0:
+                p.exch[ne][:] = [i, j, da, db, dc]
+                p.exch_J[ne] = J
+                ne += 1
+            if dn != 0:
+                p.dmi[nd][:] = [i, j, da, db, dc]
+                p.dmi_D[nd] = dn
+                p.dmi_normal[nd][:] = [Dx, Dy, Dz]
+                nd += 1
+        p.n_exch, p.n_dmi = ne, nd
+        lib().orc_make_pairs_redundant(ctypes.byref(p))
+    else:
+        nse = int(cfg.get("n_shells_exchange", 0))
+        nsd = int(cfg.get("n_shells_dmi", 0))
+        jij = np.array(cfg.get("jij", [0.0] * max(nse, 1)), dtype=np.float64)
+        dij = np.array(cfg.get("dij", [0.0] * max(nsd, 1)), dtype=np.float64)
+        rc = lib().orc_setup_shells(ctypes.byref(p), nse, _dp(jij), nsd, _dp(dij), int(cfg.get("dm_chirality", 1)))
+        assert rc == 0
+
+    m = cfg.get("ddi_method", "none")
+    p.ddi_method = {"none": DDI_NONE, "fft": DDI_FFT, "cutoff": DDI_CUTOFF}[m]
+    im = cfg.get("ddi_n_periodic_images", [4, 4, 4])
+    for c in range(3):
+        p.ddi_n_periodic_images[c] = int(im[c])
+    p.ddi_pb_zero_padding = 1 if int(cfg.get("ddi_pb_zero_padding", 1)) else 0
+    p.ddi_radius = float(cfg.get("ddi_radius", 0.0))
+    return p, keep
+
+
+class System:
+    """An oracle spin system (one image)."""
+
+    def __init__(self, cfg, site_K=None):
+        self.params, self._keep = params_from_config(cfg, site_K)
+        self.cfg = cfg
+        self.h = lib().orc_create(ctypes.byref(self.params))
+        self.nos = lib().orc_nos(self.h)
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def positions(self):
+        out = np.zeros((self.nos, 3))
+        lib().orc_positions(self.h, _dp(out))
+        return out
+
+    def padded(self):
+        out = np.zeros(3, dtype=np.int32)
+        lib().orc_ddi_padded(self.h, _ip(out))
+        return out
+
+    def gradient_and_energy(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.nos, 3)
+        g = np.zeros((self.nos, 3))
+        e = ctypes.c_double(0)
+        lib().orc_gradient_and_energy(self.h, _dp(s), _dp(g), ctypes.byref(e))
+        return g, e.value
+
+    def gradient(self, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.nos, 3)
+        g = np.zeros((self.nos, 3))
+        lib().orc_gradient(self.h, _dp(s), _dp(g))
+        return g
+
+    def gradient_term(self, term, spins):
+        s = np.ascontiguousarray(spins, dtype=np.float64).reshape(self.nos, 3)
+        g = np.zeros((self.nos, 3))
+        lib().orc_gradient_term(self.h, int(term), _dp(s), _dp(g))
+        return g
+
+    def llg(self, spins, solver, n_steps, dt=1e-3, damping=0.3, temperature=0.0, seed=0, step0=0,
+            direct_minimization=False, noise=None):
+        s = np.array(spins, dtype=np.float64).reshape(self.nos, 3).copy()
+        par = OrcLLG()
+        par.dt, par.damping, par.temperature = dt, damping, temperature
+        par.direct_minimization = 1 if direct_minimization else 0
+        par.seed = seed
+        keep = None
+        if noise is not None:
+            keep = np.ascontiguousarray(noise, dtype=np.float64)
+            par.noise = _dp(keep)
+        e = ctypes.c_double(0)
+        t = ctypes.c_double(0)
+        eff = np.zeros((self.nos, 3))
+        lib().orc_llg_iterate(self.h, int(solver), ctypes.byref(par), _dp(s), int(n_steps), int(step0),
+                              ctypes.byref(e), ctypes.byref(t), _dp(eff))
+        del keep
+        return s, e.value, t.value, eff
+
+
+def neighbours_in_shells(cfg, n_shells, redundant=True):
+    p, _ = params_from_config(dict(cfg, n_shells_exchange=0, n_shells_dmi=0))
+    out = np.zeros((MAX_PAIRS, 6), dtype=np.int32)
+    n = lib().orc_neighbours_in_shells(ctypes.byref(p), n_shells, 1 if redundant else 0, _ip(out), MAX_PAIRS)
+    assert n >= 0
+    return out[:n].copy()
+
+
+def fft(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().orc_fft(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(a), sign)
+    return a
+
+
+def normal3(seed, step, site):
+    out = np.zeros(3)
+    lib().orc_normal3(seed, step, site, _dp(out))
+    return out
+
+
+def random_spins(nos, seed):
+    """i.i.d. uniform unit vectors (numpy Generator), the workloads' injected initial state."""
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1.0, 1.0, nos)
+    phi = rng.uniform(-math.pi, math.pi, nos)
+    r = np.sqrt(1.0 - z * z)
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)

@@ -1,0 +1,1371 @@
+/*
+ * spirit_oracle.c -- CPU restatement of the reference's FP64 effective-field
+ * and LLG step path.  TEST INFRASTRUCTURE ONLY (see spirit_oracle.h).
+ *
+ * Reference files followed (paths relative to the reference checkout):
+ *   core/src/engine/Neighbours.cpp, core/src/data/Geometry.cpp,
+ *   core/src/engine/Hamiltonian_Heisenberg.cpp, core/include/engine/Vectormath.hpp,
+ *   core/src/engine/Vectormath.cpp, core/src/engine/Method_LLG.cpp,
+ *   core/include/engine/Solver_{Depondt,Heun,SIB,RK4}.hpp, core/src/engine/Solver_Kernels.cpp
+ *
+ * Arithmetic follows the reference's expression order (Eigen evaluates small
+ * fixed-size expressions component by component, left to right), compiled with
+ * -ffp-contract=off so that no FMA is introduced.
+ */
+#include "spirit_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* small vector helpers (Eigen semantics)                              */
+/* ------------------------------------------------------------------ */
+static inline double v3_dot( const double * a, const double * b )
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static inline double v3_sqnorm( const double * a )
+{
+    return a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+}
+static inline void v3_cross( const double * a, const double * b, double * o )
+{
+    double x = a[1] * b[2] - a[2] * b[1];
+    double y = a[2] * b[0] - a[0] * b[2];
+    double z = a[0] * b[1] - a[1] * b[0];
+    o[0]     = x;
+    o[1]     = y;
+    o[2]     = z;
+}
+/* Eigen normalize(): divide by sqrt(squaredNorm) if > 0, else unchanged (Dot.h:145-151) */
+static inline void v3_normalize( double * a )
+{
+    double z = v3_sqnorm( a );
+    if( z > 0 )
+    {
+        double n = sqrt( z );
+        a[0] /= n;
+        a[1] /= n;
+        a[2] /= n;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* geometry                                                            */
+/* ------------------------------------------------------------------ */
+struct orc_sys
+{
+    orc_params p;
+    int nos;
+    int n_cells_total;
+    double * positions; /* AoS nos*3 */
+    double * mu_s;      /* nos */
+    double * site_K;    /* nos or NULL */
+    /* DDI (FFT) */
+    int padded[3];
+    int M;
+    int n_inter;
+    int inter_lookup[ORC_MAX_BASIS * ORC_MAX_BASIS];
+    double * d_hat;   /* complex [n_inter][6][M] interleaved */
+    double * s_hat;   /* complex [n_atoms][3][M] */
+    double * res_hat; /* complex [n_atoms][3][M] */
+    /* scratch for LLG */
+    double *f1, *fv1, *f2, *fv2, *sp, *tmp, *xi, *k1, *k2, *k3, *k4, *g;
+};
+
+/* positions, core/src/data/Geometry.cpp:142-161 */
+static void compute_positions( const orc_params * p, double * pos )
+{
+    int Na = p->n_cells[0], Nb = p->n_cells[1], Nc = p->n_cells[2], N = p->n_cell_atoms;
+    for( int dc = 0; dc < Nc; ++dc )
+        for( int db = 0; db < Nb; ++db )
+            for( int da = 0; da < Na; ++da )
+                for( int iatom = 0; iatom < N; ++iatom )
+                {
+                    long ispin = iatom + (long)dc * N * Nb * Na + (long)db * N * Na + (long)da * N;
+                    for( int c = 0; c < 3; ++c )
+                    {
+                        double v = ( (double)da + p->cell_atoms[iatom][0] ) * p->bravais[0][c]
+                                   + ( (double)db + p->cell_atoms[iatom][1] ) * p->bravais[1][c]
+                                   + ( (double)dc + p->cell_atoms[iatom][2] ) * p->bravais[2][c];
+                        pos[3 * ispin + c] = p->lattice_constant * v;
+                    }
+                }
+}
+
+/* cell-0 position of basis atom i (positions[i] for i < n_cell_atoms) */
+static void basis_position( const orc_params * p, int iatom, double * out )
+{
+    for( int c = 0; c < 3; ++c )
+    {
+        double v = ( 0.0 + p->cell_atoms[iatom][0] ) * p->bravais[0][c]
+                   + ( 0.0 + p->cell_atoms[iatom][1] ) * p->bravais[1][c]
+                   + ( 0.0 + p->cell_atoms[iatom][2] ) * p->bravais[2][c];
+        out[c] = p->lattice_constant * v;
+    }
+}
+
+static inline int imin( int a, int b )
+{
+    return a < b ? a : b;
+}
+
+/* core/src/engine/Neighbours.cpp:15-81 */
+void orc_shell_radii( const orc_params * p, int n_shells, double * radii )
+{
+    const double min_shell_width = 1e-3;
+    double ta[3], tb[3], tc[3];
+    for( int c = 0; c < 3; ++c )
+    {
+        ta[c] = p->lattice_constant * p->bravais[0][c];
+        tb[c] = p->lattice_constant * p->bravais[1][c];
+        tc[c] = p->lattice_constant * p->bravais[2][c];
+    }
+    int max_n = n_shells + 2;
+    int i_max = imin( max_n, p->n_cells[0] - 1 );
+    int j_max = imin( max_n, p->n_cells[1] - 1 );
+    int k_max = imin( max_n, p->n_cells[2] - 1 );
+    if( sqrt( v3_sqnorm( ta ) ) == 0.0 )
+        i_max = 0;
+    if( sqrt( v3_sqnorm( tb ) ) == 0.0 )
+        j_max = 0;
+    if( sqrt( v3_sqnorm( tc ) ) == 0.0 )
+        k_max = 0;
+
+    double outermost = 0, previous = 0;
+    for( int s = 0; s < n_shells; ++s )
+    {
+        radii[s]  = 0;
+        previous  = outermost;
+        outermost = DBL_MAX;
+        for( int a1 = 0; a1 < p->n_cell_atoms; ++a1 )
+        {
+            const double * pos_one = p->cell_atoms[a1];
+            for( int i = i_max; i >= 0; --i )
+                for( int j = j_max; j >= -j_max; --j )
+                    for( int k = k_max; k >= -k_max; --k )
+                        for( int a2 = 0; a2 < p->n_cell_atoms; ++a2 )
+                        {
+                            if( !( a1 == a2 && i == 0 && j == 0 && k == 0 ) )
+                            {
+                                double d[3];
+                                for( int c = 0; c < 3; ++c )
+                                {
+                                    double pos_two = p->cell_atoms[a2][c] + i * ta[c] + j * tb[c] + k * tc[c];
+                                    d[c]           = pos_one[c] - pos_two;
+                                }
+                                double delta = sqrt( v3_sqnorm( d ) );
+                                if( delta - previous > min_shell_width && delta < outermost )
+                                {
+                                    outermost = delta;
+                                    radii[s]  = delta;
+                                }
+                            }
+                        }
+        }
+    }
+}
+
+/* core/src/engine/Neighbours.cpp:83-156 */
+int orc_neighbours_in_shells( const orc_params * p, int n_shells, int redundant, int * out, int max_pairs )
+{
+    const double min_shell_width = 1e-3;
+    double radii[64];
+    if( n_shells > 64 )
+        return 0;
+    orc_shell_radii( p, n_shells, radii );
+    double ta[3], tb[3], tc[3];
+    for( int c = 0; c < 3; ++c )
+    {
+        ta[c] = p->lattice_constant * p->bravais[0][c];
+        tb[c] = p->lattice_constant * p->bravais[1][c];
+        tc[c] = p->lattice_constant * p->bravais[2][c];
+    }
+    int max_n = n_shells + 2;
+    int i_max = imin( max_n, p->n_cells[0] - 1 );
+    int j_max = imin( max_n, p->n_cells[1] - 1 );
+    int k_max = imin( max_n, p->n_cells[2] - 1 );
+    int i_min = -i_max, j_min = -j_max, k_min = -k_max;
+    if( sqrt( v3_sqnorm( ta ) ) == 0.0 )
+        i_max = 0;
+    if( sqrt( v3_sqnorm( tb ) ) == 0.0 )
+        j_max = 0;
+    if( sqrt( v3_sqnorm( tc ) ) == 0.0 )
+        k_max = 0;
+
+    int n = 0;
+    for( int a1 = 0; a1 < p->n_cell_atoms; ++a1 )
+    {
+        int second_atom_min = redundant ? 0 : a1;
+        const double * pos_one = p->cell_atoms[a1];
+        for( int s = 0; s < n_shells; ++s )
+        {
+            double radius = radii[s];
+            for( int i = i_max; i >= i_min; --i )
+                for( int j = j_max; j >= j_min; --j )
+                    for( int k = k_max; k >= k_min; --k )
+                        for( int a2 = second_atom_min; a2 < p->n_cell_atoms; ++a2 )
+                        {
+                            if( ( a2 > a1 ) || ( i > 0 || ( i == 0 && j > 0 ) || ( i == 0 && j == 0 && k > 0 ) )
+                                || redundant )
+                            {
+                                double d[3];
+                                for( int c = 0; c < 3; ++c )
+                                {
+                                    double pos_two = p->cell_atoms[a2][c] + i * ta[c] + j * tb[c] + k * tc[c];
+                                    d[c]           = pos_one[c] - pos_two;
+                                }
+                                double delta = sqrt( v3_sqnorm( d ) );
+                                if( fabs( delta - radius ) < min_shell_width )
+                                {
+                                    if( n < max_pairs )
+                                    {
+                                        int * o = out + 6 * n;
+                                        o[0]    = a1;
+                                        o[1]    = a2;
+                                        o[2]    = i;
+                                        o[3]    = j;
+                                        o[4]    = k;
+                                        o[5]    = s;
+                                    }
+                                    ++n;
+                                }
+                            }
+                        }
+        }
+    }
+    return n <= max_pairs ? n : -n;
+}
+
+/* core/src/engine/Neighbours.cpp:249-286 */
+void orc_dmi_normal( const orc_params * p, const int pair[5], int chirality, double out[3] )
+{
+    double ipos[3], jpos[3], d[3];
+    basis_position( p, pair[0], ipos );
+    basis_position( p, pair[1], jpos );
+    for( int c = 0; c < 3; ++c )
+    {
+        double ta = p->lattice_constant * p->bravais[0][c];
+        double tb = p->lattice_constant * p->bravais[1][c];
+        double tc = p->lattice_constant * p->bravais[2][c];
+        jpos[c]   = jpos[c] + pair[2] * ta + pair[3] * tb + pair[4] * tc;
+    }
+    const double ez[3] = { 0, 0, 1 };
+    if( chirality == 1 || chirality == 2 || chirality == -2 )
+    {
+        for( int c = 0; c < 3; ++c )
+            d[c] = jpos[c] - ipos[c];
+        v3_normalize( d );
+        if( chirality == 1 )
+        {
+            out[0] = d[0], out[1] = d[1], out[2] = d[2];
+        }
+        else if( chirality == 2 )
+            v3_cross( d, ez, out );
+        else
+            v3_cross( ez, d, out );
+    }
+    else if( chirality == -1 )
+    {
+        for( int c = 0; c < 3; ++c )
+            d[c] = ipos[c] - jpos[c];
+        v3_normalize( d );
+        out[0] = d[0], out[1] = d[1], out[2] = d[2];
+    }
+    else
+    {
+        out[0] = out[1] = out[2] = 0;
+    }
+}
+
+/* Hamiltonian_Heisenberg.cpp:101-160 (shell branch, redundant neighbours) */
+int orc_setup_shells( orc_params * p, int n_shells_exch, const double * jij, int n_shells_dmi, const double * dij,
+                      int chirality )
+{
+    int buf[6 * ORC_MAX_PAIRS];
+    p->n_exch = 0;
+    p->n_dmi  = 0;
+    if( n_shells_exch > 0 )
+    {
+        int n = orc_neighbours_in_shells( p, n_shells_exch, 1, buf, ORC_MAX_PAIRS );
+        if( n < 0 )
+            return -1;
+        for( int k = 0; k < n; ++k )
+        {
+            for( int q = 0; q < 5; ++q )
+                p->exch[k][q] = buf[6 * k + q];
+            p->exch_J[k] = jij[buf[6 * k + 5]];
+        }
+        p->n_exch = n;
+    }
+    if( n_shells_dmi > 0 )
+    {
+        int n = orc_neighbours_in_shells( p, n_shells_dmi, 1, buf, ORC_MAX_PAIRS );
+        if( n < 0 )
+            return -1;
+        for( int k = 0; k < n; ++k )
+        {
+            for( int q = 0; q < 5; ++q )
+                p->dmi[k][q] = buf[6 * k + q];
+            orc_dmi_normal( p, p->dmi[k], chirality, p->dmi_normal[k] );
+            p->dmi_D[k] = dij[buf[6 * k + 5]];
+        }
+        p->n_dmi = n;
+    }
+    return 0;
+}
+
+/* Hamiltonian_Heisenberg.cpp:121-133 and 152-163 */
+void orc_make_pairs_redundant( orc_params * p )
+{
+    int ne = p->n_exch;
+    for( int k = 0; k < ne && p->n_exch < ORC_MAX_PAIRS; ++k )
+    {
+        int * q = p->exch[p->n_exch];
+        q[0]    = p->exch[k][1];
+        q[1]    = p->exch[k][0];
+        q[2]    = -p->exch[k][2];
+        q[3]    = -p->exch[k][3];
+        q[4]    = -p->exch[k][4];
+        p->exch_J[p->n_exch] = p->exch_J[k];
+        p->n_exch++;
+    }
+    int nd = p->n_dmi;
+    for( int k = 0; k < nd && p->n_dmi < ORC_MAX_PAIRS; ++k )
+    {
+        int * q = p->dmi[p->n_dmi];
+        q[0]    = p->dmi[k][1];
+        q[1]    = p->dmi[k][0];
+        q[2]    = -p->dmi[k][2];
+        q[3]    = -p->dmi[k][3];
+        q[4]    = -p->dmi[k][4];
+        p->dmi_D[p->n_dmi] = p->dmi_D[k];
+        for( int c = 0; c < 3; ++c )
+            p->dmi_normal[p->n_dmi][c] = -p->dmi_normal[k][c];
+        p->n_dmi++;
+    }
+}
+
+/* core/include/engine/Vectormath.hpp:437-528 */
+int orc_idx_from_pair( int ispin, const int bc[3], const int n_cells[3], int N, const int pair[5] )
+{
+    if( pair[0] != ispin % N )
+        return -1;
+    int Na = n_cells[0], Nb = n_cells[1], Nc = n_cells[2];
+    if( abs( pair[2] ) > Na || abs( pair[3] ) > Nb || abs( pair[4] ) > Nc )
+        return -1;
+    int nic = ispin / ( N * Na * Nb );
+    int nib = ( ispin - nic * N * Na * Nb ) / ( N * Na );
+    int nia = ( ispin - nic * N * Na * Nb - nib * N * Na ) / N;
+    int nja = nia + pair[2], njb = nib + pair[3], njc = nic + pair[4];
+    if( bc[0] || ( 0 <= nja && nja < Na ) )
+    {
+        if( nja < 0 )
+            nja += Na;
+        if( nja >= Na )
+            nja -= Na;
+    }
+    else
+        return -1;
+    if( bc[1] || ( 0 <= njb && njb < Nb ) )
+    {
+        if( njb < 0 )
+            njb += Nb;
+        if( njb >= Nb )
+            njb -= Nb;
+    }
+    else
+        return -1;
+    if( bc[2] || ( 0 <= njc && njc < Nc ) )
+    {
+        if( njc < 0 )
+            njc += Nc;
+        if( njc >= Nc )
+            njc -= Nc;
+    }
+    else
+        return -1;
+    return pair[1] + nja * N + njb * N * Na + njc * N * Na * Nb;
+}
+
+/* ------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix DFT (decimation in time), any length      */
+/* ------------------------------------------------------------------ */
+typedef struct
+{
+    double re, im;
+} cpx;
+
+static int smallest_factor( int n )
+{
+    if( n % 4 == 0 )
+        return 4;
+    if( n % 2 == 0 )
+        return 2;
+    for( int f = 3; f * f <= n; f += 2 )
+        if( n % f == 0 )
+            return f;
+    return n;
+}
+
+/* W: table of the N-th roots exp(sign*2*pi*i*e/N), e in [0,N).  wstep = N/n */
+static void fft_rec( const cpx * in, long istride, cpx * out, int n, const cpx * W, int N, int wstep, cpx * scratch )
+{
+    if( n == 1 )
+    {
+        out[0] = in[0];
+        return;
+    }
+    int p = smallest_factor( n );
+    int m = n / p;
+    for( int r = 0; r < p; ++r )
+        fft_rec( in + r * istride, istride * p, out + r * m, m, W, N, wstep * p, scratch );
+    for( int k = 0; k < m; ++k )
+    {
+        for( int q = 0; q < p; ++q )
+        {
+            double sr = 0, si = 0;
+            for( int r = 0; r < p; ++r )
+            {
+                long e      = ( (long)r * ( k + (long)q * m ) ) % n;
+                const cpx w = W[e * wstep];
+                const cpx y = out[r * m + k];
+                sr += y.re * w.re - y.im * w.im;
+                si += y.re * w.im + y.im * w.re;
+            }
+            scratch[q].re = sr;
+            scratch[q].im = si;
+        }
+        for( int q = 0; q < p; ++q )
+            out[k + q * m] = scratch[q];
+    }
+}
+
+static cpx * make_roots( int N, int sign )
+{
+    cpx * W = (cpx *)malloc( sizeof( cpx ) * (size_t)N );
+    for( int e = 0; e < N; ++e )
+    {
+        long double a = (long double)sign * 2.0L * 3.14159265358979323846264338327950288L * (long double)e
+                        / (long double)N;
+        W[e].re = (double)cosl( a );
+        W[e].im = (double)sinl( a );
+    }
+    return W;
+}
+
+void orc_fft( double * data, int n, int sign )
+{
+    cpx * W   = make_roots( n, sign );
+    cpx * in  = (cpx *)malloc( sizeof( cpx ) * (size_t)n );
+    cpx * scr = (cpx *)malloc( sizeof( cpx ) * (size_t)n );
+    memcpy( in, data, sizeof( cpx ) * (size_t)n );
+    fft_rec( in, 1, (cpx *)data, n, W, n, 1, scr );
+    free( W );
+    free( in );
+    free( scr );
+}
+
+/* 3-D complex FFT over grid [c][b][a] (a fastest), `count` consecutive arrays of M points. */
+static void fft3d( cpx * data, int count, const int dims[3], int sign )
+{
+    int Pa = dims[0], Pb = dims[1], Pc = dims[2];
+    long M = (long)Pa * Pb * Pc;
+    for( int axis = 0; axis < 3; ++axis )
+    {
+        int n = dims[axis];
+        if( n == 1 )
+            continue;
+        cpx * W      = make_roots( n, sign );
+        long stride  = axis == 0 ? 1 : ( axis == 1 ? Pa : (long)Pa * Pb );
+        long n_lines = M / n;
+#pragma omp parallel
+        {
+            cpx * buf = (cpx *)malloc( sizeof( cpx ) * (size_t)n * 3 );
+            cpx * out = buf + n;
+            cpx * scr = buf + 2 * n;
+#pragma omp for schedule( static )
+            for( long t = 0; t < (long)count * n_lines; ++t )
+            {
+                long arr  = t / n_lines;
+                long line = t % n_lines;
+                long base;
+                if( axis == 0 )
+                    base = line * Pa;
+                else if( axis == 1 )
+                    base = ( line / Pa ) * (long)Pa * Pb + ( line % Pa );
+                else
+                    base = line;
+                cpx * d = data + arr * M + base;
+                for( int i = 0; i < n; ++i )
+                    buf[i] = d[i * stride];
+                fft_rec( buf, 1, out, n, W, n, 1, scr );
+                for( int i = 0; i < n; ++i )
+                    d[i * stride] = out[i];
+            }
+            free( buf );
+        }
+        free( W );
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* DDI setup: Hamiltonian_Heisenberg.cpp:1406-1595                      */
+/* ------------------------------------------------------------------ */
+static double ddi_mult( void )
+{
+    return ORC_MU_0 * ORC_MU_B * ORC_MU_B / ( 4 * ORC_PI * 1e-30 );
+}
+
+void orc_ddi_padded( const orc_sys * s, int out[3] )
+{
+    for( int i = 0; i < 3; ++i )
+    {
+        out[i]   = s->p.n_cells[i];
+        int pad  = s->p.n_cells[i] > 1 && ( s->p.bc[i] == 0 || s->p.ddi_pb_zero_padding );
+        if( pad )
+            out[i] *= 2;
+    }
+}
+
+static void ddi_inter_lookup( const orc_params * p, int * lookup, int * n_inter )
+{
+    int b_inter = -1;
+    int N       = p->n_cell_atoms;
+    for( int b1 = 0; b1 < N; ++b1 )
+        for( int b2 = 0; b2 < N; ++b2 )
+        {
+            if( b1 == b2 && b1 != 0 )
+            {
+                lookup[b1 + b2 * N] = 0;
+                continue;
+            }
+            b_inter++;
+            lookup[b1 + b2 * N] = b_inter;
+        }
+    *n_inter = b_inter + 1;
+}
+
+/* Real-space dipole matrices on the padded grid, Hamiltonian_Heisenberg.cpp:1406-1499.
+   out: [b_inter][6][M], grid index a + Pa*(b + Pb*c). */
+void orc_ddi_dipole_matrices( const orc_sys * s, double * out )
+{
+    const orc_params * p = &s->p;
+    const double mult    = ddi_mult();
+    int Na = p->n_cells[0], Nb = p->n_cells[1], Nc = p->n_cells[2];
+    int Pa = s->padded[0], Pb = s->padded[1], Pc = s->padded[2];
+    long M    = (long)Pa * Pb * Pc;
+    int img_a = p->bc[0] == 0 ? 0 : p->ddi_n_periodic_images[0];
+    int img_b = p->bc[1] == 0 ? 0 : p->ddi_n_periodic_images[1];
+    int img_c = p->bc[2] == 0 ? 0 : p->ddi_n_periodic_images[2];
+    int N     = p->n_cell_atoms;
+    int b_inter = -1;
+    for( int b1 = 0; b1 < N; ++b1 )
+        for( int b2 = 0; b2 < N; ++b2 )
+        {
+            if( b1 == b2 && b1 != 0 )
+                continue;
+            b_inter++;
+            double * o = out + (long)b_inter * 6 * M;
+#pragma omp parallel for collapse( 2 )
+            for( int c = 0; c < Pc; ++c )
+                for( int b = 0; b < Pb; ++b )
+                    for( int a = 0; a < Pa; ++a )
+                    {
+                        int a_idx = a < Na ? a : a - Pa;
+                        int b_idx = b < Nb ? b : b - Pb;
+                        int c_idx = c < Nc ? c : c - Pc;
+                        double Dxx = 0, Dxy = 0, Dxz = 0, Dyy = 0, Dyz = 0, Dzz = 0;
+                        for( int a_pb = -img_a; a_pb <= img_a; a_pb++ )
+                            for( int b_pb = -img_b; b_pb <= img_b; b_pb++ )
+                                for( int c_pb = -img_c; c_pb <= img_c; c_pb++ )
+                                {
+                                    double fa = (double)( a_idx + a_pb * Na ) + p->cell_atoms[b1][0]
+                                                - p->cell_atoms[b2][0];
+                                    double fb = (double)( b_idx + b_pb * Nb ) + p->cell_atoms[b1][1]
+                                                - p->cell_atoms[b2][1];
+                                    double fc = (double)( c_idx + c_pb * Nc ) + p->cell_atoms[b1][2]
+                                                - p->cell_atoms[b2][2];
+                                    double diff[3];
+                                    for( int q = 0; q < 3; ++q )
+                                        diff[q] = p->lattice_constant
+                                                  * ( fa * p->bravais[0][q] + fb * p->bravais[1][q]
+                                                      + fc * p->bravais[2][q] );
+                                    double d = sqrt( v3_sqnorm( diff ) );
+                                    if( d > 1e-10 )
+                                    {
+                                        double d3 = d * d * d;
+                                        double d5 = d * d * d * d * d;
+                                        Dxx += mult * ( 3 * diff[0] * diff[0] / d5 - 1 / d3 );
+                                        Dxy += mult * 3 * diff[0] * diff[1] / d5;
+                                        Dxz += mult * 3 * diff[0] * diff[2] / d5;
+                                        Dyy += mult * ( 3 * diff[1] * diff[1] / d5 - 1 / d3 );
+                                        Dyz += mult * 3 * diff[1] * diff[2] / d5;
+                                        Dzz += mult * ( 3 * diff[2] * diff[2] / d5 - 1 / d3 );
+                                    }
+                                }
+                        long idx       = a + (long)Pa * ( b + (long)Pb * c );
+                        o[0 * M + idx] = Dxx;
+                        o[1 * M + idx] = Dxy;
+                        o[2 * M + idx] = Dxz;
+                        o[3 * M + idx] = Dyy;
+                        o[4 * M + idx] = Dyz;
+                        o[5 * M + idx] = Dzz;
+                    }
+        }
+}
+
+static void ddi_prepare( orc_sys * s )
+{
+    orc_ddi_padded( s, s->padded );
+    s->M = s->padded[0] * s->padded[1] * s->padded[2];
+    ddi_inter_lookup( &s->p, s->inter_lookup, &s->n_inter );
+    long M     = s->M;
+    double * D = (double *)malloc( sizeof( double ) * 6 * (size_t)s->n_inter * (size_t)M );
+    orc_ddi_dipole_matrices( s, D );
+    s->d_hat = (double *)calloc( 2 * 6 * (size_t)s->n_inter * (size_t)M, sizeof( double ) );
+    for( long t = 0; t < 6L * s->n_inter * M; ++t )
+        s->d_hat[2 * t] = D[t];
+    free( D );
+    fft3d( (cpx *)s->d_hat, 6 * s->n_inter, s->padded, -1 );
+    s->s_hat   = (double *)malloc( sizeof( double ) * 2 * 3 * (size_t)s->p.n_cell_atoms * (size_t)M );
+    s->res_hat = (double *)malloc( sizeof( double ) * 2 * 3 * (size_t)s->p.n_cell_atoms * (size_t)M );
+}
+
+/* Gradient_DDI_FFT + FFT_Spins, Hamiltonian_Heisenberg.cpp:920-1014, 1373-1404 */
+static void gradient_ddi_fft( orc_sys * s, const double * spins, double * grad )
+{
+    const orc_params * p = &s->p;
+    int Na = p->n_cells[0], Nb = p->n_cells[1], Nc = p->n_cells[2], N = p->n_cell_atoms;
+    int Pa = s->padded[0], Pb = s->padded[1];
+    long M = s->M;
+    memset( s->s_hat, 0, sizeof( double ) * 2 * 3 * (size_t)N * (size_t)M );
+#pragma omp parallel for collapse( 2 )
+    for( int c = 0; c < Nc; ++c )
+        for( int b = 0; b < Nb; ++b )
+            for( int a = 0; a < Na; ++a )
+                for( int bi = 0; bi < N; ++bi )
+                {
+                    long io  = bi + (long)N * ( a + (long)Na * ( b + (long)Nb * c ) );
+                    long idx = a + (long)Pa * ( b + (long)Pb * c );
+                    for( int q = 0; q < 3; ++q )
+                        s->s_hat[2 * ( ( (long)bi * 3 + q ) * M + idx )] = spins[3 * io + q] * s->mu_s[io];
+                }
+    fft3d( (cpx *)s->s_hat, 3 * N, s->padded, -1 );
+
+    const cpx * Sh = (const cpx *)s->s_hat;
+    const cpx * Dh = (const cpx *)s->d_hat;
+    cpx * R        = (cpx *)s->res_hat;
+#pragma omp parallel for
+    for( long k = 0; k < M; ++k )
+    {
+        for( int b1 = 0; b1 < N; ++b1 )
+        {
+            cpx acc[3] = { { 0, 0 }, { 0, 0 }, { 0, 0 } };
+            for( int b2 = 0; b2 < N; ++b2 )
+            {
+                int bi         = s->inter_lookup[b1 + b2 * N];
+                const cpx * d  = Dh + (long)bi * 6 * M + k;
+                const cpx sx = Sh[( (long)b2 * 3 + 0 ) * M + k];
+                const cpx sy = Sh[( (long)b2 * 3 + 1 ) * M + k];
+                const cpx sz = Sh[( (long)b2 * 3 + 2 ) * M + k];
+                /* rows (xx,xy,xz),(xy,yy,yz),(xz,yz,zz), FFT.hpp:89-96 */
+                const int rows[3][3] = { { 0, 1, 2 }, { 1, 3, 4 }, { 2, 4, 5 } };
+                for( int q = 0; q < 3; ++q )
+                {
+                    cpx d1 = d[rows[q][0] * M], d2 = d[rows[q][1] * M], d3 = d[rows[q][2] * M];
+                    double re = d1.re * sx.re - d1.im * sx.im + d2.re * sy.re - d2.im * sy.im + d3.re * sz.re
+                                - d3.im * sz.im;
+                    double im = d1.re * sx.im + d1.im * sx.re + d2.re * sy.im + d2.im * sy.re + d3.re * sz.im
+                                + d3.im * sz.re;
+                    if( b2 == 0 )
+                    {
+                        acc[q].re = re;
+                        acc[q].im = im;
+                    }
+                    else
+                    {
+                        acc[q].re += re;
+                        acc[q].im += im;
+                    }
+                }
+            }
+            for( int q = 0; q < 3; ++q )
+                R[( (long)b1 * 3 + q ) * M + k] = acc[q];
+        }
+    }
+    fft3d( R, 3 * N, s->padded, +1 );
+
+#pragma omp parallel for collapse( 2 )
+    for( int c = 0; c < Nc; ++c )
+        for( int b = 0; b < Nb; ++b )
+            for( int a = 0; a < Na; ++a )
+                for( int b1 = 0; b1 < N; ++b1 )
+                {
+                    long io  = b1 + (long)N * ( a + (long)Na * ( b + (long)Nb * c ) );
+                    long idx = a + (long)Pa * ( b + (long)Pb * c );
+                    for( int q = 0; q < 3; ++q )
+                        grad[3 * io + q] -= s->mu_s[io] * R[( (long)b1 * 3 + q ) * M + idx].re / (double)M;
+                }
+}
+
+/* Gradient_DDI_Direct, Hamiltonian_Heisenberg.cpp:1016-1071 */
+static void gradient_ddi_direct( orc_sys * s, const double * spins, double * grad )
+{
+    const orc_params * p = &s->p;
+    const double mult    = ddi_mult();
+    int img_a = p->bc[0] == 0 ? 0 : p->ddi_n_periodic_images[0];
+    int img_b = p->bc[1] == 0 ? 0 : p->ddi_n_periodic_images[1];
+    int img_c = p->bc[2] == 0 ? 0 : p->ddi_n_periodic_images[2];
+    int nos   = s->nos;
+#pragma omp parallel for
+    for( int i1 = 0; i1 < nos; i1++ )
+    {
+        for( int i2 = 0; i2 < nos; i2++ )
+        {
+            const double * m2 = spins + 3 * i2;
+            double diff[3];
+            for( int q = 0; q < 3; ++q )
+                diff[q] = s->positions[3 * i2 + q] - s->positions[3 * i1 + q];
+            double Dxx = 0, Dxy = 0, Dxz = 0, Dyy = 0, Dyz = 0, Dzz = 0;
+            for( int a_pb = -img_a; a_pb <= img_a; a_pb++ )
+                for( int b_pb = -img_b; b_pb <= img_b; b_pb++ )
+                    for( int c_pb = -img_c; c_pb <= img_c; c_pb++ )
+                    {
+                        double di[3];
+                        for( int q = 0; q < 3; ++q )
+                            di[q] = diff[q] + a_pb * p->n_cells[0] * p->bravais[0][q] * p->lattice_constant
+                                    + b_pb * p->n_cells[1] * p->bravais[1][q] * p->lattice_constant
+                                    + c_pb * p->n_cells[2] * p->bravais[2][q] * p->lattice_constant;
+                        double d = sqrt( v3_sqnorm( di ) );
+                        if( d > 1e-10 )
+                        {
+                            double d3 = d * d * d;
+                            double d5 = d * d * d * d * d;
+                            Dxx += mult * ( 3 * di[0] * di[0] / d5 - 1 / d3 );
+                            Dxy += mult * 3 * di[0] * di[1] / d5;
+                            Dxz += mult * 3 * di[0] * di[2] / d5;
+                            Dyy += mult * ( 3 * di[1] * di[1] / d5 - 1 / d3 );
+                            Dyz += mult * 3 * di[1] * di[2] / d5;
+                            Dzz += mult * ( 3 * di[2] * di[2] / d5 - 1 / d3 );
+                        }
+                    }
+            double mm = s->mu_s[i1] * s->mu_s[i2];
+            grad[3 * i1 + 0] -= ( Dxx * m2[0] + Dxy * m2[1] + Dxz * m2[2] ) * mm;
+            grad[3 * i1 + 1] -= ( Dxy * m2[0] + Dyy * m2[1] + Dyz * m2[2] ) * mm;
+            grad[3 * i1 + 2] -= ( Dxz * m2[0] + Dyz * m2[1] + Dzz * m2[2] ) * mm;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* system                                                              */
+/* ------------------------------------------------------------------ */
+orc_sys * orc_create( const orc_params * p )
+{
+    orc_sys * s = (orc_sys *)calloc( 1, sizeof( orc_sys ) );
+    s->p        = *p;
+    s->nos      = p->n_cell_atoms * p->n_cells[0] * p->n_cells[1] * p->n_cells[2];
+    s->n_cells_total = p->n_cells[0] * p->n_cells[1] * p->n_cells[2];
+    s->positions     = (double *)malloc( sizeof( double ) * 3 * (size_t)s->nos );
+    compute_positions( p, s->positions );
+    s->mu_s = (double *)malloc( sizeof( double ) * (size_t)s->nos );
+    for( int i = 0; i < s->nos; ++i )
+        s->mu_s[i] = p->mu_s[i % p->n_cell_atoms];
+    if( p->site_aniso_K )
+    {
+        s->site_K = (double *)malloc( sizeof( double ) * (size_t)s->nos );
+        memcpy( s->site_K, p->site_aniso_K, sizeof( double ) * (size_t)s->nos );
+    }
+    s->p.site_aniso_K = NULL;
+    if( p->ddi_method == ORC_DDI_FFT )
+        ddi_prepare( s );
+    size_t n3 = sizeof( double ) * 3 * (size_t)s->nos;
+    s->f1     = (double *)malloc( n3 );
+    s->fv1    = (double *)malloc( n3 );
+    s->f2     = (double *)malloc( n3 );
+    s->fv2    = (double *)malloc( n3 );
+    s->sp     = (double *)malloc( n3 );
+    s->tmp    = (double *)malloc( n3 );
+    s->xi     = (double *)calloc( 3 * (size_t)s->nos, sizeof( double ) );
+    s->k1     = (double *)malloc( n3 );
+    s->k2     = (double *)malloc( n3 );
+    s->k3     = (double *)malloc( n3 );
+    s->k4     = (double *)malloc( n3 );
+    s->g      = (double *)malloc( n3 );
+    return s;
+}
+
+void orc_destroy( orc_sys * s )
+{
+    if( !s )
+        return;
+    free( s->positions );
+    free( s->mu_s );
+    free( s->site_K );
+    free( s->d_hat );
+    free( s->s_hat );
+    free( s->res_hat );
+    free( s->f1 );
+    free( s->fv1 );
+    free( s->f2 );
+    free( s->fv2 );
+    free( s->sp );
+    free( s->tmp );
+    free( s->xi );
+    free( s->k1 );
+    free( s->k2 );
+    free( s->k3 );
+    free( s->k4 );
+    free( s->g );
+    free( s );
+}
+
+int orc_nos( const orc_sys * s )
+{
+    return s->nos;
+}
+
+void orc_positions( const orc_sys * s, double * out )
+{
+    memcpy( out, s->positions, sizeof( double ) * 3 * (size_t)s->nos );
+}
+
+/* Gradient_Zeeman, Hamiltonian_Heisenberg.cpp:768-783 */
+static void grad_zeeman( orc_sys * s, double * g )
+{
+    const double B = s->p.field_tesla * ORC_MU_B;
+#pragma omp parallel for
+    for( int i = 0; i < s->nos; ++i )
+        for( int q = 0; q < 3; ++q )
+            g[3 * i + q] -= s->mu_s[i] * B * s->p.field_normal[q];
+}
+
+/* Gradient_Anisotropy, :785-800 (+ per-site extension) */
+static void grad_aniso( orc_sys * s, const double * sp, double * g )
+{
+    const orc_params * p = &s->p;
+    int N                = p->n_cell_atoms;
+#pragma omp parallel for
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+    {
+        for( int ia = 0; ia < p->n_aniso; ++ia )
+        {
+            int i          = icell * N + p->aniso_idx[ia];
+            const double * n = p->aniso_normal[ia];
+            double d       = v3_dot( n, sp + 3 * i );
+            for( int q = 0; q < 3; ++q )
+                g[3 * i + q] -= 2.0 * p->aniso_K[ia] * n[q] * d;
+        }
+        if( s->site_K )
+        {
+            for( int b = 0; b < N; ++b )
+            {
+                int i          = icell * N + b;
+                const double * n = p->site_aniso_normal;
+                double d       = v3_dot( n, sp + 3 * i );
+                for( int q = 0; q < 3; ++q )
+                    g[3 * i + q] -= 2.0 * s->site_K[i] * n[q] * d;
+            }
+        }
+    }
+}
+
+/* Gradient_Cubic_Anisotropy, :802-820 */
+static void grad_cubic( orc_sys * s, const double * sp, double * g )
+{
+    const orc_params * p = &s->p;
+    int N                = p->n_cell_atoms;
+#pragma omp parallel for
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+        for( int ia = 0; ia < p->n_cubic; ++ia )
+        {
+            int i = icell * N + p->cubic_idx[ia];
+            for( int q = 0; q < 3; ++q )
+                g[3 * i + q] -= 2.0 * p->cubic_K[ia] * pow( sp[3 * i + q], 3.0 );
+        }
+}
+
+/* Gradient_Exchange, :822-842 (OpenMP gather form) */
+static void grad_exchange( orc_sys * s, const double * sp, double * g )
+{
+    const orc_params * p = &s->p;
+#pragma omp parallel for
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+        for( int k = 0; k < p->n_exch; ++k )
+        {
+            int i = p->exch[k][0] + icell * p->n_cell_atoms;
+            int j = orc_idx_from_pair( i, p->bc, p->n_cells, p->n_cell_atoms, p->exch[k] );
+            if( j >= 0 )
+                for( int q = 0; q < 3; ++q )
+                    g[3 * i + q] -= p->exch_J[k] * sp[3 * j + q];
+        }
+}
+
+/* Gradient_DMI, :844-864 (OpenMP gather form) */
+static void grad_dmi( orc_sys * s, const double * sp, double * g )
+{
+    const orc_params * p = &s->p;
+#pragma omp parallel for
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+        for( int k = 0; k < p->n_dmi; ++k )
+        {
+            int i = p->dmi[k][0] + icell * p->n_cell_atoms;
+            int j = orc_idx_from_pair( i, p->bc, p->n_cells, p->n_cell_atoms, p->dmi[k] );
+            if( j >= 0 )
+            {
+                double c[3];
+                v3_cross( sp + 3 * j, p->dmi_normal[k], c );
+                for( int q = 0; q < 3; ++q )
+                    g[3 * i + q] -= p->dmi_D[k] * c[q];
+            }
+        }
+}
+
+static void grad_ddi( orc_sys * s, const double * sp, double * g )
+{
+    if( s->p.ddi_method == ORC_DDI_FFT )
+        gradient_ddi_fft( s, sp, g );
+    else if( s->p.ddi_method == ORC_DDI_CUTOFF && s->p.ddi_radius < 0 )
+        gradient_ddi_direct( s, sp, g );
+}
+
+void orc_gradient_term( orc_sys * s, int term, const double * spins, double * grad )
+{
+    switch( term )
+    {
+        case 0: grad_zeeman( s, grad ); break;
+        case 1: grad_aniso( s, spins, grad ); break;
+        case 2: grad_cubic( s, spins, grad ); break;
+        case 3: grad_exchange( s, spins, grad ); break;
+        case 4: grad_dmi( s, spins, grad ); break;
+        case 5: grad_ddi( s, spins, grad ); break;
+        case 6: gradient_ddi_direct( s, spins, grad ); break;
+        default: break;
+    }
+}
+
+static int has_zeeman( const orc_sys * s )
+{
+    return fabs( s->p.field_tesla * ORC_MU_B ) > 1e-60;
+}
+
+/* Hamiltonian_Heisenberg.cpp:670-702 */
+void orc_gradient( orc_sys * s, const double * spins, double * grad )
+{
+    memset( grad, 0, sizeof( double ) * 3 * (size_t)s->nos );
+    if( has_zeeman( s ) )
+        grad_zeeman( s, grad );
+    if( s->p.n_aniso > 0 || s->site_K )
+        grad_aniso( s, spins, grad );
+    if( s->p.n_cubic > 0 )
+        grad_cubic( s, spins, grad );
+    if( s->p.n_exch > 0 )
+        grad_exchange( s, spins, grad );
+    if( s->p.n_dmi > 0 )
+        grad_dmi( s, spins, grad );
+    if( s->p.ddi_method != ORC_DDI_NONE )
+        grad_ddi( s, spins, grad );
+}
+
+/* Hamiltonian_Heisenberg.cpp:704-766 */
+void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, double * energy )
+{
+    memset( grad, 0, sizeof( double ) * 3 * (size_t)s->nos );
+    double E = 0;
+    if( s->p.n_aniso > 0 || s->site_K )
+        grad_aniso( s, spins, grad );
+    if( s->p.n_exch > 0 )
+        grad_exchange( s, spins, grad );
+    if( s->p.n_dmi > 0 )
+        grad_dmi( s, spins, grad );
+    if( s->p.ddi_method != ORC_DDI_NONE )
+        grad_ddi( s, spins, grad );
+    double e1 = 0;
+#pragma omp parallel for reduction( + : e1 )
+    for( int i = 0; i < s->nos; ++i )
+        e1 += 0.5 * v3_dot( grad + 3 * i, spins + 3 * i );
+    E += e1;
+    if( s->p.n_cubic > 0 )
+    {
+        grad_cubic( s, spins, grad );
+        const orc_params * p = &s->p;
+        double ec            = 0;
+#pragma omp parallel for reduction( + : ec )
+        for( int icell = 0; icell < s->n_cells_total; ++icell )
+            for( int ia = 0; ia < p->n_cubic; ++ia )
+            {
+                int i         = icell * p->n_cell_atoms + p->cubic_idx[ia];
+                const double * v = spins + 3 * i;
+                ec -= p->cubic_K[ia] / 2 * ( pow( v[0], 4.0 ) + pow( v[1], 4.0 ) + pow( v[2], 4.0 ) );
+            }
+        E += ec;
+    }
+    if( has_zeeman( s ) )
+    {
+        double B = s->p.field_tesla * ORC_MU_B;
+        double ext[3];
+        for( int q = 0; q < 3; ++q )
+            ext[q] = s->p.field_normal[q] * B;
+        grad_zeeman( s, grad );
+        double ez = 0;
+#pragma omp parallel for reduction( + : ez )
+        for( int i = 0; i < s->nos; ++i )
+            ez += -s->mu_s[i] * v3_dot( ext, spins + 3 * i );
+        E += ez;
+    }
+    *energy = E;
+}
+
+/* ------------------------------------------------------------------ */
+/* RNG: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller              */
+/* ------------------------------------------------------------------ */
+static void philox4x32_10( uint32_t ctr[4], uint32_t key[2] )
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t k0 = key[0], k1 = key[1];
+    for( int r = 0; r < 10; ++r )
+    {
+        uint64_t p0 = (uint64_t)M0 * ctr[0];
+        uint64_t p1 = (uint64_t)M1 * ctr[2];
+        uint32_t hi0 = (uint32_t)( p0 >> 32 ), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)( p1 >> 32 ), lo1 = (uint32_t)p1;
+        uint32_t n0  = hi1 ^ ctr[1] ^ k0;
+        uint32_t n1  = lo1;
+        uint32_t n2  = hi0 ^ ctr[3] ^ k1;
+        uint32_t n3  = lo0;
+        ctr[0]       = n0;
+        ctr[1]       = n1;
+        ctr[2]       = n2;
+        ctr[3]       = n3;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+static inline double u53( uint32_t a, uint32_t b )
+{
+    return ( (double)( a >> 5 ) * 67108864.0 + (double)( b >> 6 ) ) * ( 1.0 / 9007199254740992.0 );
+}
+
+void orc_normal3( uint64_t seed, uint64_t step, uint64_t site, double out[3] )
+{
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)( seed >> 32 ) };
+    uint32_t c0[4]  = { (uint32_t)site, (uint32_t)( site >> 32 ), (uint32_t)step, (uint32_t)( step >> 32 ) };
+    uint32_t c1[4]  = { c0[0], c0[1] | 0x80000000u, c0[2], c0[3] };
+    philox4x32_10( c0, key );
+    philox4x32_10( c1, key );
+    double u1 = 1.0 - u53( c0[0], c0[1] );
+    double u2 = u53( c0[2], c0[3] );
+    double u3 = 1.0 - u53( c1[0], c1[1] );
+    double u4 = u53( c1[2], c1[3] );
+    double r1 = sqrt( -2.0 * log( u1 ) );
+    double r2 = sqrt( -2.0 * log( u3 ) );
+    out[0]    = r1 * cos( 2.0 * ORC_PI * u2 );
+    out[1]    = r1 * sin( 2.0 * ORC_PI * u2 );
+    out[2]    = r2 * cos( 2.0 * ORC_PI * u4 );
+}
+
+/* ------------------------------------------------------------------ */
+/* LLG                                                                 */
+/* ------------------------------------------------------------------ */
+void orc_rotate( const double v[3], const double axis[3], double angle, double out[3] )
+{
+    double c = cos( angle ), sn = sin( angle );
+    double cr[3];
+    v3_cross( axis, v, cr );
+    double d = v3_dot( axis, v );
+    double o[3];
+    for( int q = 0; q < 3; ++q )
+        o[q] = v[q] * c + cr[q] * sn + axis[q] * d * ( 1 - c );
+    out[0] = o[0], out[1] = o[1], out[2] = o[2];
+}
+
+void orc_sib_transform( const double s[3], const double f[3], double out[3] )
+{
+    double e1[3] = { s[0], s[1], s[2] };
+    double A[3]  = { 0.5 * f[0], 0.5 * f[1], 0.5 * f[2] };
+    double detAi = 1.0 / ( 1 + pow( sqrt( v3_sqnorm( A ) ), 2.0 ) );
+    double c[3], a2[3];
+    v3_cross( e1, A, c );
+    for( int q = 0; q < 3; ++q )
+        a2[q] = e1[q] - c[q];
+    double o0 = ( a2[0] * ( A[0] * A[0] + 1 ) + a2[1] * ( A[0] * A[1] - A[2] ) + a2[2] * ( A[0] * A[2] + A[1] ) ) * detAi;
+    double o1 = ( a2[0] * ( A[1] * A[0] + A[2] ) + a2[1] * ( A[1] * A[1] + 1 ) + a2[2] * ( A[1] * A[2] - A[0] ) ) * detAi;
+    double o2 = ( a2[0] * ( A[2] * A[0] - A[1] ) + a2[1] * ( A[2] * A[1] + A[0] ) + a2[2] * ( A[2] * A[2] + 1 ) ) * detAi;
+    out[0] = o0, out[1] = o1, out[2] = o2;
+}
+
+/* Method_LLG::Prepare_Thermal_Field, Method_LLG.cpp:66-110 (generator replaced, see header) */
+static void prepare_thermal( orc_sys * s, const orc_llg * par, uint64_t step, int local_step )
+{
+    if( !( par->temperature > 0 ) )
+        return;
+    double a   = par->damping;
+    double eps = sqrt( 2 * a * par->dt * ORC_GAMMA / ORC_MU_B * ORC_K_B ) / ( 1 + a * a );
+#pragma omp parallel for
+    for( int i = 0; i < s->nos; ++i )
+    {
+        double z[3];
+        if( par->noise )
+        {
+            const double * src = par->noise + ( (size_t)local_step * s->nos + i ) * 3;
+            z[0] = src[0], z[1] = src[1], z[2] = src[2];
+        }
+        else
+            orc_normal3( par->seed, step, (uint64_t)i, z );
+        double f = eps * sqrt( par->temperature / s->mu_s[i] );
+        for( int q = 0; q < 3; ++q )
+            s->xi[3 * i + q] = f * z[q];
+    }
+}
+
+/* Method_LLG::Calculate_Force, Method_LLG.cpp:113-129 */
+static double calc_force( orc_sys * s, const double * spins, double * force )
+{
+    double E;
+    orc_gradient_and_energy( s, spins, s->g, &E );
+    for( long t = 0; t < 3L * s->nos; ++t )
+        force[t] = -1 * s->g[t];
+    return E;
+}
+
+/* Method_LLG::Calculate_Force_Virtual, Method_LLG.cpp:132-226 (no STT) */
+static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, const double * spins, const double * F,
+                                double * Fv )
+{
+    double a = par->damping;
+    if( par->direct_minimization || solver == ORC_SOLVER_VP )
+    {
+        double dtg = par->dt * ORC_GAMMA / ORC_MU_B;
+#pragma omp parallel for
+        for( int i = 0; i < s->nos; ++i )
+        {
+            double c[3];
+            v3_cross( spins + 3 * i, F + 3 * i, c );
+            for( int q = 0; q < 3; ++q )
+                Fv[3 * i + q] = dtg * c[q];
+        }
+        return;
+    }
+    double dtg = par->dt * ORC_GAMMA / ORC_MU_B / ( 1 + a * a );
+    int thermal = par->temperature > 0;
+#pragma omp parallel for
+    for( int i = 0; i < s->nos; ++i )
+    {
+        double c[3], v[3];
+        for( int q = 0; q < 3; ++q )
+            v[q] = dtg * F[3 * i + q];
+        v3_cross( spins + 3 * i, F + 3 * i, c );
+        for( int q = 0; q < 3; ++q )
+            v[q] += dtg * a * c[q];
+        for( int q = 0; q < 3; ++q )
+            v[q] /= s->mu_s[i];
+        if( thermal )
+        {
+            const double * xi = s->xi + 3 * i;
+            for( int q = 0; q < 3; ++q )
+                v[q] += 1 * xi[q];
+            v3_cross( spins + 3 * i, xi, c );
+            for( int q = 0; q < 3; ++q )
+                v[q] += a * c[q];
+        }
+        for( int q = 0; q < 3; ++q )
+            Fv[3 * i + q] = v[q];
+    }
+}
+
+/* -(a x b) helper: out = -1 * a.cross(b) (set_c_cross(-1, a, b, out)) */
+static inline void neg_cross( const double * a, const double * b, double * o )
+{
+    double c[3];
+    v3_cross( a, b, c );
+    o[0] = -1 * c[0];
+    o[1] = -1 * c[1];
+    o[2] = -1 * c[2];
+}
+
+void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spins, int n_steps, uint64_t step0,
+                      double * energy, double * max_torque, double * eff_field )
+{
+    int nos = s->nos;
+    double E = 0;
+    for( int it = 0; it < n_steps; ++it )
+    {
+        prepare_thermal( s, par, step0 + (uint64_t)it, it );
+        if( solver == ORC_SOLVER_DEPONDT )
+        {
+            /* Solver_Depondt.hpp:29-77 */
+            E = calc_force( s, spins, s->f1 );
+            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                double axis[3] = { 1 * s->fv1[3 * i], 1 * s->fv1[3 * i + 1], 1 * s->fv1[3 * i + 2] };
+                double angle   = sqrt( v3_sqnorm( s->fv1 + 3 * i ) );
+                v3_normalize( axis );
+                orc_rotate( spins + 3 * i, axis, angle, s->sp + 3 * i );
+            }
+            E = calc_force( s, s->sp, s->f2 );
+            calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                double t[3];
+                for( int q = 0; q < 3; ++q )
+                    t[q] = 0.5 * s->fv1[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 0.5 * s->fv2[3 * i + q];
+                double angle = sqrt( v3_sqnorm( t ) );
+                v3_normalize( t );
+                orc_rotate( spins + 3 * i, t, angle, spins + 3 * i );
+            }
+        }
+        else if( solver == ORC_SOLVER_HEUN )
+        {
+            /* Solver_Heun.hpp:30-83 */
+            E = calc_force( s, spins, s->f1 );
+            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                double * t  = s->tmp + 3 * i;
+                double * pr = s->sp + 3 * i;
+                neg_cross( spins + 3 * i, s->fv1 + 3 * i, t );
+                for( int q = 0; q < 3; ++q )
+                    pr[q] = 1 * spins[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    pr[q] += 1 * t[q];
+                v3_normalize( pr );
+            }
+            E = calc_force( s, s->sp, s->f2 );
+            calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                double * t = s->tmp + 3 * i;
+                double t1[3];
+                for( int q = 0; q < 3; ++q )
+                    t[q] *= 0.5;
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 1 * spins[3 * i + q];
+                neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, t1 );
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 0.5 * t1[q];
+                v3_normalize( t );
+                for( int q = 0; q < 3; ++q )
+                    spins[3 * i + q] = t[q];
+            }
+        }
+        else if( solver == ORC_SOLVER_SIB )
+        {
+            /* Solver_SIB.hpp:22-52 */
+            E = calc_force( s, spins, s->f1 );
+            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                double * pr = s->sp + 3 * i;
+                orc_sib_transform( spins + 3 * i, s->fv1 + 3 * i, pr );
+                for( int q = 0; q < 3; ++q )
+                    pr[q] += 1 * spins[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    pr[q] *= 0.5;
+            }
+            E = calc_force( s, s->sp, s->f2 );
+            calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+                orc_sib_transform( spins + 3 * i, s->fv2 + 3 * i, spins + 3 * i );
+        }
+        else if( solver == ORC_SOLVER_RK4 )
+        {
+            /* Solver_RK4.hpp:41-147 */
+            double * ks[4] = { s->k1, s->k2, s->k3, s->k4 };
+            const double cpred[3] = { 0.5, 0.5, 1.0 };
+            E = calc_force( s, spins, s->f1 );
+            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+            const double * conf = spins;
+            const double * fv   = s->fv1;
+            for( int stage = 0; stage < 3; ++stage )
+            {
+                double * k = ks[stage];
+#pragma omp parallel for
+                for( int i = 0; i < nos; ++i )
+                {
+                    neg_cross( conf + 3 * i, fv + 3 * i, k + 3 * i );
+                    double pr[3];
+                    for( int q = 0; q < 3; ++q )
+                        pr[q] = 1 * spins[3 * i + q];
+                    for( int q = 0; q < 3; ++q )
+                        pr[q] += cpred[stage] * k[3 * i + q];
+                    v3_normalize( pr );
+                    for( int q = 0; q < 3; ++q )
+                        s->tmp[3 * i + q] = pr[q];
+                }
+                memcpy( s->sp, s->tmp, sizeof( double ) * 3 * (size_t)nos );
+                E = calc_force( s, s->sp, s->f2 );
+                calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+                conf = s->sp;
+                fv   = s->fv2;
+            }
+#pragma omp parallel for
+            for( int i = 0; i < nos; ++i )
+            {
+                neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, s->k4 + 3 * i );
+                double t[3];
+                for( int q = 0; q < 3; ++q )
+                    t[q] = 1 * spins[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 1.0 / 6.0 * s->k1[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 1.0 / 3.0 * s->k2[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 1.0 / 3.0 * s->k3[3 * i + q];
+                for( int q = 0; q < 3; ++q )
+                    t[q] += 1.0 / 6.0 * s->k4[3 * i + q];
+                v3_normalize( t );
+                for( int q = 0; q < 3; ++q )
+                    spins[3 * i + q] = t[q];
+            }
+        }
+    }
+    if( energy )
+        *energy = E;
+    /* Hook_Post_Iteration, Method_LLG.cpp:247-301: torque of the first virtual force, projected on the new spins */
+    if( max_torque && n_steps > 0 )
+    {
+        double m = 0;
+#pragma omp parallel for reduction( max : m )
+        for( int i = 0; i < nos; ++i )
+        {
+            double * f = s->fv1 + 3 * i;
+            const double * v = spins + 3 * i;
+            double d       = v3_dot( f, v );
+            double pr[3];
+            for( int q = 0; q < 3; ++q )
+                pr[q] = f[q] - d * v[q];
+            double n2 = v3_sqnorm( pr );
+            if( n2 > m )
+                m = n2;
+        }
+        *max_torque = sqrt( m );
+    }
+    if( eff_field && n_steps > 0 )
+    {
+        for( int i = 0; i < nos; ++i )
+        {
+            const double * f = s->f1 + 3 * i;
+            const double * v = spins + 3 * i;
+            double d       = v3_dot( f, v );
+            for( int q = 0; q < 3; ++q )
+                eff_field[3 * i + q] = f[q] - d * v[q];
+        }
+    }
+}

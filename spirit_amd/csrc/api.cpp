@@ -1,0 +1,724 @@
+// The C API of the engine (libSpirit.so): the reference's exported functions
+// (core/include/Spirit/*.h) backed by the HIP runner.  Every entry point catches
+// all exceptions, logs them and returns a default value, as the reference does
+// (core/include/utility/Exception.hpp:97-127).
+#include "api_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <thread>
+
+using namespace spirit_hip;
+
+namespace spirit_hip
+{
+
+Image::Image( const Image & o )
+        : geometry( o.geometry ),
+          hamiltonian( std::make_shared<Hamiltonian>( *o.hamiltonian ) ),
+          llg( std::make_shared<LLGParameters>( *o.llg ) ),
+          prng( o.prng ),
+          spins( o.spins ),
+          eff_field( o.eff_field ),
+          E( o.E ),
+          max_torque( o.max_torque ),
+          Rx( o.Rx )
+{
+}
+
+void from_indices( State * state, int & idx_image, int & idx_chain, std::shared_ptr<Image> & image,
+                   std::shared_ptr<Chain> & chain )
+{
+    if( !state || !state->chain )
+        throw api_error( 2, "The State pointer is invalid" );
+    idx_chain = 0;
+    chain     = state->chain;
+    if( idx_image >= chain->noi() )
+        throw api_error( 3, "Index " + std::to_string( idx_image ) + " points to non-existent image (NOI="
+                                + std::to_string( chain->noi() ) + "). No action taken." );
+    if( idx_image < 0 )
+    {
+        image     = state->active_image;
+        idx_image = state->idx_active_image;
+    }
+    else
+        image = chain->images[idx_image];
+}
+
+void handle_exception( int idx_image, int idx_chain )
+{
+    try
+    {
+        throw;
+    }
+    catch( const api_error & e )
+    {
+        log_message( e.level, e.what() );
+    }
+    catch( const std::exception & e )
+    {
+        log_message( 2, std::string( "API call failed (image " ) + std::to_string( idx_image ) + ", chain "
+                            + std::to_string( idx_chain ) + "): " + e.what() );
+    }
+    catch( ... )
+    {
+        log_message( 2, "API call failed with an unknown exception" );
+    }
+}
+
+std::shared_ptr<Image> make_image( const std::string & cfg )
+{
+    auto img         = std::make_shared<Image>();
+    img->geometry    = geometry_from_config( cfg );
+    img->hamiltonian = hamiltonian_from_config( cfg, *img->geometry );
+    img->llg         = std::make_shared<LLGParameters>( llg_parameters_from_config( cfg ) );
+    img->prng        = std::mt19937( (unsigned)img->llg->seed );
+    img->spins.assign( 3 * (size_t)img->geometry->nos, 0.0 );
+    img->eff_field.assign( 3 * (size_t)img->geometry->nos, 0.0 );
+    for( int i = 0; i < img->geometry->nos; ++i )
+        img->spins[3 * i + 2] = 1.0;
+    return img;
+}
+
+// Bring the host mirror of an image up to date with a single-shot method's device spins.
+void sync_image_host( State * state, int idx_image )
+{
+    auto it = state->method_image.find( idx_image );
+    if( it == state->method_image.end() || !it->second )
+        return;
+    auto & m = *it->second;
+    if( m.runner && m.device_newer )
+    {
+        m.runner->download_spins();
+        m.runner->download_effective_field( 0, m.image->eff_field.data() );
+        m.device_newer = false;
+    }
+}
+
+void sync_all_hosts( State * state )
+{
+    for( auto & kv : state->method_image )
+        sync_image_host( state, kv.first );
+    gneb_sync_host( state );
+}
+
+// Parameters or spins changed on the host while a single-shot method holds device copies.
+void notify_image_changed( State * state, int idx_image, bool spins_changed, bool ham_changed, bool llg_changed )
+{
+    auto it = state->method_image.find( idx_image );
+    if( it == state->method_image.end() || !it->second || !it->second->runner )
+        return;
+    auto & m = *it->second;
+    if( ham_changed )
+        m.runner->refresh_hamiltonian();
+    if( llg_changed )
+        m.runner->set_llg( 0, *m.image->llg );
+    if( spins_changed )
+    {
+        m.runner->upload_spins();
+        m.device_newer = false;
+    }
+}
+
+bool stop_file_present()
+{
+    std::ifstream f( "STOP" );
+    return f.good();
+}
+
+} // namespace spirit_hip
+
+// ===========================================================================
+// State.h
+// ===========================================================================
+extern "C" State * State_Setup( const char * config_file, bool quiet ) noexcept
+try
+{
+    std::string cfg = config_file ? config_file : "";
+    g_quiet         = quiet;
+    if( !cfg.empty() )
+    {
+        ConfigFile f( cfg );
+        if( f.ok() )
+        {
+            int lvl = g_log_console_level;
+            if( f.read( lvl, "log_console_level" ) )
+                g_log_console_level = lvl;
+        }
+        else
+            log_message( 2, "Could not open config file \"" + cfg + "\"; using defaults" );
+    }
+    auto * state       = new State();
+    state->config_file = cfg;
+    state->quiet       = quiet;
+    {
+        std::time_t t = std::time( nullptr );
+        char buf[64];
+        std::strftime( buf, sizeof( buf ), "%Y-%m-%d_%H-%M-%S", std::localtime( &t ) );
+        state->datetime = buf;
+    }
+    state->chain       = std::make_shared<Chain>();
+    state->chain->gneb = std::make_shared<GNEBParameters>( gneb_parameters_from_config( cfg ) );
+    auto img           = make_image( cfg );
+    // Configurations::Random would need the image's prng; the reference sets a random state here
+    // (State.cpp:125-130) -- the setup keeps +z and leaves the choice to the caller.
+    state->chain->images.push_back( img );
+    state->chain->image_type.push_back( GNEB_NORMAL );
+    state->chain->Rx.assign( 1, 0.0 );
+    state->active_image     = img;
+    state->idx_active_image = 0;
+    state->nos              = img->nos();
+    state->noi              = 1;
+    state->clipboard_image  = nullptr;
+    // State_Setup seeds a random configuration, State.cpp:125-130
+    {
+        std::uniform_real_distribution<double> dist( -1, 1 );
+        for( int i = 0; i < img->nos(); ++i )
+        {
+            double vz  = dist( img->prng );
+            double phi = dist( img->prng ) * Pi;
+            double rxy = std::sqrt( 1 - vz * vz );
+            img->spins[3 * i]     = rxy * std::cos( phi );
+            img->spins[3 * i + 1] = rxy * std::sin( phi );
+            img->spins[3 * i + 2] = vz;
+        }
+    }
+    return state;
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+    return nullptr;
+}
+
+extern "C" void State_Delete( State * state ) noexcept
+try
+{
+    if( !state )
+        return;
+    state->method_image.clear();
+    state->method_chain.reset();
+    delete state;
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+}
+
+extern "C" void State_Update( State * state ) noexcept
+try
+{
+    if( !state )
+        return;
+    auto & c = *state->chain;
+    state->noi = c.noi();
+    if( state->idx_active_image >= c.noi() )
+        state->idx_active_image = c.noi() - 1;
+    state->active_image = c.images[state->idx_active_image];
+    state->nos          = state->active_image->nos();
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+}
+
+extern "C" void State_To_Config( State * state, const char * config_file, const char * comment ) noexcept
+try
+{
+    auto & img = *state->active_image;
+    auto & g   = *img.geometry;
+    auto & h   = *img.hamiltonian;
+    std::ofstream f( config_file );
+    if( comment && *comment )
+        f << "# " << comment << "\n";
+    f.precision( 17 );
+    f << "bravais_vectors\n";
+    for( int k = 0; k < 3; ++k )
+        f << g.bravais[k][0] << " " << g.bravais[k][1] << " " << g.bravais[k][2] << "\n";
+    f << "lattice_constant " << g.lattice_constant << "\n";
+    f << "n_basis_cells " << g.n_cells[0] << " " << g.n_cells[1] << " " << g.n_cells[2] << "\n";
+    f << "basis\n" << g.nb << "\n";
+    for( auto & a : g.cell_atoms )
+        f << a[0] << " " << a[1] << " " << a[2] << "\n";
+    f << "mu_s";
+    for( double m : g.mu_basis )
+        f << " " << m;
+    f << "\nboundary_conditions " << h.bc[0] << " " << h.bc[1] << " " << h.bc[2] << "\n";
+    f << "external_field_magnitude " << h.field_magnitude / mu_B << "\n";
+    f << "external_field_normal " << h.field_normal[0] << " " << h.field_normal[1] << " " << h.field_normal[2]
+      << "\n";
+    if( !h.aniso_K.empty() )
+    {
+        f << "anisotropy_magnitude " << h.aniso_K[0] << "\n";
+        f << "anisotropy_normal " << h.aniso_n[0][0] << " " << h.aniso_n[0][1] << " " << h.aniso_n[0][2] << "\n";
+    }
+    if( !h.cubic_K.empty() )
+        f << "cubic_anisotropy_magnitude " << h.cubic_K[0] << "\n";
+    f << "hamiltonian heisenberg_neighbours\n";
+    f << "n_shells_exchange " << h.exchange_shell_J.size() << "\njij";
+    for( double v : h.exchange_shell_J )
+        f << " " << v;
+    f << "\nn_shells_dmi " << h.dmi_shell_D.size() << "\ndij";
+    for( double v : h.dmi_shell_D )
+        f << " " << v;
+    f << "\ndm_chirality " << h.dmi_chirality << "\n";
+    const char * dm[] = { "none", "fft", "fmm", "cutoff" };
+    f << "ddi_method " << dm[std::max( 0, std::min( 3, h.ddi_method ) )] << "\n";
+    f << "ddi_n_periodic_images " << h.ddi_images[0] << " " << h.ddi_images[1] << " " << h.ddi_images[2] << "\n";
+    f << "ddi_pb_zero_padding " << ( h.ddi_pb_zero_padding ? 1 : 0 ) << "\n";
+    auto & p = *img.llg;
+    f << "llg_seed " << p.seed << "\nllg_dt " << p.dt << "\nllg_damping " << p.damping << "\nllg_temperature "
+      << p.temperature << "\nllg_force_convergence " << p.force_convergence << "\nllg_n_iterations "
+      << p.n_iterations << "\nllg_n_iterations_log " << p.n_iterations_log << "\n";
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+}
+
+extern "C" const char * State_DateTime( State * state ) noexcept
+try
+{
+    return state->datetime.c_str();
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+    return "";
+}
+
+// ===========================================================================
+// Simulation.h
+// ===========================================================================
+extern "C" void free_run_info( Simulation_Run_Info info ) noexcept
+{
+    delete[] info.history_energy;
+    delete[] info.history_iteration;
+    delete[] info.history_max_torque;
+}
+
+static const char * solver_name( int s )
+{
+    switch( s )
+    {
+        case 0: return "VP";
+        case 1: return "SIB";
+        case 2: return "Depondt";
+        case 3: return "Heun";
+        case 4: return "RK4";
+        case 5: return "LBFGS_OSO";
+        case 6: return "LBFGS_Atlas";
+        case 7: return "VP_OSO";
+        default: return "unknown";
+    }
+}
+
+// Finish a method: copy results to the host mirror and release the device copy.
+static void llg_finalize( MethodLLG & m )
+{
+    if( m.runner )
+    {
+        m.runner->download_spins();
+        m.runner->download_effective_field( 0, m.image->eff_field.data() );
+        m.runner.reset();
+    }
+    m.device_newer                = false;
+    m.image->iteration_allowed    = false;
+    m.image->singleshot_allowed   = false;
+}
+
+static bool llg_continue( const MethodLLG & m )
+{
+    return m.iteration < m.n_iterations && m.image->iteration_allowed && !stop_file_present()
+           && !( m.image->max_torque < m.image->llg->force_convergence );
+}
+
+static bool walltime_expired( const MethodLLG & m )
+{
+    if( m.image->llg->max_walltime_sec <= 0 )
+        return false;
+    double s = std::chrono::duration<double>( std::chrono::steady_clock::now() - m.t_start ).count();
+    return s > (double)m.image->llg->max_walltime_sec;
+}
+
+extern "C" void Simulation_LLG_Start( State * state, int solver_type, int n_iterations, int n_iterations_log,
+                                      bool singleshot, Simulation_Run_Info * info, int idx_image,
+                                      int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( image->iteration_allowed || chain->iteration_allowed )
+        throw api_error( 3, "Tried to use Simulation_Start on image " + std::to_string( idx_image ) + " of chain "
+                                + std::to_string( idx_chain ) + ", but there is already a simulation running." );
+    auto m = std::make_shared<MethodLLG>();
+    {
+        std::lock_guard<std::recursive_mutex> lk( image->mtx );
+        image->iteration_allowed  = true;
+        image->singleshot_allowed = singleshot;
+        if( n_iterations > 0 )
+            image->llg->n_iterations = n_iterations;
+        if( n_iterations_log > 0 )
+            image->llg->n_iterations_log = n_iterations_log;
+        m->image            = image;
+        m->solver           = solver_type;
+        m->solver_name      = solver_name( solver_type );
+        m->n_iterations     = std::max( 1L, image->llg->n_iterations );
+        m->n_iterations_log = std::min( image->llg->n_iterations_log, m->n_iterations );
+        if( m->n_iterations_log <= 0 )
+            m->n_iterations_log = m->n_iterations;
+        m->amortize = std::max( 1L, std::min( image->llg->n_iterations_amortize, m->n_iterations_log ) );
+        try
+        {
+            ImageSpec sp{ image->geometry, image->hamiltonian, *image->llg, image->spins.data() };
+            m->runner.reset( new LLGRunner( { sp }, solver_type ) );
+        }
+        catch( ... )
+        {
+            image->iteration_allowed  = false;
+            image->singleshot_allowed = false;
+            throw;
+        }
+        // constructor force evaluation and Hook_Post_Iteration (Method_LLG.cpp:22-63)
+        m->runner->initial_observe();
+        image->E          = m->runner->energy( 0 );
+        image->max_torque = m->runner->max_torque( 0 );
+        m->picoseconds    = image->llg->dt;
+        m->t_start        = std::chrono::steady_clock::now();
+    }
+    state->method_image[idx_image] = m;
+    if( singleshot )
+        return;
+
+    // Method::Iterate, Method.cpp:57-113: convergence is tested every n_iterations_amortize steps,
+    // on the device; the host checks the stop requests between batches.
+    m->iteration  = 0;
+    const long am = m->amortize;
+    while( llg_continue( *m ) && !walltime_expired( *m ) )
+    {
+        long outer_left = ( m->n_iterations - m->iteration + am - 1 ) / am;
+        long outer      = std::min( outer_left, std::max( 1L, 2000 / am ) );
+        auto t0         = std::chrono::steady_clock::now();
+        long done       = m->runner->run( outer * am, (int)am, true, image->llg->force_convergence );
+        auto t1         = std::chrono::steady_clock::now();
+        double sec      = std::chrono::duration<double>( t1 - t0 ).count();
+        if( sec > 0 )
+            m->ips = done / sec;
+        m->iteration += done;
+        m->picoseconds += done * image->llg->dt;
+        image->E          = m->runner->energy( 0 );
+        image->max_torque = m->runner->max_torque( 0 );
+    }
+    llg_finalize( *m );
+    if( info )
+    {
+        info->max_torque       = (float)image->max_torque;
+        info->total_iterations = (int)m->iteration;
+        double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - m->t_start ).count();
+        info->total_walltime = (int)ms;
+        info->total_ips      = ms > 0 ? float( m->iteration / ms * 1000.0 ) : 0.f;
+    }
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Simulation_MC_Start( State * state, int n_iterations, int n_iterations_log, bool singleshot,
+                                     Simulation_Run_Info * info, int idx_image, int idx_chain ) noexcept
+{
+    log_message( 2, "Simulation_MC_Start: Monte Carlo is not part of the HIP engine" );
+}
+
+extern "C" void Simulation_MMF_Start( State * state, int solver_type, int n_iterations, int n_iterations_log,
+                                      bool singleshot, Simulation_Run_Info * info, int idx_image,
+                                      int idx_chain ) noexcept
+{
+    log_message( 2, "Simulation_MMF_Start: minimum mode following is not part of the HIP engine" );
+}
+
+extern "C" void Simulation_EMA_Start( State * state, int n_iterations, int n_iterations_log, bool singleshot,
+                                      Simulation_Run_Info * info, int idx_image, int idx_chain ) noexcept
+{
+    log_message( 2, "Simulation_EMA_Start: eigenmode analysis is not part of the HIP engine" );
+}
+
+extern "C" void Simulation_N_Shot( State * state, int N, int idx_image, int idx_chain ) noexcept;
+
+extern "C" void Simulation_SingleShot( State * state, int idx_image, int idx_chain ) noexcept
+{
+    Simulation_N_Shot( state, 1, idx_image, idx_chain );
+}
+
+// Simulation.cpp:454-540
+extern "C" void Simulation_N_Shot( State * state, int N, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( chain->iteration_allowed && chain->singleshot_allowed )
+    {
+        gneb_n_shot( state, N );
+        return;
+    }
+    if( !( image->iteration_allowed && image->singleshot_allowed ) )
+        throw api_error( 3, "Tried to use Simulation_SingleShot on image " + std::to_string( idx_image )
+                                + " but no SingleShot simulation has been started." );
+    auto m = state->method_image[idx_image];
+    if( llg_continue( *m ) && !walltime_expired( *m ) )
+    {
+        std::lock_guard<std::recursive_mutex> lk( image->mtx );
+        long n = std::max( 0, N );
+        m->runner->run( n, 0, false, image->llg->force_convergence );
+        m->iteration += n;
+        m->picoseconds += n * image->llg->dt;
+        image->E          = m->runner->energy( 0 );
+        image->max_torque = m->runner->max_torque( 0 );
+        m->device_newer   = true;
+        if( state->host_sync )
+            sync_image_host( state, idx_image );
+    }
+    if( !llg_continue( *m ) || walltime_expired( *m ) )
+        llg_finalize( *m );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Simulation_Stop( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( image->iteration_allowed )
+    {
+        std::lock_guard<std::recursive_mutex> lk( image->mtx );
+        image->iteration_allowed = false;
+        if( image->singleshot_allowed )
+        {
+            auto m = state->method_image[idx_image];
+            if( m )
+                llg_finalize( *m );
+        }
+    }
+    else if( chain->iteration_allowed )
+    {
+        chain->iteration_allowed = false;
+        if( chain->singleshot_allowed )
+            gneb_finalize( state );
+    }
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Simulation_Stop_All( State * state ) noexcept
+try
+{
+    for( int i = 0; i < state->chain->noi(); ++i )
+        Simulation_Stop( state, i, -1 );
+    Simulation_Stop( state, -1, -1 );
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+}
+
+extern "C" float Simulation_Get_MaxTorqueComponent( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( chain->iteration_allowed && state->method_chain )
+        return (float)state->method_chain->max_torque;
+    return (float)image->max_torque;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" float Simulation_Get_MaxTorqueNorm( State * state, int idx_image, int idx_chain ) noexcept
+{
+    return Simulation_Get_MaxTorqueComponent( state, idx_image, idx_chain );
+}
+
+extern "C" void Simulation_Get_Chain_MaxTorqueComponents( State * state, float * torques, int idx_chain ) noexcept
+try
+{
+    if( state->method_chain )
+        for( size_t i = 0; i < state->method_chain->max_torque_all.size(); ++i )
+            torques[i] = (float)state->method_chain->max_torque_all[i];
+}
+catch( ... )
+{
+    handle_exception( -1, idx_chain );
+}
+
+extern "C" void Simulation_Get_Chain_MaxTorqueNorms( State * state, float * torques, int idx_chain ) noexcept
+{
+    Simulation_Get_Chain_MaxTorqueComponents( state, torques, idx_chain );
+}
+
+extern "C" float Simulation_Get_IterationsPerSecond( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( chain->iteration_allowed && state->method_chain )
+        return (float)state->method_chain->ips;
+    auto it = state->method_image.find( idx_image );
+    if( it != state->method_image.end() && it->second )
+        return (float)it->second->ips;
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" int Simulation_Get_Iteration( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( state->method_chain && ( chain->iteration_allowed || !state->method_image.count( idx_image ) ) )
+        return (int)state->method_chain->iteration;
+    auto it = state->method_image.find( idx_image );
+    if( it != state->method_image.end() && it->second )
+        return (int)it->second->iteration;
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" float Simulation_Get_Time( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    auto it = state->method_image.find( idx_image );
+    if( it != state->method_image.end() && it->second )
+        return (float)it->second->picoseconds;
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" int Simulation_Get_Wall_Time( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    auto it = state->method_image.find( idx_image );
+    if( it != state->method_image.end() && it->second )
+        return (int)std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now()
+                                                               - it->second->t_start )
+            .count();
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" const char * Simulation_Get_Solver_Name( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( chain->iteration_allowed && state->method_chain )
+        return state->method_chain->solver_name.c_str();
+    auto it = state->method_image.find( idx_image );
+    if( it != state->method_image.end() && it->second && image->iteration_allowed )
+        return it->second->solver_name.c_str();
+    return "";
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return "";
+}
+
+extern "C" const char * Simulation_Get_Method_Name( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( chain->iteration_allowed )
+        return "GNEB";
+    if( image->iteration_allowed )
+        return "LLG";
+    return "None";
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return "";
+}
+
+extern "C" bool Simulation_Running_On_Image( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    return image->iteration_allowed;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return false;
+}
+
+extern "C" bool Simulation_Running_On_Chain( State * state, int idx_chain ) noexcept
+try
+{
+    return state->chain->iteration_allowed;
+}
+catch( ... )
+{
+    handle_exception( -1, idx_chain );
+    return false;
+}
+
+extern "C" bool Simulation_Running_Anywhere_On_Chain( State * state, int idx_chain ) noexcept
+try
+{
+    if( state->chain->iteration_allowed )
+        return true;
+    for( auto & im : state->chain->images )
+        if( im->iteration_allowed )
+            return true;
+    return false;
+}
+catch( ... )
+{
+    handle_exception( -1, idx_chain );
+    return false;
+}

@@ -1,0 +1,102 @@
+// Plain-old-data structures shared between the host engine and the HIP kernels.
+// Spin-like fields live in HBM as structure-of-arrays per image:
+//   field[q * nos + ispin], q = 0,1,2 (x,y,z), ispin = ib + nb*(a + Na*(b + Nb*c))
+// (the reference's index convention, core/src/data/Geometry.cpp:150-157).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spirit_hip
+{
+
+constexpr int MAX_BASIS      = 8;
+constexpr int MAX_PAIRS      = 64;
+constexpr int MAX_FFT_STAGES = 16;
+
+// Physical constants, core/include/utility/Constants.hpp:18-40
+constexpr double mu_B  = 0.057883817555;
+constexpr double mu_0  = 2.0133545 * 1e-28;
+constexpr double k_B   = 0.08617330350;
+constexpr double gamma_e = 0.1760859644; // gyromagnetic ratio of the electron [rad/(ps*T)]
+constexpr double Pi    = 3.141592653589793238462643383279502884197169399375105820974;
+
+enum SolverId
+{
+    SOLVER_VP          = 0,
+    SOLVER_SIB         = 1,
+    SOLVER_DEPONDT     = 2,
+    SOLVER_HEUN        = 3,
+    SOLVER_RK4         = 4,
+    SOLVER_LBFGS_OSO   = 5,
+    SOLVER_LBFGS_ATLAS = 6,
+    SOLVER_VP_OSO      = 7,
+};
+
+// A neighbour pair (i, j, translation) with its exchange constant J, or its DM
+// magnitude D and unit DM vector n (Vectormath_Defines.hpp:98-107).
+struct PairDev
+{
+    int i, j, da, db, dc, pad;
+    double J;
+    double D;
+    double n[3];
+};
+
+// Everything the fused local-field kernel needs for one image.
+struct HamDev
+{
+    int Na, Nb, Nc, nb; // cells and basis atoms
+    int nos;
+    int bc[3];
+    int n_exch, n_dmi, n_aniso, n_cubic;
+    int has_zeeman;
+    int has_ddi;
+    int has_site_K;
+    int pad0;
+    double B;                // external_field_magnitude * mu_B (Hamiltonian_Heisenberg.cpp:35)
+    double Bn[3];            // external_field_normal
+    double mu[MAX_BASIS];    // mu_s per basis atom
+    int aniso_idx[MAX_BASIS];
+    double aniso_K[MAX_BASIS];
+    double aniso_n[MAX_BASIS][3];
+    int cubic_idx[MAX_BASIS];
+    double cubic_K[MAX_BASIS];
+    double site_n[3];
+    const double * site_K;   // per-site uniaxial K_i (nullable)
+    const PairDev * exch;    // n_exch redundant exchange pairs
+    const PairDev * dmi;     // n_dmi redundant DMI pairs
+};
+
+// LLG parameters of one image (Method_LLG.cpp:132-226, 66-110).
+struct LLGDev
+{
+    double dt;
+    double damping;
+    double temperature;
+    int direct_minimization;
+    int thermal; // temperature > 0
+    uint64_t seed;
+};
+
+// One image of a batched launch: where its fields live and how it is parameterised.
+struct ImageDev
+{
+    HamDev ham;
+    LLGDev llg;
+    long long off;      // spin offset of this image inside the batched SoA buffers (fields at 3*off)
+    int block_begin;    // first block of this image in the launch
+    int n_blocks;       // blocks of this image
+};
+
+// One FFT length: Stockham stage radices and the n-th roots of unity exp(-2 pi i e / n).
+struct FftLen
+{
+    int n;
+    int n_stages;
+    int radix[MAX_FFT_STAGES];
+    const double2 * W;
+};
+
+} // namespace spirit_hip

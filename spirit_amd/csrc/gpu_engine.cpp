@@ -1,0 +1,581 @@
+// Device-resident LLG runner and DDI plan.
+#include "gpu_engine.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace spirit_hip
+{
+
+void hip_check( hipError_t e, const char * what )
+{
+    if( e != hipSuccess )
+        throw std::runtime_error( std::string( "HIP error " ) + hipGetErrorString( e ) + " in " + what );
+}
+
+DeviceBuffer::~DeviceBuffer()
+{
+    free();
+}
+
+void DeviceBuffer::free()
+{
+    if( p )
+        (void)hipFree( p );
+    p     = nullptr;
+    bytes = 0;
+}
+
+void DeviceBuffer::resize( size_t n )
+{
+    if( n == bytes && p )
+        return;
+    free();
+    if( n == 0 )
+        return;
+    HIPCHECK( hipMalloc( &p, n ) );
+    bytes = n;
+}
+
+// ---------------------------------------------------------------------------
+// FFT lengths
+// ---------------------------------------------------------------------------
+void FftLenHost::build( int n )
+{
+    if( n < 1 || n > 4096 )
+        throw std::runtime_error( "DDI FFT: padded length " + std::to_string( n ) + " outside [1, 4096]" );
+    dev.n        = n;
+    dev.n_stages = 0;
+    int m        = n;
+    auto push    = [&]( int r ) {
+        if( dev.n_stages >= MAX_FFT_STAGES )
+            throw std::runtime_error( "DDI FFT: too many stages" );
+        dev.radix[dev.n_stages++] = r;
+        m /= r;
+    };
+    while( m % 8 == 0 )
+        push( 8 );
+    while( m % 4 == 0 )
+        push( 4 );
+    while( m % 2 == 0 )
+        push( 2 );
+    for( int r : { 3, 5, 7, 11, 13 } )
+        while( m % r == 0 )
+            push( r );
+    if( m != 1 )
+        throw std::runtime_error( "DDI FFT: padded length " + std::to_string( n )
+                                  + " has a prime factor above 13 (supported radices 2,3,4,5,7,8,11,13)" );
+    std::vector<double2> w( n );
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    for( int e = 0; e < n; ++e )
+    {
+        long double a = -twopi * (long double)e / (long double)n;
+        w[e]          = make_double2( (double)cosl( a ), (double)sinl( a ) );
+    }
+    W.resize( sizeof( double2 ) * n );
+    HIPCHECK( hipMemcpy( W.as<void>(), w.data(), sizeof( double2 ) * n, hipMemcpyHostToDevice ) );
+    dev.W = W.as<double2>();
+}
+
+std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h )
+{
+    std::array<int, 3> p;
+    for( int i = 0; i < 3; ++i )
+    {
+        p[i]     = g.n_cells[i];
+        bool pad = g.n_cells[i] > 1 && ( h.ddi_bc[i] == 0 || h.ddi_pb_zero_padding );
+        if( pad )
+            p[i] *= 2;
+    }
+    return p;
+}
+
+size_t DdiPlan::bytes() const
+{
+    return X.size() + Dh.size();
+}
+
+void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
+{
+    auto P = ddi_padded_dims( g, h );
+    if( g.nb > MAX_BASIS )
+        throw std::runtime_error( "DDI: too many basis atoms" );
+    La.build( P[0] );
+    Lb.build( P[1] );
+    Lc.build( P[2] );
+    d     = DdiDev{};
+    d.Na  = g.n_cells[0];
+    d.Nb  = g.n_cells[1];
+    d.Nc  = g.n_cells[2];
+    d.nb  = g.nb;
+    d.Pa  = P[0];
+    d.Pb  = P[1];
+    d.Pc  = P[2];
+    d.Ha  = P[0] / 2 + 1;
+    d.M   = (double)P[0] * (double)P[1] * (double)P[2];
+    d.La  = La.dev;
+    d.Lb  = Lb.dev;
+    d.Lc  = Lc.dev;
+    // inter_sublattice_lookup, Hamiltonian_Heisenberg.cpp:1415-1428 and 1545-1554
+    int b_inter = -1;
+    for( int b1 = 0; b1 < g.nb; ++b1 )
+        for( int b2 = 0; b2 < g.nb; ++b2 )
+        {
+            if( b1 == b2 && b1 != 0 )
+            {
+                d.inter[b1 + b2 * g.nb] = 0;
+                continue;
+            }
+            d.inter[b1 + b2 * g.nb] = ++b_inter;
+        }
+    d.n_inter = b_inter + 1;
+    for( int b = 0; b < g.nb; ++b )
+        d.mu[b] = g.mu_basis[b];
+
+    const size_t nX  = (size_t)3 * g.nb * d.Nc * (size_t)d.Pb * d.Ha;
+    const size_t nDh = (size_t)d.Pb * d.Ha * (size_t)d.Pc * 6 * d.n_inter;
+    X.resize( sizeof( double2 ) * nX );
+    Dh.resize( sizeof( double2 ) * nDh );
+    d.X  = X.as<double2>();
+    d.Dh = Dh.as<double2>();
+
+    DipoleArgs a{};
+    a.Na = d.Na, a.Nb = d.Nb, a.Nc = d.Nc, a.nb = d.nb;
+    a.Pa = d.Pa, a.Pb = d.Pb, a.Pc = d.Pc;
+    for( int i = 0; i < 3; ++i )
+        a.img[i] = h.ddi_bc[i] == 0 ? 0 : h.ddi_images[i];
+    a.lattice_constant = g.lattice_constant;
+    for( int k = 0; k < 3; ++k )
+        for( int c = 0; c < 3; ++c )
+            a.bravais[k][c] = g.bravais[k][c];
+    for( int b = 0; b < g.nb; ++b )
+        for( int c = 0; c < 3; ++c )
+            a.cell_atoms[b][c] = g.cell_atoms[b][c];
+
+    const size_t M = (size_t)d.Pa * d.Pb * d.Pc;
+    DeviceBuffer work( sizeof( double ) * 6 * d.n_inter * M );
+    DeviceBuffer work2( sizeof( double2 ) * 6 * d.n_inter * (size_t)d.Pc * d.Pb * d.Ha );
+    ddi_launch_kernel_spectrum( d, a, work.as<double>(), work2.as<double2>(), Dh.as<double2>(), st );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipStreamSynchronize( st ) );
+}
+
+void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st )
+{
+    ddi_launch_field( d, spins_soa, h_soa, nos, st );
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian upload
+// ---------------------------------------------------------------------------
+void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
+{
+    if( g.nb > MAX_BASIS )
+        throw std::runtime_error( "too many basis atoms (max " + std::to_string( MAX_BASIS ) + ")" );
+    if( (int)h.aniso_idx.size() > MAX_BASIS || (int)h.cubic_idx.size() > MAX_BASIS )
+        throw std::runtime_error( "too many anisotropy entries" );
+    HamDev H{};
+    H.Na  = g.n_cells[0];
+    H.Nb  = g.n_cells[1];
+    H.Nc  = g.n_cells[2];
+    H.nb  = g.nb;
+    H.nos = g.nos;
+    for( int c = 0; c < 3; ++c )
+        H.bc[c] = h.bc[c];
+    H.has_zeeman = h.has_zeeman() ? 1 : 0;
+    H.B          = h.field_magnitude;
+    for( int c = 0; c < 3; ++c )
+        H.Bn[c] = h.field_normal[c];
+    for( int b = 0; b < g.nb; ++b )
+        H.mu[b] = g.mu_basis[b];
+    H.n_aniso = (int)h.aniso_idx.size();
+    for( int k = 0; k < H.n_aniso; ++k )
+    {
+        H.aniso_idx[k] = h.aniso_idx[k];
+        H.aniso_K[k]   = h.aniso_K[k];
+        for( int c = 0; c < 3; ++c )
+            H.aniso_n[k][c] = h.aniso_n[k][c];
+    }
+    H.n_cubic = (int)h.cubic_idx.size();
+    for( int k = 0; k < H.n_cubic; ++k )
+    {
+        H.cubic_idx[k] = h.cubic_idx[k];
+        H.cubic_K[k]   = h.cubic_K[k];
+    }
+    std::vector<PairDev> pv;
+    for( size_t k = 0; k < h.exch_pairs.size(); ++k )
+    {
+        const Pair & p = h.exch_pairs[k];
+        PairDev q{};
+        q.i = p.i, q.j = p.j, q.da = p.t[0], q.db = p.t[1], q.dc = p.t[2];
+        q.J = h.exch_J[k];
+        pv.push_back( q );
+    }
+    for( size_t k = 0; k < h.dmi_pairs.size(); ++k )
+    {
+        const Pair & p = h.dmi_pairs[k];
+        PairDev q{};
+        q.i = p.i, q.j = p.j, q.da = p.t[0], q.db = p.t[1], q.dc = p.t[2];
+        q.D = h.dmi_D[k];
+        for( int c = 0; c < 3; ++c )
+            q.n[c] = h.dmi_n[k][c];
+        pv.push_back( q );
+    }
+    for( auto & q : pv )
+        if( q.i < 0 || q.i >= g.nb || q.j < 0 || q.j >= g.nb )
+            throw std::runtime_error( "pair references a basis atom that does not exist" );
+    H.n_exch = (int)h.exch_pairs.size();
+    H.n_dmi  = (int)h.dmi_pairs.size();
+    pairs.resize( sizeof( PairDev ) * std::max<size_t>( 1, pv.size() ) );
+    if( !pv.empty() )
+        HIPCHECK( hipMemcpy( pairs.as<void>(), pv.data(), sizeof( PairDev ) * pv.size(), hipMemcpyHostToDevice ) );
+    H.exch = pairs.as<PairDev>();
+    H.dmi  = pairs.as<PairDev>() + H.n_exch;
+    if( !h.site_K.empty() )
+    {
+        if( (int)h.site_K.size() != g.nos )
+            throw std::runtime_error( "per-site anisotropy has the wrong length" );
+        site_K.resize( sizeof( double ) * g.nos );
+        HIPCHECK( hipMemcpy( site_K.as<void>(), h.site_K.data(), sizeof( double ) * g.nos, hipMemcpyHostToDevice ) );
+        H.has_site_K = 1;
+        H.site_K     = site_K.as<double>();
+        for( int c = 0; c < 3; ++c )
+            H.site_n[c] = h.site_n[c];
+    }
+    H.has_ddi   = h.ddi_method == DDI_FFT ? 1 : 0;
+    dev         = H;
+    version     = h.version;
+    ddi_version = h.ddi_version;
+}
+
+// ---------------------------------------------------------------------------
+// LLG runner
+// ---------------------------------------------------------------------------
+static void aos_to_soa( const double * aos, double * soa, long long n )
+{
+    for( long long i = 0; i < n; ++i )
+    {
+        soa[i]         = aos[3 * i];
+        soa[n + i]     = aos[3 * i + 1];
+        soa[2 * n + i] = aos[3 * i + 2];
+    }
+}
+static void soa_to_aos( const double * soa, double * aos, long long n )
+{
+    for( long long i = 0; i < n; ++i )
+    {
+        aos[3 * i]     = soa[i];
+        aos[3 * i + 1] = soa[n + i];
+        aos[3 * i + 2] = soa[2 * n + i];
+    }
+}
+
+static LLGDev make_llg_dev( const LLGParameters & p )
+{
+    LLGDev L{};
+    L.dt                  = p.dt;
+    L.damping             = p.damping;
+    L.temperature         = p.temperature;
+    L.direct_minimization = p.direct_minimization ? 1 : 0;
+    L.thermal             = p.temperature > 0 ? 1 : 0;
+    L.seed                = p.seed;
+    return L;
+}
+
+LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : specs( images ), solver( solver_ )
+{
+    if( solver != SOLVER_DEPONDT && solver != SOLVER_HEUN && solver != SOLVER_SIB && solver != SOLVER_RK4 )
+        throw std::runtime_error( "LLG solver " + std::to_string( solver ) + " is not available in the HIP engine" );
+    if( specs.empty() )
+        throw std::runtime_error( "LLGRunner: no images" );
+    HIPCHECK( hipStreamCreateWithFlags( &stream, hipStreamNonBlocking ) );
+    long long off = 0;
+    n_blocks      = 0;
+    for( size_t k = 0; k < specs.size(); ++k )
+    {
+        auto & sp = specs[k];
+        hams.emplace_back( new HamUpload() );
+        hams.back()->upload( *sp.geometry, *sp.hamiltonian );
+        ImageDev I{};
+        I.ham         = hams.back()->dev;
+        I.llg         = make_llg_dev( sp.llg );
+        I.off         = off;
+        I.block_begin = n_blocks;
+        I.n_blocks    = ( sp.geometry->nos + LLG_THREADS - 1 ) / LLG_THREADS;
+        for( int b = 0; b < I.n_blocks; ++b )
+            blk_img_host.push_back( (int)k );
+        n_blocks += I.n_blocks;
+        off += sp.geometry->nos;
+        imgs_host.push_back( I );
+        if( sp.hamiltonian->ddi_method == DDI_FFT )
+        {
+            ddi.emplace_back( new DdiPlan() );
+            ddi.back()->build( *sp.geometry, *sp.hamiltonian, stream );
+            any_ddi = true;
+        }
+        else
+        {
+            if( sp.hamiltonian->ddi_method != DDI_NONE )
+                log_message( 3, "DDI method other than FFT is not available in the HIP engine; DDI ignored" );
+            ddi.emplace_back( nullptr );
+        }
+    }
+    n_total         = off;
+    const size_t f3 = sizeof( double ) * 3 * (size_t)n_total;
+    b_s.resize( f3 );
+    b_sp.resize( f3 );
+    if( solver == SOLVER_RK4 )
+    {
+        b_sp2.resize( f3 );
+        b_aux2.resize( f3 );
+    }
+    b_aux.resize( f3 );
+    b_eff.resize( f3 );
+    b_efft.resize( f3 );
+    if( any_ddi )
+    {
+        b_h.resize( f3 );
+        HIPCHECK( hipMemset( b_h.as<void>(), 0, f3 ) );
+    }
+    b_pE.resize( sizeof( double ) * n_blocks );
+    b_pT.resize( sizeof( double ) * n_blocks );
+    b_stop.resize( sizeof( int ) );
+    b_stopstep.resize( sizeof( long long ) );
+    b_obs.resize( sizeof( double ) * 2 * specs.size() );
+    HIPCHECK( hipMemset( b_obs.as<void>(), 0, sizeof( double ) * 2 * specs.size() ) );
+    HIPCHECK( hipMemset( b_eff.as<void>(), 0, f3 ) );
+    obs_host.assign( 2 * specs.size(), 0.0 );
+    b_blk.resize( sizeof( int ) * blk_img_host.size() );
+    HIPCHECK( hipMemcpy( b_blk.as<void>(), blk_img_host.data(), sizeof( int ) * blk_img_host.size(),
+                         hipMemcpyHostToDevice ) );
+    upload_images();
+    upload_spins();
+}
+
+LLGRunner::~LLGRunner()
+{
+    if( stream )
+    {
+        (void)hipStreamSynchronize( stream );
+        (void)hipStreamDestroy( stream );
+    }
+}
+
+void LLGRunner::upload_images()
+{
+    b_imgs.resize( sizeof( ImageDev ) * imgs_host.size() );
+    HIPCHECK( hipMemcpy( b_imgs.as<void>(), imgs_host.data(), sizeof( ImageDev ) * imgs_host.size(),
+                         hipMemcpyHostToDevice ) );
+}
+
+void LLGRunner::set_llg( int img, const LLGParameters & p )
+{
+    specs[img].llg      = p;
+    imgs_host[img].llg  = make_llg_dev( p );
+    upload_images();
+}
+
+void LLGRunner::refresh_hamiltonian()
+{
+    for( size_t k = 0; k < specs.size(); ++k )
+    {
+        const uint64_t old_ddi = hams[k]->ddi_version;
+        hams[k]->upload( *specs[k].geometry, *specs[k].hamiltonian );
+        imgs_host[k].ham = hams[k]->dev;
+        if( specs[k].hamiltonian->ddi_method == DDI_FFT )
+        {
+            if( !ddi[k] || old_ddi != specs[k].hamiltonian->ddi_version )
+            {
+                ddi[k].reset( new DdiPlan() );
+                ddi[k]->build( *specs[k].geometry, *specs[k].hamiltonian, stream );
+            }
+            if( !any_ddi )
+            {
+                any_ddi = true;
+                b_h.resize( sizeof( double ) * 3 * (size_t)n_total );
+                HIPCHECK( hipMemset( b_h.as<void>(), 0, b_h.size() ) );
+            }
+        }
+        else
+            ddi[k].reset();
+    }
+    upload_images();
+}
+
+void LLGRunner::upload_spins()
+{
+    std::vector<double> soa( 3 * (size_t)n_total );
+    for( size_t k = 0; k < specs.size(); ++k )
+        aos_to_soa( specs[k].spins_aos, soa.data() + 3 * imgs_host[k].off, specs[k].geometry->nos );
+    HIPCHECK( hipMemcpy( b_s.as<void>(), soa.data(), sizeof( double ) * soa.size(), hipMemcpyHostToDevice ) );
+}
+
+void LLGRunner::download_spins()
+{
+    std::vector<double> soa( 3 * (size_t)n_total );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    HIPCHECK( hipMemcpy( soa.data(), b_s.as<void>(), sizeof( double ) * soa.size(), hipMemcpyDeviceToHost ) );
+    for( size_t k = 0; k < specs.size(); ++k )
+        soa_to_aos( soa.data() + 3 * imgs_host[k].off, specs[k].spins_aos, specs[k].geometry->nos );
+}
+
+void LLGRunner::download_effective_field( int img, double * aos )
+{
+    const int nos = specs[img].geometry->nos;
+    std::vector<double> soa( 3 * (size_t)nos );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    HIPCHECK( hipMemcpy( soa.data(), b_eff.as<double>() + 3 * imgs_host[img].off, sizeof( double ) * soa.size(),
+                         hipMemcpyDeviceToHost ) );
+    soa_to_aos( soa.data(), aos, nos );
+}
+
+void LLGRunner::set_injected_noise( const double * noise, long n_steps )
+{
+    if( !noise || n_steps <= 0 )
+    {
+        b_noise.free();
+        noise_steps = noise_pos = 0;
+        return;
+    }
+    const size_t n = (size_t)n_steps * 3 * (size_t)n_total;
+    b_noise.resize( sizeof( double ) * n );
+    HIPCHECK( hipMemcpy( b_noise.as<void>(), noise, sizeof( double ) * n, hipMemcpyHostToDevice ) );
+    noise_steps = n_steps;
+    noise_pos   = 0;
+}
+
+void LLGRunner::ddi_field( const double * x )
+{
+    for( size_t k = 0; k < specs.size(); ++k )
+        if( ddi[k] )
+        {
+            const long long base = 3 * imgs_host[k].off;
+            ddi[k]->field( x + base, b_h.as<double>() + base, specs[k].geometry->nos, stream );
+        }
+}
+
+static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuffer & s, DeviceBuffer & sp,
+                              DeviceBuffer & sp2, DeviceBuffer & aux, DeviceBuffer & aux2, DeviceBuffer & h,
+                              DeviceBuffer & eff, DeviceBuffer & efft, DeviceBuffer & pE, DeviceBuffer & pT,
+                              DeviceBuffer & stop )
+{
+    StepArgs A{};
+    A.imgs    = imgs.as<ImageDev>();
+    A.blk_img = blk.as<int>();
+    A.s       = s.as<double>();
+    A.sp      = sp.as<double>();
+    A.sp2     = sp2.as<double>();
+    A.aux     = aux.as<double>();
+    A.aux2    = aux2.as<double>();
+    A.h       = h.as<double>();
+    A.eff     = eff.as<double>();
+    A.eff_tmp = efft.as<double>();
+    A.part_E  = pE.as<double>();
+    A.part_T  = pT.as<double>();
+    A.stop    = stop.as<int>();
+    return A;
+}
+
+void LLGRunner::step( bool obs, bool count, bool check, double conv )
+{
+    (void)count;
+    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
+                               b_stop );
+    A.step  = step_counter;
+    A.obs   = obs ? 1 : 0;
+    A.noise = nullptr;
+    if( noise_steps > 0 )
+    {
+        if( noise_pos >= noise_steps )
+            throw std::runtime_error( "injected thermal noise exhausted" );
+        A.noise = b_noise.as<double>() + (size_t)noise_pos * 3 * (size_t)n_total;
+        ++noise_pos;
+    }
+    const int n_stages = solver == SOLVER_RK4 ? 4 : 2;
+    for( int st = 0; st < n_stages; ++st )
+    {
+        const double * x = st == 0 ? A.s : ( ( solver == SOLVER_RK4 && st == 2 ) ? A.sp2 : A.sp );
+        if( any_ddi )
+            ddi_field( x );
+        llg_launch_stage( solver, st, A, n_blocks, stream );
+    }
+    if( obs )
+        llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_stopstep.as<long long>(),
+                             (long long)step_counter, conv, check ? 1 : 0, stream );
+    ++step_counter;
+}
+
+long LLGRunner::run( long n, int observe_every, bool check_convergence, double force_convergence )
+{
+    if( n <= 0 )
+        return 0;
+    HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
+    const uint64_t first = step_counter;
+    auto t0              = std::chrono::steady_clock::now();
+    for( long it = 0; it < n; ++it )
+    {
+        bool obs = ( it == n - 1 ) || ( observe_every > 0 && ( it + 1 ) % observe_every == 0 );
+        step( obs, true, check_convergence && obs, force_convergence );
+    }
+    HIPCHECK( hipGetLastError() );
+    int stopped         = 0;
+    long long stop_step = 0;
+    HIPCHECK( hipMemcpyAsync( &stopped, b_stop.as<void>(), sizeof( int ), hipMemcpyDeviceToHost, stream ) );
+    HIPCHECK( hipMemcpyAsync( &stop_step, b_stopstep.as<void>(), sizeof( long long ), hipMemcpyDeviceToHost,
+                              stream ) );
+    HIPCHECK( hipMemcpyAsync( obs_host.data(), b_obs.as<void>(), sizeof( double ) * obs_host.size(),
+                              hipMemcpyDeviceToHost, stream ) );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    auto t1      = std::chrono::steady_clock::now();
+    last_step_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count() / (double)n;
+    long done    = n;
+    if( stopped )
+    {
+        done         = (long)( stop_step - (long long)first + 1 );
+        step_counter = first + (uint64_t)done;
+    }
+    return done;
+}
+
+void LLGRunner::initial_observe()
+{
+    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
+                               b_stop );
+    if( any_ddi )
+        ddi_field( A.s );
+    HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
+    llg_launch_observe( A, solver, n_blocks, stream );
+    llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_stopstep.as<long long>(), -1, 0.0, 0, stream );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipMemcpyAsync( obs_host.data(), b_obs.as<void>(), sizeof( double ) * obs_host.size(),
+                              hipMemcpyDeviceToHost, stream ) );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+}
+
+double LLGRunner::gradient( int img, double * grad_aos )
+{
+    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
+                               b_stop );
+    if( any_ddi )
+        ddi_field( A.s );
+    DeviceBuffer g( sizeof( double ) * 3 * (size_t)n_total );
+    llg_launch_gradient( A, A.s, g.as<double>(), n_blocks, stream );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    const ImageDev & I = imgs_host[img];
+    const int nos      = specs[img].geometry->nos;
+    std::vector<double> soa( 3 * (size_t)nos ), pE( I.n_blocks );
+    HIPCHECK( hipMemcpy( soa.data(), g.as<double>() + 3 * I.off, sizeof( double ) * soa.size(),
+                         hipMemcpyDeviceToHost ) );
+    HIPCHECK( hipMemcpy( pE.data(), b_pE.as<double>() + I.block_begin, sizeof( double ) * I.n_blocks,
+                         hipMemcpyDeviceToHost ) );
+    if( grad_aos )
+        soa_to_aos( soa.data(), grad_aos, nos );
+    double E = 0;
+    for( double v : pE )
+        E += v;
+    return E;
+}
+
+} // namespace spirit_hip

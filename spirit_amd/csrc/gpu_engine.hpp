@@ -1,0 +1,169 @@
+// Device-resident runners: DDI plan, batched LLG runner and GNEB chain runner.
+#pragma once
+
+#include "ddi.hpp"
+#include "host_core.hpp"
+#include "llg.hpp"
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace spirit_hip
+{
+
+void hip_check( hipError_t e, const char * what );
+#define HIPCHECK( x ) ::spirit_hip::hip_check( ( x ), #x )
+
+// Owning device allocation.
+class DeviceBuffer
+{
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer( size_t bytes )
+    {
+        resize( bytes );
+    }
+    ~DeviceBuffer();
+    DeviceBuffer( const DeviceBuffer & )             = delete;
+    DeviceBuffer & operator=( const DeviceBuffer & ) = delete;
+    void resize( size_t bytes );
+    void free();
+    template<typename T>
+    T * as() const
+    {
+        return static_cast<T *>( p );
+    }
+    size_t size() const
+    {
+        return bytes;
+    }
+
+private:
+    void * p     = nullptr;
+    size_t bytes = 0;
+};
+
+// Radix plan + twiddle table of one FFT length.
+struct FftLenHost
+{
+    FftLen dev{};
+    DeviceBuffer W;
+    void build( int n );
+};
+
+// One zero-padded FFT convolution plan (one image geometry).
+class DdiPlan
+{
+public:
+    // throws std::runtime_error for unsupported grids
+    void build( const Geometry & g, const Hamiltonian & h, hipStream_t st );
+    void field( const double * spins_soa, double * h_soa, int nos, hipStream_t st );
+    const DdiDev & dev() const
+    {
+        return d;
+    }
+    size_t bytes() const;
+
+private:
+    DdiDev d{};
+    FftLenHost La, Lb, Lc;
+    DeviceBuffer X, Dh;
+};
+
+// Padded DDI grid, Hamiltonian_Heisenberg.cpp:1501-1530 (without the kissFFT-only length-1 workaround).
+std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h );
+
+// Builds the POD image description a kernel reads (pair tables are uploaded to `pairs`).
+struct HamUpload
+{
+    HamDev dev{};
+    DeviceBuffer pairs;
+    DeviceBuffer site_K;
+    uint64_t version     = ~0ull;
+    uint64_t ddi_version = ~0ull;
+    void upload( const Geometry & g, const Hamiltonian & h );
+};
+
+struct ImageSpec
+{
+    std::shared_ptr<Geometry> geometry;
+    std::shared_ptr<Hamiltonian> hamiltonian;
+    LLGParameters llg;
+    double * spins_aos = nullptr; // host mirror (nos*3)
+};
+
+// Runs the LLG step loop of one or several images (a batched ensemble) on the GPU.
+class LLGRunner
+{
+public:
+    LLGRunner( const std::vector<ImageSpec> & images, int solver );
+    ~LLGRunner();
+
+    void upload_spins();   // host AoS -> device
+    void download_spins(); // device -> host AoS
+    // Update per-image LLG parameters (dt, damping, temperature, ...) without reallocation.
+    void set_llg( int img, const LLGParameters & p );
+    void refresh_hamiltonian(); // re-upload after parameter changes
+
+    // Enqueue n steps.  observe_every > 0 computes energy/torque every that many steps
+    // (always on the last step); with check_convergence the run stops on the device once the
+    // max torque drops below force_convergence.  Returns the number of steps performed.
+    long run( long n, int observe_every, bool check_convergence, double force_convergence );
+    // Energy, max torque and effective field of the current spins (no step taken).
+    void initial_observe();
+
+    double energy( int img ) const
+    {
+        return obs_host[2 * img];
+    }
+    double max_torque( int img ) const
+    {
+        return obs_host[2 * img + 1];
+    }
+    void download_effective_field( int img, double * aos );
+    // gradient (AoS) and energy of the current device spins of image img
+    double gradient( int img, double * grad_aos );
+    // injected N(0,1) noise for the next `n_steps` steps, AoS per spin over all images
+    void set_injected_noise( const double * noise, long n_steps );
+
+    int n_images() const
+    {
+        return (int)specs.size();
+    }
+    long long total_spins() const
+    {
+        return n_total;
+    }
+    uint64_t step_counter = 0;
+    hipStream_t stream = nullptr;
+    double last_step_ms = 0;
+
+    // raw device pointers (for the adjoint module and tests)
+    double * dev_spins()
+    {
+        return b_s.as<double>();
+    }
+
+private:
+    void step( bool obs, bool count, bool check, double conv );
+    void ddi_field( const double * x );
+    void upload_images();
+
+    std::vector<ImageSpec> specs;
+    int solver;
+    long long n_total = 0;
+    int n_blocks      = 0;
+    std::vector<ImageDev> imgs_host;
+    std::vector<int> blk_img_host;
+    std::vector<std::unique_ptr<HamUpload>> hams;
+    std::vector<std::unique_ptr<DdiPlan>> ddi;
+    DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
+        b_noise, b_stopstep;
+    std::vector<double> obs_host;
+    bool any_ddi = false;
+    long noise_steps = 0, noise_pos = 0;
+};
+
+} // namespace spirit_hip

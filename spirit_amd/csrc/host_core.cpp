@@ -1,0 +1,744 @@
+// Host-side geometry, neighbour tables, Hamiltonian parameters and input-file reader.
+#include "host_core.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <random>
+
+namespace spirit_hip
+{
+
+int g_log_console_level = 5;
+bool g_quiet            = false;
+int g_n_errors          = 0;
+int g_n_warnings        = 0;
+
+// Log levels follow core/include/utility/Logging.hpp: All=0, Severe=1, Error=2, Warning=3,
+// Parameter=4, Info=5, Debug=6.
+void log_message( int level, const std::string & msg )
+{
+    if( level == 2 || level == 1 )
+        ++g_n_errors;
+    if( level == 3 )
+        ++g_n_warnings;
+    if( g_quiet && level > 2 )
+        return;
+    if( level <= g_log_console_level )
+        std::fprintf( stderr, "[spirit-hip] %s\n", msg.c_str() );
+}
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+void Geometry::basis_position( int iatom, double * out ) const
+{
+    for( int c = 0; c < 3; ++c )
+    {
+        double v = ( 0.0 + cell_atoms[iatom][0] ) * bravais[0][c] + ( 0.0 + cell_atoms[iatom][1] ) * bravais[1][c]
+                   + ( 0.0 + cell_atoms[iatom][2] ) * bravais[2][c];
+        out[c] = lattice_constant * v;
+    }
+}
+
+// positions: Geometry.cpp:142-161
+void Geometry::generate()
+{
+    nb            = (int)cell_atoms.size();
+    n_cells_total = n_cells[0] * n_cells[1] * n_cells[2];
+    nos           = nb * n_cells_total;
+    positions.assign( 3 * (size_t)nos, 0.0 );
+    mu_s.assign( (size_t)nos, 1.0 );
+    if( (int)mu_basis.size() < nb )
+        mu_basis.resize( nb, mu_basis.empty() ? 1.0 : mu_basis[0] );
+    for( long dc = 0; dc < n_cells[2]; ++dc )
+        for( long db = 0; db < n_cells[1]; ++db )
+            for( long da = 0; da < n_cells[0]; ++da )
+                for( int ia = 0; ia < nb; ++ia )
+                {
+                    long ispin = ia + dc * nb * n_cells[1] * n_cells[0] + db * nb * n_cells[0] + da * nb;
+                    for( int c = 0; c < 3; ++c )
+                    {
+                        double v = ( (double)da + cell_atoms[ia][0] ) * bravais[0][c]
+                                   + ( (double)db + cell_atoms[ia][1] ) * bravais[1][c]
+                                   + ( (double)dc + cell_atoms[ia][2] ) * bravais[2][c];
+                        positions[3 * ispin + c] = lattice_constant * v;
+                    }
+                    mu_s[ispin] = mu_basis[ia];
+                }
+    for( int c = 0; c < 3; ++c )
+    {
+        bounds_min[c] = DBL_MAX;
+        bounds_max[c] = -DBL_MAX;
+    }
+    for( int i = 0; i < nos; ++i )
+        for( int c = 0; c < 3; ++c )
+        {
+            bounds_min[c] = std::min( bounds_min[c], positions[3 * i + c] );
+            bounds_max[c] = std::max( bounds_max[c], positions[3 * i + c] );
+        }
+    for( int c = 0; c < 3; ++c )
+    {
+        center[c]          = 0.5 * ( bounds_min[c] + bounds_max[c] );
+        cell_bounds_min[c] = DBL_MAX;
+        cell_bounds_max[c] = -DBL_MAX;
+    }
+    for( int ia = 0; ia < nb; ++ia )
+    {
+        double p[3];
+        basis_position( ia, p );
+        for( int c = 0; c < 3; ++c )
+        {
+            cell_bounds_min[c] = std::min( cell_bounds_min[c], p[c] );
+            cell_bounds_max[c] = std::max( cell_bounds_max[c], p[c] );
+        }
+    }
+    dimensionality = 0;
+    for( int c = 0; c < 3; ++c )
+        if( n_cells[c] > 1 )
+            ++dimensionality;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbours, core/src/engine/Neighbours.cpp:15-156
+// ---------------------------------------------------------------------------
+static double norm3( const double * v )
+{
+    return std::sqrt( v[0] * v[0] + v[1] * v[1] + v[2] * v[2] );
+}
+
+std::vector<double> shell_radii( const Geometry & g, int n_shells )
+{
+    const double min_shell_width = 1e-3;
+    std::vector<double> radii( n_shells, 0.0 );
+    double ta[3], tb[3], tc[3];
+    for( int c = 0; c < 3; ++c )
+    {
+        ta[c] = g.lattice_constant * g.bravais[0][c];
+        tb[c] = g.lattice_constant * g.bravais[1][c];
+        tc[c] = g.lattice_constant * g.bravais[2][c];
+    }
+    int max_n = n_shells + 2;
+    int i_max = std::min( max_n, g.n_cells[0] - 1 );
+    int j_max = std::min( max_n, g.n_cells[1] - 1 );
+    int k_max = std::min( max_n, g.n_cells[2] - 1 );
+    if( norm3( ta ) == 0.0 )
+        i_max = 0;
+    if( norm3( tb ) == 0.0 )
+        j_max = 0;
+    if( norm3( tc ) == 0.0 )
+        k_max = 0;
+    double outermost = 0, previous = 0;
+    for( auto & radius : radii )
+    {
+        previous  = outermost;
+        outermost = DBL_MAX;
+        for( int a1 = 0; a1 < (int)g.cell_atoms.size(); ++a1 )
+        {
+            const Vec3 & pos_one = g.cell_atoms[a1];
+            for( int i = i_max; i >= 0; --i )
+                for( int j = j_max; j >= -j_max; --j )
+                    for( int k = k_max; k >= -k_max; --k )
+                        for( int a2 = 0; a2 < (int)g.cell_atoms.size(); ++a2 )
+                        {
+                            if( a1 == a2 && i == 0 && j == 0 && k == 0 )
+                                continue;
+                            double d[3];
+                            for( int c = 0; c < 3; ++c )
+                            {
+                                double pos_two = g.cell_atoms[a2][c] + i * ta[c] + j * tb[c] + k * tc[c];
+                                d[c]           = pos_one[c] - pos_two;
+                            }
+                            double delta = norm3( d );
+                            if( delta - previous > min_shell_width && delta < outermost )
+                            {
+                                outermost = delta;
+                                radius    = delta;
+                            }
+                        }
+        }
+    }
+    return radii;
+}
+
+void neighbours_in_shells( const Geometry & g, int n_shells, bool redundant, std::vector<Pair> & pairs,
+                           std::vector<int> & shells )
+{
+    const double min_shell_width = 1e-3;
+    auto radii                   = shell_radii( g, n_shells );
+    double ta[3], tb[3], tc[3];
+    for( int c = 0; c < 3; ++c )
+    {
+        ta[c] = g.lattice_constant * g.bravais[0][c];
+        tb[c] = g.lattice_constant * g.bravais[1][c];
+        tc[c] = g.lattice_constant * g.bravais[2][c];
+    }
+    int max_n = n_shells + 2;
+    int i_max = std::min( max_n, g.n_cells[0] - 1 );
+    int j_max = std::min( max_n, g.n_cells[1] - 1 );
+    int k_max = std::min( max_n, g.n_cells[2] - 1 );
+    int i_min = -i_max, j_min = -j_max, k_min = -k_max;
+    if( norm3( ta ) == 0.0 )
+        i_max = 0;
+    if( norm3( tb ) == 0.0 )
+        j_max = 0;
+    if( norm3( tc ) == 0.0 )
+        k_max = 0;
+    const int nb = (int)g.cell_atoms.size();
+    for( int a1 = 0; a1 < nb; ++a1 )
+    {
+        int second_atom_min = redundant ? 0 : a1;
+        const Vec3 & pos_one = g.cell_atoms[a1];
+        for( int s = 0; s < n_shells; ++s )
+        {
+            double radius = radii[s];
+            for( int i = i_max; i >= i_min; --i )
+                for( int j = j_max; j >= j_min; --j )
+                    for( int k = k_max; k >= k_min; --k )
+                        for( int a2 = second_atom_min; a2 < nb; ++a2 )
+                        {
+                            if( ( a2 > a1 ) || ( i > 0 || ( i == 0 && j > 0 ) || ( i == 0 && j == 0 && k > 0 ) )
+                                || redundant )
+                            {
+                                double d[3];
+                                for( int c = 0; c < 3; ++c )
+                                {
+                                    double pos_two = g.cell_atoms[a2][c] + i * ta[c] + j * tb[c] + k * tc[c];
+                                    d[c]           = pos_one[c] - pos_two;
+                                }
+                                double delta = norm3( d );
+                                if( std::fabs( delta - radius ) < min_shell_width )
+                                {
+                                    pairs.push_back( Pair{ a1, a2, { i, j, k } } );
+                                    shells.push_back( s );
+                                }
+                            }
+                        }
+        }
+    }
+}
+
+// DMI_Normal_from_Pair, Neighbours.cpp:249-286
+Vec3 dmi_normal_from_pair( const Geometry & g, const Pair & p, int chirality )
+{
+    double ipos[3], jpos[3], d[3];
+    g.basis_position( p.i, ipos );
+    g.basis_position( p.j, jpos );
+    for( int c = 0; c < 3; ++c )
+    {
+        double ta = g.lattice_constant * g.bravais[0][c];
+        double tb = g.lattice_constant * g.bravais[1][c];
+        double tc = g.lattice_constant * g.bravais[2][c];
+        jpos[c]   = jpos[c] + p.t[0] * ta + p.t[1] * tb + p.t[2] * tc;
+    }
+    auto normalized = []( double * v ) {
+        double z = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if( z > 0 )
+        {
+            double n = std::sqrt( z );
+            v[0] /= n;
+            v[1] /= n;
+            v[2] /= n;
+        }
+    };
+    auto cross = []( const double * a, const double * b ) {
+        return Vec3{ a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0] };
+    };
+    const double ez[3] = { 0, 0, 1 };
+    if( chirality == 1 || chirality == 2 || chirality == -2 )
+    {
+        for( int c = 0; c < 3; ++c )
+            d[c] = jpos[c] - ipos[c];
+        normalized( d );
+        if( chirality == 1 )
+            return Vec3{ d[0], d[1], d[2] };
+        if( chirality == 2 )
+            return cross( d, ez );
+        return cross( ez, d );
+    }
+    if( chirality == -1 )
+    {
+        for( int c = 0; c < 3; ++c )
+            d[c] = ipos[c] - jpos[c];
+        normalized( d );
+        return Vec3{ d[0], d[1], d[2] };
+    }
+    return Vec3{ 0, 0, 0 };
+}
+
+// Hamiltonian_Heisenberg::Update_Interactions, Hamiltonian_Heisenberg.cpp:101-198
+void Hamiltonian::update_interactions( const Geometry & g )
+{
+    exch_pairs.clear();
+    exch_J.clear();
+    if( !exchange_shell_J.empty() )
+    {
+        std::vector<int> shells;
+        neighbours_in_shells( g, (int)exchange_shell_J.size(), true, exch_pairs, shells );
+        for( size_t k = 0; k < exch_pairs.size(); ++k )
+            exch_J.push_back( exchange_shell_J[shells[k]] );
+    }
+    else
+    {
+        exch_pairs = exch_pairs_in;
+        exch_J     = exch_J_in;
+        for( size_t k = 0; k < exch_pairs_in.size(); ++k )
+        {
+            const Pair & p = exch_pairs_in[k];
+            exch_pairs.push_back( Pair{ p.j, p.i, { -p.t[0], -p.t[1], -p.t[2] } } );
+            exch_J.push_back( exch_J_in[k] );
+        }
+    }
+    dmi_pairs.clear();
+    dmi_D.clear();
+    dmi_n.clear();
+    if( !dmi_shell_D.empty() )
+    {
+        std::vector<int> shells;
+        neighbours_in_shells( g, (int)dmi_shell_D.size(), true, dmi_pairs, shells );
+        for( size_t k = 0; k < dmi_pairs.size(); ++k )
+        {
+            dmi_n.push_back( dmi_normal_from_pair( g, dmi_pairs[k], dmi_chirality ) );
+            dmi_D.push_back( dmi_shell_D[shells[k]] );
+        }
+    }
+    else
+    {
+        dmi_pairs = dmi_pairs_in;
+        dmi_D     = dmi_D_in;
+        dmi_n     = dmi_n_in;
+        for( size_t k = 0; k < dmi_pairs_in.size(); ++k )
+        {
+            const Pair & p = dmi_pairs_in[k];
+            dmi_pairs.push_back( Pair{ p.j, p.i, { -p.t[0], -p.t[1], -p.t[2] } } );
+            dmi_D.push_back( dmi_D_in[k] );
+            dmi_n.push_back( Vec3{ -dmi_n_in[k][0], -dmi_n_in[k][1], -dmi_n_in[k][2] } );
+        }
+    }
+    ddi_bc = bc;
+    ++version;
+    ++ddi_version;
+}
+
+bool Hamiltonian::has_zeeman() const
+{
+    return std::fabs( field_magnitude ) > 1e-60;
+}
+
+// ---------------------------------------------------------------------------
+// Input file
+// ---------------------------------------------------------------------------
+ConfigFile::ConfigFile( const std::string & path )
+{
+    std::ifstream f( path, std::ios::in | std::ios::binary );
+    if( !f.is_open() )
+        return;
+    opened = true;
+    std::string line;
+    while( std::getline( f, line ) )
+    {
+        line.erase( std::remove_if( line.begin(), line.end(), []( char ch ) { return ch == '|' || ch == '+'; } ),
+                    line.end() );
+        auto pos = line.find( '#' );
+        if( pos == 0 )
+            continue;
+        if( pos != std::string::npos )
+            line.erase( pos );
+        if( !line.empty() && line.back() == '\r' )
+            line.pop_back();
+        lines.push_back( line );
+    }
+}
+
+bool ConfigFile::find( const std::string & keyword )
+{
+    std::string kw = keyword;
+    std::transform( kw.begin(), kw.end(), kw.begin(), ::tolower );
+    for( size_t k = 0; k < lines.size(); ++k )
+    {
+        std::string l = lines[k];
+        std::transform( l.begin(), l.end(), l.begin(), ::tolower );
+        if( l.compare( 0, kw.size(), kw ) == 0 )
+        {
+            iss.clear();
+            iss.str( lines[k] );
+            std::string dump;
+            std::istringstream kws( keyword );
+            while( kws >> dump )
+                iss >> dump;
+            cursor = k + 1;
+            return true;
+        }
+    }
+    return false;
+}
+
+bool ConfigFile::next_line()
+{
+    if( cursor >= lines.size() )
+        return false;
+    iss.clear();
+    iss.str( lines[cursor++] );
+    return true;
+}
+
+long duration_from_string( const std::string & s )
+{
+    // "h:m:s" -> seconds (utility/Timing.cpp DurationFromString)
+    long h = 0, m = 0, sec = 0;
+    char c1 = 0, c2 = 0;
+    std::istringstream is( s );
+    if( is >> h >> c1 >> m >> c2 >> sec )
+        return h * 3600 + m * 60 + sec;
+    return 0;
+}
+
+std::shared_ptr<Geometry> geometry_from_config( const std::string & path )
+{
+    auto g = std::make_shared<Geometry>();
+    if( !path.empty() )
+    {
+        ConfigFile f( path );
+        if( !f.ok() )
+            log_message( 2, "Could not open config file \"" + path + "\"; using defaults" );
+        else
+        {
+            f.read( g->lattice_constant, "lattice_constant" );
+            if( f.find( "bravais_lattice" ) )
+            {
+                std::string t;
+                f.stream() >> t;
+                std::transform( t.begin(), t.end(), t.begin(), ::tolower );
+                const double h = 0.5 * std::sqrt( 3.0 );
+                if( t == "sc" )
+                {
+                    g->bravais_type = BravaisType::SC;
+                    g->bravais      = { Vec3{ 1, 0, 0 }, Vec3{ 0, 1, 0 }, Vec3{ 0, 0, 1 } };
+                }
+                else if( t == "fcc" )
+                {
+                    g->bravais_type = BravaisType::FCC;
+                    g->bravais      = { Vec3{ 0.5, 0.0, 0.5 }, Vec3{ 0.5, 0.5, 0.0 }, Vec3{ 0.0, 0.5, 0.5 } };
+                }
+                else if( t == "bcc" )
+                {
+                    g->bravais_type = BravaisType::BCC;
+                    g->bravais      = { Vec3{ 0.5, 0.5, -0.5 }, Vec3{ -0.5, 0.5, -0.5 }, Vec3{ 0.5, -0.5, -0.5 } };
+                }
+                else if( t == "hex2d" || t == "hex2d60" )
+                {
+                    g->bravais_type = BravaisType::Hex2D;
+                    g->bravais      = { Vec3{ h, -0.5, 0 }, Vec3{ h, 0.5, 0 }, Vec3{ 0, 0, 1 } };
+                }
+                else if( t == "hex2d120" )
+                {
+                    g->bravais_type = BravaisType::Hex2D;
+                    g->bravais      = { Vec3{ 0.5, -h, 0 }, Vec3{ 0.5, h, 0 }, Vec3{ 0, 0, 1 } };
+                }
+                else
+                    log_message( 3, "Bravais lattice \"" + t + "\" unknown. Using simple cubic..." );
+            }
+            else if( f.find( "bravais_vectors" ) )
+            {
+                g->bravais_type = BravaisType::Irregular;
+                for( int k = 0; k < 3; ++k )
+                {
+                    f.next_line();
+                    f.stream() >> g->bravais[k][0] >> g->bravais[k][1] >> g->bravais[k][2];
+                }
+            }
+            else if( f.find( "bravais_matrix" ) )
+            {
+                g->bravais_type = BravaisType::Irregular;
+                for( int k = 0; k < 3; ++k )
+                {
+                    f.next_line();
+                    f.stream() >> g->bravais[0][k] >> g->bravais[1][k] >> g->bravais[2][k];
+                }
+            }
+            f.read3( g->n_cells, "n_basis_cells" );
+            if( f.find( "basis" ) )
+            {
+                f.next_line();
+                int n = 0;
+                f.stream() >> n;
+                if( n > 0 )
+                {
+                    g->cell_atoms.assign( n, Vec3{ 0, 0, 0 } );
+                    g->mu_basis.assign( n, 1.0 );
+                    for( int a = 0; a < n; ++a )
+                    {
+                        f.next_line();
+                        f.stream() >> g->cell_atoms[a][0] >> g->cell_atoms[a][1] >> g->cell_atoms[a][2];
+                    }
+                }
+            }
+            if( f.find( "mu_s" ) )
+            {
+                for( size_t a = 0; a < g->cell_atoms.size(); ++a )
+                {
+                    double v;
+                    if( f.stream() >> v )
+                        g->mu_basis[a] = v;
+                    else
+                        g->mu_basis[a] = g->mu_basis[0];
+                }
+            }
+            else
+                log_message( 2, "Keyword 'mu_s' not found. Using Default: 1" );
+        }
+    }
+    g->generate();
+    return g;
+}
+
+static void normalize_vec( Vec3 & v )
+{
+    double z = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if( z > 0 )
+    {
+        double n = std::sqrt( z );
+        for( auto & x : v )
+            x /= n;
+    }
+}
+
+std::shared_ptr<Hamiltonian> hamiltonian_from_config( const std::string & path, const Geometry & g )
+{
+    auto h = std::make_shared<Hamiltonian>();
+    double B = 0, K = 0, K4 = 0;
+    Vec3 Bn{ 0, 0, 1 }, Kn{ 0, 0, 1 };
+    std::string ham_type = "heisenberg_neighbours";
+    int n_shells_exch = 0, n_shells_dmi = 0;
+    std::vector<double> jij, dij;
+    std::string ddi = "none";
+    if( !path.empty() )
+    {
+        ConfigFile f( path );
+        if( f.ok() )
+        {
+            f.read( ham_type, "hamiltonian" );
+            std::array<int, 3> bci{ 0, 0, 0 };
+            f.read3( bci, "boundary_conditions" );
+            for( int c = 0; c < 3; ++c )
+                h->bc[c] = bci[c] != 0 ? 1 : 0;
+            f.read( B, "external_field_magnitude" );
+            f.read3( Bn, "external_field_normal" );
+            normalize_vec( Bn );
+            if( std::sqrt( Bn[0] * Bn[0] + Bn[1] * Bn[1] + Bn[2] * Bn[2] ) < 1e-8 )
+                Bn = { 0, 0, 1 };
+            f.read( K, "anisotropy_magnitude" );
+            f.read3( Kn, "anisotropy_normal" );
+            normalize_vec( Kn );
+            f.read( K4, "cubic_anisotropy_magnitude" );
+            if( ham_type == "heisenberg_pairs" )
+            {
+                h->from_shells = false;
+                if( f.find( "n_interaction_pairs" ) )
+                {
+                    int n = 0;
+                    f.stream() >> n;
+                    f.next_line();
+                    std::vector<std::string> cols;
+                    std::string w;
+                    while( f.stream() >> w )
+                    {
+                        std::transform( w.begin(), w.end(), w.begin(), ::tolower );
+                        cols.push_back( w );
+                    }
+                    for( int r = 0; r < n && f.next_line(); ++r )
+                    {
+                        int pi = 0, pj = 0, da = 0, db = 0, dc = 0;
+                        double J = 0, Dij = 0, D1 = 0, D2 = 0, D3 = 0;
+                        bool has_Dij = false;
+                        for( auto & cname : cols )
+                        {
+                            if( cname == "i" )
+                                f.stream() >> pi;
+                            else if( cname == "j" )
+                                f.stream() >> pj;
+                            else if( cname == "da" )
+                                f.stream() >> da;
+                            else if( cname == "db" )
+                                f.stream() >> db;
+                            else if( cname == "dc" )
+                                f.stream() >> dc;
+                            else if( cname == "jij" )
+                                f.stream() >> J;
+                            else if( cname == "dij" )
+                            {
+                                f.stream() >> Dij;
+                                has_Dij = true;
+                            }
+                            else if( cname == "dijx" )
+                                f.stream() >> D1;
+                            else if( cname == "dijy" )
+                                f.stream() >> D2;
+                            else if( cname == "dijz" )
+                                f.stream() >> D3;
+                            else
+                            {
+                                std::string dump;
+                                f.stream() >> dump;
+                            }
+                        }
+                        // Dataparser.cpp Pairs_from_File: DM vector normalisation
+                        double dn = std::sqrt( std::pow( D1, 2 ) + std::pow( D2, 2 ) + std::pow( D3, 2 ) );
+                        if( dn != 0 )
+                        {
+                            D1 = D1 / dn;
+                            D2 = D2 / dn;
+                            D3 = D3 / dn;
+                        }
+                        if( !has_Dij )
+                            Dij = dn;
+                        if( J != 0 )
+                        {
+                            bool found = false;
+                            for( size_t k = 0; k < h->exch_pairs_in.size(); ++k )
+                            {
+                                auto & p = h->exch_pairs_in[k];
+                                if( ( pi == p.i && pj == p.j && da == p.t[0] && db == p.t[1] && dc == p.t[2] )
+                                    || ( pi == p.j && pj == p.i && da == -p.t[0] && db == -p.t[1]
+                                         && dc == -p.t[2] ) )
+                                {
+                                    h->exch_J_in[k] += J;
+                                    found = true;
+                                    break;
+                                }
+                            }
+                            if( !found )
+                            {
+                                h->exch_pairs_in.push_back( Pair{ pi, pj, { da, db, dc } } );
+                                h->exch_J_in.push_back( J );
+                            }
+                        }
+                        if( Dij != 0 )
+                        {
+                            h->dmi_pairs_in.push_back( Pair{ pi, pj, { da, db, dc } } );
+                            h->dmi_D_in.push_back( Dij );
+                            h->dmi_n_in.push_back( Vec3{ D1, D2, D3 } );
+                        }
+                    }
+                }
+            }
+            else
+            {
+                f.read( n_shells_exch, "n_shells_exchange" );
+                jij.assign( n_shells_exch, 0.0 );
+                if( n_shells_exch > 0 && f.find( "jij" ) )
+                    for( int s = 0; s < n_shells_exch; ++s )
+                        f.stream() >> jij[s];
+                f.read( n_shells_dmi, "n_shells_dmi" );
+                dij.assign( n_shells_dmi, 0.0 );
+                if( n_shells_dmi > 0 && f.find( "dij" ) )
+                    for( int s = 0; s < n_shells_dmi; ++s )
+                        f.stream() >> dij[s];
+                f.read( h->dmi_chirality, "dm_chirality" );
+            }
+            f.read( ddi, "ddi_method" );
+            f.read3( h->ddi_images, "ddi_n_periodic_images" );
+            int zp = 1;
+            if( f.read( zp, "ddi_pb_zero_padding" ) )
+                h->ddi_pb_zero_padding = zp != 0;
+            f.read( h->ddi_radius, "ddi_radius" );
+        }
+    }
+    h->field_magnitude = B * 0.057883817555;
+    h->field_normal    = Bn;
+    int nb             = (int)g.cell_atoms.size();
+    if( K != 0 )
+        for( int a = 0; a < nb; ++a )
+        {
+            h->aniso_idx.push_back( a );
+            h->aniso_K.push_back( K );
+            h->aniso_n.push_back( Kn );
+        }
+    if( K4 != 0 )
+        for( int a = 0; a < nb; ++a )
+        {
+            h->cubic_idx.push_back( a );
+            h->cubic_K.push_back( K4 );
+        }
+    h->exchange_shell_J = jij;
+    h->dmi_shell_D      = dij;
+    if( ddi == "fft" )
+        h->ddi_method = DDI_FFT;
+    else if( ddi == "fmm" )
+        h->ddi_method = DDI_FMM;
+    else if( ddi == "cutoff" )
+        h->ddi_method = DDI_CUTOFF;
+    else
+        h->ddi_method = DDI_NONE;
+    h->update_interactions( g );
+    return h;
+}
+
+LLGParameters llg_parameters_from_config( const std::string & path )
+{
+    LLGParameters p;
+    p.seed = (uint64_t)std::random_device{}();
+    if( path.empty() )
+        return p;
+    ConfigFile f( path );
+    if( !f.ok() )
+        return p;
+    f.read( p.output_file_tag, "output_file_tag" );
+    f.read( p.output_folder, "llg_output_folder" );
+    f.read( p.output_any, "llg_output_any" );
+    f.read( p.output_initial, "llg_output_initial" );
+    f.read( p.output_final, "llg_output_final" );
+    std::string wt;
+    if( f.read( wt, "llg_max_walltime" ) )
+        p.max_walltime_sec = duration_from_string( wt );
+    long seed;
+    if( f.read( seed, "llg_seed" ) )
+        p.seed = (uint64_t)seed;
+    f.read( p.n_iterations, "llg_n_iterations" );
+    f.read( p.n_iterations_log, "llg_n_iterations_log" );
+    f.read( p.n_iterations_amortize, "llg_n_iterations_amortize" );
+    f.read( p.dt, "llg_dt" );
+    f.read( p.temperature, "llg_temperature" );
+    f.read3( p.temperature_gradient_direction, "llg_temperature_gradient_direction" );
+    normalize_vec( p.temperature_gradient_direction );
+    f.read( p.temperature_gradient_inclination, "llg_temperature_gradient_inclination" );
+    f.read( p.damping, "llg_damping" );
+    f.read( p.beta, "llg_beta" );
+    f.read( p.stt_use_gradient, "llg_stt_use_gradient" );
+    f.read( p.stt_magnitude, "llg_stt_magnitude" );
+    f.read3( p.stt_polarisation_normal, "llg_stt_polarisation_normal" );
+    normalize_vec( p.stt_polarisation_normal );
+    f.read( p.force_convergence, "llg_force_convergence" );
+    return p;
+}
+
+GNEBParameters gneb_parameters_from_config( const std::string & path )
+{
+    GNEBParameters p;
+    if( path.empty() )
+        return p;
+    ConfigFile f( path );
+    if( !f.ok() )
+        return p;
+    f.read( p.output_file_tag, "output_file_tag" );
+    f.read( p.output_folder, "gneb_output_folder" );
+    std::string wt;
+    if( f.read( wt, "gneb_max_walltime" ) )
+        p.max_walltime_sec = duration_from_string( wt );
+    f.read( p.spring_constant, "gneb_spring_constant" );
+    f.read( p.force_convergence, "gneb_force_convergence" );
+    f.read( p.n_iterations, "gneb_n_iterations" );
+    f.read( p.n_iterations_log, "gneb_n_iterations_log" );
+    f.read( p.n_iterations_amortize, "gneb_n_iterations_amortize" );
+    f.read( p.n_E_interpolations, "gneb_n_energy_interpolations" );
+    f.read( p.moving_endpoints, "gneb_moving_endpoints" );
+    f.read( p.translating_endpoints, "gneb_translating_endpoints" );
+    f.read( p.escape_first, "gneb_escape_first" );
+    f.read( p.spring_force_ratio, "gneb_spring_force_ratio" );
+    f.read( p.path_shortening_constant, "gneb_path_shortening_constant" );
+    return p;
+}
+
+} // namespace spirit_hip

@@ -1,0 +1,40 @@
+// Launch interface of the fused effective-field + LLG stage kernels.
+#pragma once
+
+#include "device_types.hpp"
+
+namespace spirit_hip
+{
+
+constexpr int LLG_THREADS = 256;
+
+// Arguments of one stage launch.  Fields are batched SoA buffers; image k's
+// three components start at 3*imgs[k].off.
+struct StepArgs
+{
+    const ImageDev * imgs; // device array of images
+    const int * blk_img;   // device array: block -> image
+    double * s;            // current spins (updated by the last stage)
+    double * sp;           // predictor spins
+    double * sp2;          // second predictor buffer (RK4)
+    double * aux;          // first virtual force (Depondt/Heun, SIB on observed steps) or RK4 accumulator
+    double * aux2;         // RK4: first virtual force on observed steps
+    const double * h;      // DDI field of the stage's input configuration (nullable)
+    double * eff;          // effective field output (observed steps)
+    double * eff_tmp;      // first force, kept for the effective field (observed steps)
+    const double * noise;  // injected N(0,1) thermal noise of this step, AoS per spin (nullable)
+    double * part_E;       // per-block energy partial sums
+    double * part_T;       // per-block max squared torque
+    int * stop;            // set once the run has converged; every stage returns early
+    uint64_t step;         // global step index (thermal-noise counter)
+    int obs;               // compute observables (energy, torque, effective field)
+    int pad;
+};
+
+void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, hipStream_t st );
+void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, int n_blocks, hipStream_t st );
+void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
+void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, long long * stop_step, long long step_index,
+                          double force_convergence, int check_convergence, hipStream_t st );
+
+} // namespace spirit_hip

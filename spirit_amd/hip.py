@@ -1,0 +1,67 @@
+"""Engine-only extensions of the C API (include/Spirit/Spirit_HIP.h)."""
+import ctypes
+
+import numpy as np
+
+from spirit_amd import spiritlib
+
+_spirit = spiritlib.load_spirit_library()
+_vp, _i, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+_PD = ctypes.POINTER(_d)
+_F3 = ctypes.c_float * 3
+
+
+def _fn(name, argtypes, restype=None):
+    f = getattr(_spirit, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    return f
+
+
+_Set_Host_Sync = _fn("Spirit_HIP_Set_Host_Sync", [_vp, _i])
+_Set_Thermal_Noise = _fn("Spirit_HIP_Set_Thermal_Noise", [_vp, _PD, _i, _i, _i])
+_Set_Site_Anisotropy = _fn("Spirit_HIP_Hamiltonian_Set_Site_Anisotropy", [_vp, _PD, _F3, _i, _i])
+_Gradient_and_Energy = _fn("Spirit_HIP_Gradient_and_Energy", [_vp, _PD, _PD, _i, _i], _i)
+_Get_Last_Step_Time = _fn("Spirit_HIP_Get_Last_Step_Time", [_vp, _i, _i], _d)
+_Device_Name = _fn("Spirit_HIP_Device_Name", [], ctypes.c_char_p)
+
+
+def set_host_sync(p_state, mode):
+    _Set_Host_Sync(ctypes.c_void_p(p_state), _i(1 if mode else 0))
+
+
+def set_thermal_noise(p_state, noise, idx_image=-1, idx_chain=-1):
+    """Inject N(0,1) noise of shape (n_steps, nos, 3) for the next n_steps LLG steps (None: off)."""
+    if noise is None:
+        _Set_Thermal_Noise(ctypes.c_void_p(p_state), None, 0, _i(idx_image), _i(idx_chain))
+        return
+    arr = np.ascontiguousarray(noise, dtype=np.float64)
+    _Set_Thermal_Noise(ctypes.c_void_p(p_state), arr.ctypes.data_as(_PD), _i(arr.shape[0]), _i(idx_image),
+                       _i(idx_chain))
+
+
+def set_site_anisotropy(p_state, K, normal=(0.0, 0.0, 1.0), idx_image=-1, idx_chain=-1):
+    if K is None:
+        _Set_Site_Anisotropy(ctypes.c_void_p(p_state), None, _F3(*normal), _i(idx_image), _i(idx_chain))
+        return
+    arr = np.ascontiguousarray(K, dtype=np.float64)
+    _Set_Site_Anisotropy(ctypes.c_void_p(p_state), arr.ctypes.data_as(_PD), _F3(*normal), _i(idx_image),
+                         _i(idx_chain))
+
+
+def gradient_and_energy(p_state, nos, idx_image=-1, idx_chain=-1):
+    g = np.zeros((nos, 3))
+    e = _d(0)
+    rc = _Gradient_and_Energy(ctypes.c_void_p(p_state), g.ctypes.data_as(_PD), ctypes.byref(e), _i(idx_image),
+                              _i(idx_chain))
+    if rc != 0:
+        raise RuntimeError("Spirit_HIP_Gradient_and_Energy failed")
+    return g, e.value
+
+
+def last_step_ms(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Last_Step_Time(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def device_name():
+    return bytes(_Device_Name()).decode()

@@ -1,0 +1,84 @@
+"""Parameters of the LLG method (core/python/spirit/parameters/llg.py)."""
+import ctypes
+
+from spirit_amd import spiritlib
+
+_spirit = spiritlib.load_spirit_library()
+_vp, _i, _b, _f = ctypes.c_void_p, ctypes.c_int, ctypes.c_bool, ctypes.c_float
+_F3 = ctypes.c_float * 3
+
+
+def _fn(name, argtypes, restype=None):
+    f = getattr(_spirit, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    return f
+
+
+_Set_N_Iterations = _fn("Parameters_LLG_Set_N_Iterations", [_vp, _i, _i, _i, _i])
+_Set_Direct_Minimization = _fn("Parameters_LLG_Set_Direct_Minimization", [_vp, _b, _i, _i])
+_Set_Convergence = _fn("Parameters_LLG_Set_Convergence", [_vp, _f, _i, _i])
+_Set_Time_Step = _fn("Parameters_LLG_Set_Time_Step", [_vp, _f, _i, _i])
+_Set_Damping = _fn("Parameters_LLG_Set_Damping", [_vp, _f, _i, _i])
+_Set_Temperature = _fn("Parameters_LLG_Set_Temperature", [_vp, _f, _i, _i])
+_Set_Output_General = _fn("Parameters_LLG_Set_Output_General", [_vp, _b, _b, _b, _i, _i])
+_Get_N_Iterations = _fn("Parameters_LLG_Get_N_Iterations", [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), _i, _i])
+_Get_Direct_Minimization = _fn("Parameters_LLG_Get_Direct_Minimization", [_vp, _i, _i], _b)
+_Get_Convergence = _fn("Parameters_LLG_Get_Convergence", [_vp, _i, _i], _f)
+_Get_Time_Step = _fn("Parameters_LLG_Get_Time_Step", [_vp, _i, _i], _f)
+_Get_Damping = _fn("Parameters_LLG_Get_Damping", [_vp, _i, _i], _f)
+_Get_Temperature = _fn("Parameters_LLG_Get_Temperature", [_vp, _i, _i], _f)
+
+
+def set_iterations(p_state, n_iterations, n_iterations_log, idx_image=-1, idx_chain=-1):
+    _Set_N_Iterations(ctypes.c_void_p(p_state), _i(n_iterations), _i(n_iterations_log), _i(idx_image), _i(idx_chain))
+
+
+def set_direct_minimization(p_state, use_minimization, idx_image=-1, idx_chain=-1):
+    _Set_Direct_Minimization(ctypes.c_void_p(p_state), _b(use_minimization), _i(idx_image), _i(idx_chain))
+
+
+def set_convergence(p_state, convergence, idx_image=-1, idx_chain=-1):
+    _Set_Convergence(ctypes.c_void_p(p_state), _f(convergence), _i(idx_image), _i(idx_chain))
+
+
+def set_timestep(p_state, dt, idx_image=-1, idx_chain=-1):
+    _Set_Time_Step(ctypes.c_void_p(p_state), _f(dt), _i(idx_image), _i(idx_chain))
+
+
+def set_damping(p_state, damping, idx_image=-1, idx_chain=-1):
+    _Set_Damping(ctypes.c_void_p(p_state), _f(damping), _i(idx_image), _i(idx_chain))
+
+
+def set_temperature(p_state, temperature, idx_image=-1, idx_chain=-1):
+    _Set_Temperature(ctypes.c_void_p(p_state), _f(temperature), _i(idx_image), _i(idx_chain))
+
+
+def set_output_general(p_state, any=True, initial=False, final=False, idx_image=-1, idx_chain=-1):
+    _Set_Output_General(ctypes.c_void_p(p_state), _b(any), _b(initial), _b(final), _i(idx_image), _i(idx_chain))
+
+
+def get_iterations(p_state, idx_image=-1, idx_chain=-1):
+    n, nl = _i(), _i()
+    _Get_N_Iterations(ctypes.c_void_p(p_state), ctypes.byref(n), ctypes.byref(nl), _i(idx_image), _i(idx_chain))
+    return int(n.value), int(nl.value)
+
+
+def get_direct_minimization(p_state, idx_image=-1, idx_chain=-1):
+    return bool(_Get_Direct_Minimization(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_convergence(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Convergence(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_timestep(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Time_Step(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_damping(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Damping(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_temperature(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Temperature(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))

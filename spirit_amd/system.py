@@ -1,0 +1,63 @@
+"""System: one image's spins, effective field and energy (core/python/spirit/system.py)."""
+import ctypes
+
+import numpy as np
+
+from spirit_amd import spiritlib
+
+_spirit = spiritlib.load_spirit_library()
+_vp, _i, _b, _f, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_bool, ctypes.c_float, ctypes.c_double
+
+
+def _fn(name, argtypes, restype=None):
+    f = getattr(_spirit, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    return f
+
+
+_Get_Index = _fn("System_Get_Index", [_vp], _i)
+_Get_NOS = _fn("System_Get_NOS", [_vp, _i, _i], _i)
+_Get_Spin_Directions = _fn("System_Get_Spin_Directions", [_vp, _i, _i], ctypes.POINTER(_d))
+_Get_Effective_Field = _fn("System_Get_Effective_Field", [_vp, _i, _i], ctypes.POINTER(_d))
+_Get_Energy = _fn("System_Get_Energy", [_vp, _i, _i], _f)
+_Get_Rx = _fn("System_Get_Rx", [_vp, _i, _i], _f)
+_Update_Data = _fn("System_Update_Data", [_vp, _i, _i])
+_Print_Energy_Array = _fn("System_Print_Energy_Array", [_vp, _i, _i])
+
+
+def get_index(p_state):
+    return int(_Get_Index(ctypes.c_void_p(p_state)))
+
+
+def get_nos(p_state, idx_image=-1, idx_chain=-1):
+    return int(_Get_NOS(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_spin_directions(p_state, idx_image=-1, idx_chain=-1):
+    """A zero-copy (nos, 3) view of the image's spins; writes are seen by the next method start."""
+    nos = get_nos(p_state, idx_image, idx_chain)
+    ptr = _Get_Spin_Directions(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
+    return np.ctypeslib.as_array(ptr, shape=(nos, 3))
+
+
+def get_effective_field(p_state, idx_image=-1, idx_chain=-1):
+    nos = get_nos(p_state, idx_image, idx_chain)
+    ptr = _Get_Effective_Field(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
+    return np.ctypeslib.as_array(ptr, shape=(nos, 3))
+
+
+def get_energy(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Energy(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def get_rx(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Rx(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+def update_data(p_state, idx_image=-1, idx_chain=-1):
+    _Update_Data(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
+
+
+def print_energy_array(p_state, idx_image=-1, idx_chain=-1):
+    _Print_Energy_Array(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
