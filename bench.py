@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the HIP spin-dynamics engine on one GPU.
+
+Workload (default): Depondt LLG with dipole-dipole interaction by zero-padded FFT
+convolution on a 256x256x128 simple-cubic lattice (open boundaries, padded grid
+512x512x256), J=5 meV, Bloch DMI 1 meV, per-site uniaxial K_i ~ U[0.02, 0.1] meV
+along z, B=0.2 T along z, mu_s=2, dt=5e-4 ps, alpha=0.3, T=0, FP64, random initial
+spins from a seed.  A "step" is one full Depondt iteration (two force evaluations,
+each with the FFT convolution) on the spins already resident in HBM.
+
+Prints one JSON line: spin-steps/s, the roofline of the dominant kernel (the DDI
+convolution pass, timed live with HIP events on the engine's stream) and the CPU
+oracle timed on the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "cube256_depondt_ddi": dict(cfg="CUBE_256", solver=2, site_K=True, dims=(256, 256, 128)),
+    "film128_heun_ddi": dict(cfg="FILM_128", solver=3, site_K=False, dims=(128, 128, 32)),
+    "square64_depondt": dict(cfg="SQUARE_64", solver=2, site_K=False, dims=(64, 64, 1)),
+}
+
+
+def alg_bytes_per_spin_step(dims, solver, ddi):
+    """Algorithmic HBM bytes per spin-step (FP64, SoA): every distinct array read once and
+    written once per stage; the DDI convolution as the zero-padding-aware 3-pass dataflow."""
+    local = {2: 168.0, 3: 168.0, 1: 120.0, 4: 408.0}[solver]
+    n_fe = 4 if solver == 4 else 2
+    if not ddi:
+        return local, 0.0
+    Na, Nb, Nc = dims
+    Pa, Pb, Pc = 2 * Na if Na > 1 else 1, 2 * Nb if Nb > 1 else 1, 2 * Nc if Nc > 1 else 1
+    Ha = Pa // 2 + 1
+    N = Na * Nb * Nc
+    H_NN, H_PN, H_PP = Ha * Nb * Nc, Ha * Pb * Nc, Ha * Pb * Pc
+    A = 24 * N + 48 * H_NN
+    B = 48 * H_NN + 48 * H_PN
+    C = 96 * H_PN + 96 * H_PP
+    per_fe = (2 * A + 2 * B + C) / N
+    return local + n_fe * (per_fe + 24.0), C
+
+
+def run_gpu(args, wl):
+    from spirit_amd import configs, hip, simulation, state, system
+    cfg = dict(getattr(configs, wl["cfg"]))
+    Na, Nb, Nc = wl["dims"]
+    nos = Na * Nb * Nc
+    path = os.path.join(tempfile.mkdtemp(), "bench.cfg")
+    configs.write_config(cfg, path)
+    p = state.setup(path, True)
+    if not p:
+        raise RuntimeError("State_Setup failed")
+    if wl["site_K"]:
+        hip.set_site_anisotropy(p, configs.site_anisotropy(nos, 20006), (0.0, 0.0, 1.0))
+    system.get_spin_directions(p)[:] = configs.random_spins(nos, 20006)
+    hip.set_host_sync(p, 0)
+    t0 = time.perf_counter()
+    simulation.start(p, simulation.METHOD_LLG, wl["solver"], single_shot=True)
+    setup_s = time.perf_counter() - t0
+    print("bench: setup %.2f s, device %s" % (setup_s, hip.device_name()), file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        simulation.n_shot(p, args.warmup)
+    hip.profile_enable(p, True)
+    hip.profile_reset(p)
+    t0 = time.perf_counter()
+    simulation.n_shot(p, args.steps)  # returns after the stream has drained
+    t1 = time.perf_counter()
+    conv_ms, conv_n = hip.profile_get(p, "ddi_conv")
+    stage_ms, stage_n = hip.profile_get(p, "llg_stage")
+    print("bench: %d steps in %.3f s" % (args.steps, t1 - t0), file=sys.stderr, flush=True)
+    s = system.get_spin_directions(p)
+    import numpy as np
+    norm_err = float(np.abs(np.linalg.norm(s, axis=1) - 1).max())
+    simulation.stop(p)
+    state.delete(p)
+    return dict(seconds=t1 - t0, nos=nos, conv_ms=conv_ms, conv_n=conv_n, stage_ms=stage_ms, stage_n=stage_n,
+                norm_err=norm_err, setup_s=setup_s)
+
+
+def cpu_baseline(wl, threads, steps):
+    """The CPU oracle (C + OpenMP restatement of the reference algorithm) on the host cores,
+    timed on `steps` steps of the same workload after its set-up."""
+    code = r"""
+import sys, time, json
+sys.path[:0] = [%r, %r]
+import oracle as O
+from spirit_amd import configs
+cfg = dict(getattr(configs, %r)); n = %d
+K = configs.site_anisotropy(n, 20006) if %r else None
+S = O.System(cfg, site_K=K)
+s = configs.random_spins(n, 20006)
+t0 = time.perf_counter()
+s, e, t, _ = S.llg(s, %d, %d, dt=cfg["llg_dt"], damping=cfg["llg_damping"], temperature=cfg["llg_temperature"], seed=cfg["llg_seed"])
+print(json.dumps({"seconds": time.perf_counter() - t0}))
+""" % (ROOT, os.path.join(ROOT, "oracle"), wl["cfg"], wl["dims"][0] * wl["dims"][1] * wl["dims"][2], wl["site_K"],
+       wl["solver"], steps)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=1200)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])["seconds"]
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cube256_depondt_ddi", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    wl = WORKLOADS[args.workload]
+    ddi = args.workload != "square64_depondt"
+    g = run_gpu(args, wl)
+    value = g["nos"] * args.steps / g["seconds"]
+    b_alg, conv_bytes = alg_bytes_per_spin_step(wl["dims"], wl["solver"], ddi)
+    ms_per_step = 1e3 * g["seconds"] / args.steps
+    out = {
+        "metric": "LLG spin-steps/s (Depondt+DDI) plain vs batched vs adjoint; % HBM peak",
+        "value": value,
+        "unit": "spin-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random spins and per-site anisotropy)",
+        "config": {"workload": args.workload, "lattice": list(wl["dims"]), "solver": {2: "Depondt", 3: "Heun"}.get(
+            wl["solver"], str(wl["solver"])), "ddi": "fft" if ddi else "none", "spins": g["nos"],
+            "alg_bytes_per_spin_step": b_alg, "hbm_frac_of_step": b_alg * value / (HBM_PEAK_GBS * 1e9)},
+    }
+    if g["conv_n"] > 0:
+        t = g["conv_ms"] / g["conv_n"] * 1e-3
+        ach = conv_bytes / t / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass_c_conv",
+                           "alg_bytes_per_launch": conv_bytes, "avg_launch_ms": t * 1e3}
+    else:
+        out["roofline"] = None
+    out["kernel_ms"] = {"ddi_conv_avg": g["conv_ms"] / max(1, g["conv_n"]),
+                        "llg_stage_avg": g["stage_ms"] / max(1, g["stage_n"])}
+    out["max_norm_error"] = g["norm_err"]
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            sec = cpu_baseline(wl, threads, args.cpu_steps)
+            out["cpu_baseline"] = {"value": g["nos"] * args.cpu_steps / sec, "unit": "spin-steps/s", "cores": threads,
+                                   "kind": "port", "sample": "%d step(s) of the same workload after set-up, CPU %s" % (
+                                       args.cpu_steps, cpu_model())}
+        except Exception as e:  # reported, never fatal for the GPU number
+            out["cpu_baseline"] = {"value": None, "unit": "spin-steps/s", "cores": threads, "kind": "port",
+                                   "sample": "failed: %s" % str(e)[-300:]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

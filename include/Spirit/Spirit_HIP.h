@@ -36,6 +36,16 @@ SPIRIT_API int Spirit_HIP_Gradient_and_Energy( State * state, double * gradient,
 SPIRIT_API double Spirit_HIP_Get_Last_Step_Time( State * state, int idx_image SPIRIT_DEFAULT( -1 ),
                                                  int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* For benchmarks: time kernels of the image's LLG method with HIP events on its stream.
+ * Kernel names: "ddi_conv" (DDI convolution pass), "llg_stage" (fused field + solver stages).
+ * Profile_Get returns the summed milliseconds and the launch count since the last reset. */
+SPIRIT_API void Spirit_HIP_Profile_Enable( State * state, bool enable, int idx_image SPIRIT_DEFAULT( -1 ),
+                                           int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+SPIRIT_API void Spirit_HIP_Profile_Get( State * state, const char * kernel, double * total_ms, int * count,
+                                        int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+SPIRIT_API void Spirit_HIP_Profile_Reset( State * state, int idx_image SPIRIT_DEFAULT( -1 ),
+                                          int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
 /* Name of the GPU the engine runs on (empty if none is visible). */
 SPIRIT_API const char * Spirit_HIP_Device_Name( void ) SPIRIT_NOEXCEPT;
 

@@ -1322,6 +1322,50 @@ catch( ... )
     return 0;
 }
 
+static LLGRunner * runner_of( State * state, int idx_image )
+{
+    auto it = state->method_image.find( idx_image );
+    if( it == state->method_image.end() || !it->second || !it->second->runner )
+        throw api_error( 2, "no running LLG method on image " + std::to_string( idx_image ) );
+    return it->second->runner.get();
+}
+
+extern "C" void Spirit_HIP_Profile_Enable( State * state, bool enable, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    runner_of( state, idx_image )->set_profiling( enable );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Spirit_HIP_Profile_Get( State * state, const char * kernel, double * total_ms, int * count,
+                                        int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    auto e    = runner_of( state, idx_image )->profile( kernel ? kernel : "" );
+    *total_ms = e.ms;
+    *count    = (int)e.count;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Spirit_HIP_Profile_Reset( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    runner_of( state, idx_image )->reset_profile();
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
 extern "C" const char * Spirit_HIP_Device_Name() noexcept
 {
     static std::string name;

@@ -34,7 +34,9 @@ struct DipoleArgs
 int fft_lines_per_block( const FftLen & L );
 
 // h = mu_i * (D * (mu s))_i / M for the spins in `spins` (SoA, nos), written to h (SoA).
-void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int nos, hipStream_t st );
+// evc0/evc1 (nullable) are recorded around the convolution pass (pass C) for profiling.
+void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int nos, hipStream_t st,
+                       hipEvent_t evc0 = nullptr, hipEvent_t evc1 = nullptr );
 // Setup of the kernel spectrum.
 void ddi_launch_kernel_spectrum( const DdiDev & d, const DipoleArgs & g, double * work, double2 * work2,
                                  double2 * Dh, hipStream_t st );

@@ -391,7 +391,8 @@ void ddi_launch_pass_a_fwd_spins( const DdiDev & d, const double * spins, int no
                         nos, n_arr, d.Na, d.Nb, d.Nc, d.Pb, d.Nc, d.X, lpb );
 }
 
-void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int nos, hipStream_t st )
+void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int nos, hipStream_t st,
+                       hipEvent_t evc0, hipEvent_t evc1 )
 {
     const int n_arr = 3 * d.nb;
     ddi_launch_pass_a_fwd_spins( d, spins, nos, st );
@@ -408,8 +409,12 @@ void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int n
             tk = 1;
         const int tiles  = ( d.Ha + tk - 1 ) / tk;
         const int blocks = tiles * d.Pb;
+        if( evc0 )
+            (void)hipEventRecord( evc0, st );
         hipLaunchKernelGGL( k_pass_c_conv, dim3( blocks ), dim3( FFT_THREADS ), lds_bytes( n_arr * tk, d.Lc.n ), st,
                             d, d.X, tk );
+        if( evc1 )
+            (void)hipEventRecord( evc1, st );
     }
     {
         const int tk     = fft_lines_per_block( d.Lb );

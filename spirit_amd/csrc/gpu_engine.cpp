@@ -161,9 +161,10 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
     HIPCHECK( hipStreamSynchronize( st ) );
 }
 
-void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st )
+void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0,
+                     hipEvent_t evc1 )
 {
-    ddi_launch_field( d, spins_soa, h_soa, nos, st );
+    ddi_launch_field( d, spins_soa, h_soa, nos, st, evc0, evc1 );
 }
 
 // ---------------------------------------------------------------------------
@@ -358,8 +359,65 @@ LLGRunner::~LLGRunner()
     if( stream )
     {
         (void)hipStreamSynchronize( stream );
+        for( auto e : ev_pool )
+            (void)hipEventDestroy( e );
         (void)hipStreamDestroy( stream );
     }
+}
+
+void LLGRunner::set_profiling( bool on )
+{
+    profiling = on;
+}
+
+hipEvent_t LLGRunner::next_event()
+{
+    if( ev_used == ev_pool.size() )
+    {
+        hipEvent_t e;
+        HIPCHECK( hipEventCreate( &e ) );
+        ev_pool.push_back( e );
+    }
+    return ev_pool[ev_used++];
+}
+
+void LLGRunner::collect_profile()
+{
+    for( auto & p : pending )
+    {
+        float ms = 0;
+        HIPCHECK( hipEventElapsedTime( &ms, p.second.first, p.second.second ) );
+        bool found = false;
+        for( auto & e : prof )
+            if( e.first == p.first )
+            {
+                e.second.ms += ms;
+                e.second.count += 1;
+                found = true;
+            }
+        if( !found )
+        {
+            ProfileEntry pe;
+            pe.ms    = ms;
+            pe.count = 1;
+            prof.push_back( { p.first, pe } );
+        }
+    }
+    pending.clear();
+    ev_used = 0;
+}
+
+LLGRunner::ProfileEntry LLGRunner::profile( const std::string & name ) const
+{
+    for( auto & e : prof )
+        if( e.first == name )
+            return e.second;
+    return ProfileEntry{};
+}
+
+void LLGRunner::reset_profile()
+{
+    prof.clear();
 }
 
 void LLGRunner::upload_images()
@@ -445,13 +503,20 @@ void LLGRunner::set_injected_noise( const double * noise, long n_steps )
     noise_pos   = 0;
 }
 
-void LLGRunner::ddi_field( const double * x )
+void LLGRunner::ddi_field( const double * x, bool prof_events )
 {
     for( size_t k = 0; k < specs.size(); ++k )
         if( ddi[k] )
         {
             const long long base = 3 * imgs_host[k].off;
-            ddi[k]->field( x + base, b_h.as<double>() + base, specs[k].geometry->nos, stream );
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if( prof_events )
+            {
+                e0 = next_event();
+                e1 = next_event();
+                pending.push_back( { "ddi_conv", { e0, e1 } } );
+            }
+            ddi[k]->field( x + base, b_h.as<double>() + base, specs[k].geometry->nos, stream, e0, e1 );
         }
 }
 
@@ -497,8 +562,17 @@ void LLGRunner::step( bool obs, bool count, bool check, double conv )
     {
         const double * x = st == 0 ? A.s : ( ( solver == SOLVER_RK4 && st == 2 ) ? A.sp2 : A.sp );
         if( any_ddi )
-            ddi_field( x );
-        llg_launch_stage( solver, st, A, n_blocks, stream );
+            ddi_field( x, profiling );
+        if( profiling )
+        {
+            hipEvent_t e0 = next_event(), e1 = next_event();
+            HIPCHECK( hipEventRecord( e0, stream ) );
+            llg_launch_stage( solver, st, A, n_blocks, stream );
+            HIPCHECK( hipEventRecord( e1, stream ) );
+            pending.push_back( { "llg_stage", { e0, e1 } } );
+        }
+        else
+            llg_launch_stage( solver, st, A, n_blocks, stream );
     }
     if( obs )
         llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_stopstep.as<long long>(),
@@ -529,6 +603,8 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
     HIPCHECK( hipStreamSynchronize( stream ) );
     auto t1      = std::chrono::steady_clock::now();
     last_step_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count() / (double)n;
+    if( !pending.empty() )
+        collect_profile();
     long done    = n;
     if( stopped )
     {

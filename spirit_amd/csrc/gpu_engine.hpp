@@ -59,7 +59,8 @@ class DdiPlan
 public:
     // throws std::runtime_error for unsupported grids
     void build( const Geometry & g, const Hamiltonian & h, hipStream_t st );
-    void field( const double * spins_soa, double * h_soa, int nos, hipStream_t st );
+    void field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0 = nullptr,
+                hipEvent_t evc1 = nullptr );
     const DdiDev & dev() const
     {
         return d;
@@ -146,9 +147,28 @@ public:
         return b_s.as<double>();
     }
 
+    // Kernel timing with HIP events on the runner's stream: "ddi_conv" (the convolution
+    // pass of the DDI field) and "llg_stage" (the fused field + solver stage kernels).
+    struct ProfileEntry
+    {
+        double ms  = 0;
+        long count = 0;
+    };
+    void set_profiling( bool on );
+    ProfileEntry profile( const std::string & name ) const;
+    void reset_profile();
+
 private:
+    hipEvent_t next_event();
+    void collect_profile();
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<std::pair<std::string, ProfileEntry>> prof;
+
     void step( bool obs, bool count, bool check, double conv );
-    void ddi_field( const double * x );
+    void ddi_field( const double * x, bool prof_events = false );
     void upload_images();
 
     std::vector<ImageSpec> specs;

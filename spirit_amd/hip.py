@@ -24,6 +24,25 @@ _Set_Site_Anisotropy = _fn("Spirit_HIP_Hamiltonian_Set_Site_Anisotropy", [_vp, _
 _Gradient_and_Energy = _fn("Spirit_HIP_Gradient_and_Energy", [_vp, _PD, _PD, _i, _i], _i)
 _Get_Last_Step_Time = _fn("Spirit_HIP_Get_Last_Step_Time", [_vp, _i, _i], _d)
 _Device_Name = _fn("Spirit_HIP_Device_Name", [], ctypes.c_char_p)
+_Profile_Enable = _fn("Spirit_HIP_Profile_Enable", [_vp, ctypes.c_bool, _i, _i])
+_Profile_Get = _fn("Spirit_HIP_Profile_Get", [_vp, ctypes.c_char_p, _PD, ctypes.POINTER(_i), _i, _i])
+_Profile_Reset = _fn("Spirit_HIP_Profile_Reset", [_vp, _i, _i])
+
+
+def profile_enable(p_state, enable=True, idx_image=-1, idx_chain=-1):
+    _Profile_Enable(ctypes.c_void_p(p_state), ctypes.c_bool(enable), _i(idx_image), _i(idx_chain))
+
+
+def profile_get(p_state, kernel, idx_image=-1, idx_chain=-1):
+    """(total milliseconds, launch count) of a kernel class since the last reset."""
+    ms, n = _d(0), _i(0)
+    _Profile_Get(ctypes.c_void_p(p_state), ctypes.c_char_p(kernel.encode()), ctypes.byref(ms), ctypes.byref(n),
+                 _i(idx_image), _i(idx_chain))
+    return ms.value, n.value
+
+
+def profile_reset(p_state, idx_image=-1, idx_chain=-1):
+    _Profile_Reset(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
 
 
 def set_host_sync(p_state, mode):
