@@ -46,6 +46,11 @@ SPIRIT_API void Spirit_HIP_Profile_Get( State * state, const char * kernel, doub
 SPIRIT_API void Spirit_HIP_Profile_Reset( State * state, int idx_image SPIRIT_DEFAULT( -1 ),
                                           int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* Diagnostics: describes the DDI plan of the image's running LLG method (padded grid,
+ * storage format of the kernel spectrum) into buffer; returns 1 if the image has one. */
+SPIRIT_API int Spirit_HIP_DDI_Info( State * state, char * buffer, int length, int idx_image SPIRIT_DEFAULT( -1 ),
+                                    int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
 /* Name of the GPU the engine runs on (empty if none is visible). */
 SPIRIT_API const char * Spirit_HIP_Device_Name( void ) SPIRIT_NOEXCEPT;
 

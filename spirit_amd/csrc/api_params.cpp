@@ -4,6 +4,7 @@
 #include "api_common.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 using namespace spirit_hip;
@@ -1364,6 +1365,30 @@ try
 catch( ... )
 {
     handle_exception( idx_image, idx_chain );
+}
+
+extern "C" int Spirit_HIP_DDI_Info( State * state, char * buffer, int length, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    const DdiPlan * plan = runner_of( state, idx_image )->ddi_plan( 0 );
+    if( !plan )
+        return 0;
+    const DdiDev & d = plan->dev();
+    char msg[256];
+    std::snprintf( msg, sizeof( msg ), "padded %dx%dx%d; spectrum %s; stored %dx%dx%d", d.Pa, d.Pb, d.Pc,
+                   plan->spectrum_format(), d.KB, d.Ha, d.KC );
+    if( buffer && length > 0 )
+    {
+        std::strncpy( buffer, msg, (size_t)length - 1 );
+        buffer[length - 1] = 0;
+    }
+    return 1;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
 }
 
 extern "C" const char * Spirit_HIP_Device_Name() noexcept

@@ -91,12 +91,16 @@ struct ImageDev
 };
 
 // One FFT length: Stockham stage radices and the n-th roots of unity exp(-2 pi i e / n).
+// For powers of two >= 8, tw2 holds the twiddles of the register-resident radix-8
+// plan in stage order: stage s (>= 1, radix R, Ns = 8^s) at offset 8^s - 8,
+// entry jm*(R-1) + r-1 = w_n^(r * jm * n/(Ns*R)).
 struct FftLen
 {
     int n;
     int n_stages;
     int radix[MAX_FFT_STAGES];
     const double2 * W;
+    const double2 * tw2;
 };
 
 } // namespace spirit_hip

@@ -1,0 +1,105 @@
+// Register-resident Stockham FFT for power-of-two lengths N >= 8 (FP64 complex).
+//
+// A line of N points is owned by T = N/8 threads; thread t holds x[t + T*k],
+// k = 0..7, in registers.  Stages are radix 8 (a final radix 2 or 4 when log2 N
+// is not a multiple of 3); a radix-R stage is 8/R butterflies per thread, so the
+// register set is always the same eight values.  Between stages the values pass
+// through LDS at their Stockham output positions (index padded by one slot every
+// eight, which spreads the strided writes over the banks).  Sizes are
+// compile-time constants so all index arithmetic folds to shifts.
+#pragma once
+
+#include "fft_lds.hpp"
+
+namespace spirit_hip
+{
+
+constexpr int ilog2c( int n )
+{
+    return n <= 1 ? 0 : 1 + ilog2c( n / 2 );
+}
+constexpr int ipow8c( int s )
+{
+    return s == 0 ? 1 : 8 * ipow8c( s - 1 );
+}
+__device__ __forceinline__ constexpr int pad8( int i )
+{
+    return i + ( i >> 3 );
+}
+
+template<int N>
+struct P2
+{
+    static_assert( N >= 8 && ( N & ( N - 1 ) ) == 0, "power of two >= 8" );
+    static constexpr int T   = N / 8;
+    static constexpr int LOG = ilog2c( N );
+    static constexpr int N8  = LOG / 3;
+    static constexpr int REM = LOG % 3;
+    static constexpr int NST = N8 + ( REM ? 1 : 0 );
+    static constexpr int LS  = N + N / 8 + 1; // padded LDS line stride (complex values)
+};
+
+template<int N, int S>
+struct P2Stage
+{
+    static constexpr int R     = ( S < P2<N>::N8 ) ? 8 : ( P2<N>::REM == 1 ? 2 : 4 );
+    static constexpr int Ns    = ipow8c( S );
+    static constexpr int TWOFF = ipow8c( S ) - 8; // valid for S >= 1
+};
+
+// Runs stages S.. of the transform.  On entry v holds this thread's inputs of stage S;
+// on exit the transform is in L[pad8(i)] in natural order (block-synchronised).
+template<int N, bool INV, int S>
+__device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restrict__ L,
+                                          const double2 * __restrict__ tw, int t )
+{
+    using St           = P2Stage<N, S>;
+    constexpr int R    = St::R;
+    constexpr int Ns   = St::Ns;
+    constexpr int M    = 8 / R;
+    constexpr int T    = P2<N>::T;
+#pragma unroll
+    for( int m = 0; m < M; ++m )
+    {
+        const int j  = t + T * m;
+        const int jm = j & ( Ns - 1 );
+        double2 u[R];
+#pragma unroll
+        for( int r = 0; r < R; ++r )
+            u[r] = v[m + M * r];
+        if constexpr( S > 0 )
+        {
+#pragma unroll
+            for( int r = 1; r < R; ++r )
+            {
+                double2 w = tw[St::TWOFF + jm * ( R - 1 ) + ( r - 1 )];
+                if( INV )
+                    w.y = -w.y;
+                u[r] = cmul( u[r], w );
+            }
+        }
+        dft_small<R, INV>( u, nullptr, N );
+        const int idxD = ( j / Ns ) * Ns * R + jm;
+#pragma unroll
+        for( int r = 0; r < R; ++r )
+            L[pad8( idxD + r * Ns )] = u[r];
+    }
+    __syncthreads();
+    if constexpr( S + 1 < P2<N>::NST )
+    {
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+            v[k] = L[pad8( t + T * k )];
+        __syncthreads();
+        p2_stage<N, INV, S + 1>( v, L, tw, t );
+    }
+}
+
+template<int N, bool INV>
+__device__ __forceinline__ void p2_fft( double2 ( &v )[8], double2 * __restrict__ L, const double2 * __restrict__ tw,
+                                        int t )
+{
+    p2_stage<N, INV, 0>( v, L, tw, t );
+}
+
+} // namespace spirit_hip

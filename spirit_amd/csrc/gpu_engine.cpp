@@ -3,6 +3,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 namespace spirit_hip
@@ -75,7 +76,36 @@ void FftLenHost::build( int n )
     }
     W.resize( sizeof( double2 ) * n );
     HIPCHECK( hipMemcpy( W.as<void>(), w.data(), sizeof( double2 ) * n, hipMemcpyHostToDevice ) );
-    dev.W = W.as<double2>();
+    dev.W   = W.as<double2>();
+    dev.tw2 = nullptr;
+
+    // Power-of-two plan: radix-8 stages then one radix-2/4 stage.  Stage s >= 1 has
+    // Ns = 8^s and stores, at offset 8^s - 8, w_n^(r*jm*n/(Ns*R)) at jm*(R-1) + r-1.
+    if( n >= 8 && ( n & ( n - 1 ) ) == 0 )
+    {
+        int lg = 0;
+        while( ( 1 << lg ) < n )
+            ++lg;
+        const int n8  = lg / 3;
+        const int rem = lg % 3;
+        const int nst = n8 + ( rem ? 1 : 0 );
+        std::vector<double2> t;
+        int Ns = 8;
+        for( int st = 1; st < nst; ++st, Ns *= 8 )
+        {
+            const int R = st < n8 ? 8 : ( rem == 1 ? 2 : 4 );
+            if( (int)t.size() != Ns - 8 )
+                throw std::runtime_error( "DDI FFT: twiddle table layout" );
+            for( int jm = 0; jm < Ns; ++jm )
+                for( int r = 1; r < R; ++r )
+                    t.push_back( w[( (long long)r * jm * ( n / ( Ns * R ) ) ) % n] );
+        }
+        if( t.empty() )
+            t.push_back( make_double2( 1.0, 0.0 ) );
+        TW2.resize( sizeof( double2 ) * t.size() );
+        HIPCHECK( hipMemcpy( TW2.as<void>(), t.data(), sizeof( double2 ) * t.size(), hipMemcpyHostToDevice ) );
+        dev.tw2 = TW2.as<double2>();
+    }
 }
 
 std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h )
@@ -96,6 +126,19 @@ size_t DdiPlan::bytes() const
     return X.size() + Dh.size();
 }
 
+const char * DdiPlan::spectrum_format() const
+{
+    if( d.dh_real && d.sym_b && d.sym_c )
+        return "real, mirror-symmetric in b and c";
+    if( d.dh_real && d.sym_b )
+        return "real, mirror-symmetric in b";
+    if( d.dh_real && d.sym_c )
+        return "real, mirror-symmetric in c";
+    if( d.dh_real )
+        return "real";
+    return "complex";
+}
+
 void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
 {
     auto P = ddi_padded_dims( g, h );
@@ -113,7 +156,14 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
     d.Pb  = P[1];
     d.Pc  = P[2];
     d.Ha  = P[0] / 2 + 1;
+    d.Hs  = ( d.Ha + 7 ) / 8 * 8;
     d.M   = (double)P[0] * (double)P[1] * (double)P[2];
+    {
+        int dev_id = 0, lds = 0;
+        HIPCHECK( hipGetDevice( &dev_id ) );
+        HIPCHECK( hipDeviceGetAttribute( &lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id ) );
+        d.lds_max = lds;
+    }
     d.La  = La.dev;
     d.Lb  = Lb.dev;
     d.Lc  = Lc.dev;
@@ -133,12 +183,21 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
     for( int b = 0; b < g.nb; ++b )
         d.mu[b] = g.mu_basis[b];
 
-    const size_t nX  = (size_t)3 * g.nb * d.Nc * (size_t)d.Pb * d.Ha;
-    const size_t nDh = (size_t)d.Pb * d.Ha * (size_t)d.Pc * 6 * d.n_inter;
+    if( const char * why = ddi_plan_problem( d ) )
+        throw std::runtime_error( std::string( "DDI: " ) + why );
+
+    const size_t nX = (size_t)3 * g.nb * d.Nc * (size_t)d.Pb * d.Hs;
     X.resize( sizeof( double2 ) * nX );
-    Dh.resize( sizeof( double2 ) * nDh );
-    d.X  = X.as<double2>();
-    d.Dh = Dh.as<double2>();
+    d.X = X.as<double2>();
+    // component signs of D under b -> -b and c -> -c for an orthogonal lattice
+    // (xx, xy, xz, yy, yz, zz); used only where the spectrum shows them.
+    const int sb[6] = { +1, -1, +1, +1, -1, +1 };
+    const int sc[6] = { +1, +1, -1, +1, -1, +1 };
+    for( int q = 0; q < 6; ++q )
+    {
+        d.sig_b[q] = sb[q];
+        d.sig_c[q] = sc[q];
+    }
 
     DipoleArgs a{};
     a.Na = d.Na, a.Nb = d.Nb, a.Nc = d.Nc, a.nb = d.nb;
@@ -153,12 +212,42 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
         for( int c = 0; c < 3; ++c )
             a.cell_atoms[b][c] = g.cell_atoms[b][c];
 
-    const size_t M = (size_t)d.Pa * d.Pb * d.Pc;
-    DeviceBuffer work( sizeof( double ) * 6 * d.n_inter * M );
-    DeviceBuffer work2( sizeof( double2 ) * 6 * d.n_inter * (size_t)d.Pc * d.Pb * d.Ha );
-    ddi_launch_kernel_spectrum( d, a, work.as<double>(), work2.as<double2>(), Dh.as<double2>(), st );
+    const size_t M     = (size_t)d.Pa * d.Pb * d.Pc;
+    const size_t nfull = (size_t)d.Pb * d.Ha * (size_t)d.Pc * 6 * d.n_inter;
+    DeviceBuffer full( sizeof( double2 ) * nfull );
+    {
+        DeviceBuffer work( sizeof( double ) * 6 * d.n_inter * M );
+        DeviceBuffer work2( sizeof( double2 ) * 6 * d.n_inter * (size_t)d.Pc * d.Pb * d.Hs );
+        ddi_launch_kernel_spectrum( d, a, work.as<double>(), work2.as<double2>(), full.as<double2>(), st );
+        HIPCHECK( hipGetLastError() );
+        HIPCHECK( hipStreamSynchronize( st ) );
+    }
+    // Compressed storage where the spectrum allows it (decided from the computed values:
+    // deviations below 1e-12 of the largest entry are rounding).
+    DeviceBuffer sym_dev( sizeof( DhatSymmetry ) );
+    ddi_launch_symmetry_check( d, full.as<double2>(), sym_dev.as<DhatSymmetry>(), st );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipMemcpyAsync( &sym, sym_dev.as<void>(), sizeof( DhatSymmetry ), hipMemcpyDeviceToHost, st ) );
+    HIPCHECK( hipStreamSynchronize( st ) );
+    const double tol = 1e-12 * sym.max_abs;
+    d.dh_real        = sym.max_imag <= tol;
+    d.sym_b          = d.dh_real && sym.max_asym_b <= tol && d.Pb > 2;
+    d.sym_c          = d.dh_real && sym.max_asym_c <= tol && d.Pc > 2;
+    d.KB             = d.sym_b ? d.Pb / 2 + 1 : d.Pb;
+    d.KC             = d.sym_c ? d.Pc / 2 + 1 : d.Pc;
+    const size_t nDh = (size_t)d.KB * d.Ha * (size_t)d.KC * 6 * d.n_inter;
+    Dh.resize( ( d.dh_real ? sizeof( double ) : sizeof( double2 ) ) * nDh );
+    d.Dh   = d.dh_real ? nullptr : Dh.as<double2>();
+    d.Dh_r = d.dh_real ? Dh.as<double>() : nullptr;
+    ddi_launch_compress( d, full.as<double2>(), Dh.as<void>(), st );
     HIPCHECK( hipGetLastError() );
     HIPCHECK( hipStreamSynchronize( st ) );
+    char msg[256];
+    std::snprintf( msg, sizeof( msg ),
+                   "DDI plan %dx%dx%d padded, kernel spectrum %s (|Im|/max %.2e, b-mirror %.2e, c-mirror %.2e)", d.Pa,
+                   d.Pb, d.Pc, spectrum_format(), sym.max_imag / sym.max_abs, sym.max_asym_b / sym.max_abs,
+                   sym.max_asym_c / sym.max_abs );
+    log_message( 5, msg );
 }
 
 void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0,

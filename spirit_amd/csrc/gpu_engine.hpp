@@ -49,7 +49,7 @@ private:
 struct FftLenHost
 {
     FftLen dev{};
-    DeviceBuffer W;
+    DeviceBuffer W, TW2;
     void build( int n );
 };
 
@@ -66,9 +66,16 @@ public:
         return d;
     }
     size_t bytes() const;
+    // how the kernel spectrum is stored (real / mirror-compressed / complex)
+    const char * spectrum_format() const;
+    const DhatSymmetry & symmetry() const
+    {
+        return sym;
+    }
 
 private:
     DdiDev d{};
+    DhatSymmetry sym{};
     FftLenHost La, Lb, Lc;
     DeviceBuffer X, Dh;
 };
@@ -157,6 +164,12 @@ public:
     void set_profiling( bool on );
     ProfileEntry profile( const std::string & name ) const;
     void reset_profile();
+
+    // DDI plan of image img (nullptr without DDI)
+    const DdiPlan * ddi_plan( int img ) const
+    {
+        return img >= 0 && img < (int)ddi.size() ? ddi[img].get() : nullptr;
+    }
 
 private:
     hipEvent_t next_event();

@@ -27,6 +27,7 @@ _Device_Name = _fn("Spirit_HIP_Device_Name", [], ctypes.c_char_p)
 _Profile_Enable = _fn("Spirit_HIP_Profile_Enable", [_vp, ctypes.c_bool, _i, _i])
 _Profile_Get = _fn("Spirit_HIP_Profile_Get", [_vp, ctypes.c_char_p, _PD, ctypes.POINTER(_i), _i, _i])
 _Profile_Reset = _fn("Spirit_HIP_Profile_Reset", [_vp, _i, _i])
+_DDI_Info = _fn("Spirit_HIP_DDI_Info", [_vp, ctypes.c_char_p, _i, _i, _i], _i)
 
 
 def profile_enable(p_state, enable=True, idx_image=-1, idx_chain=-1):
@@ -84,3 +85,11 @@ def last_step_ms(p_state, idx_image=-1, idx_chain=-1):
 
 def device_name():
     return bytes(_Device_Name()).decode()
+
+
+def ddi_info(p_state, idx_image=-1, idx_chain=-1):
+    """Description of the DDI plan of the image's running LLG method, or None without DDI."""
+    buf = ctypes.create_string_buffer(256)
+    if not _DDI_Info(ctypes.c_void_p(p_state), buf, _i(256), _i(idx_image), _i(idx_chain)):
+        return None
+    return buf.value.decode()

@@ -311,7 +311,26 @@ def test_full_size_properties_cube(engine):
     assert err < 1e-12, err
     e.set_spins(a)
     simulation.start(e.p, simulation.METHOD_LLG, 2, single_shot=True)
+    info = hip.ddi_info(e.p)
+    print(info)
+    assert "512x512x256" in info and "real, mirror-symmetric in b and c" in info
     simulation.n_shot(e.p, 3)
     s = e.spins()
     simulation.stop(e.p)
     assert np.abs(np.linalg.norm(s, axis=1) - 1).max() < 1e-13
+
+
+@pytest.mark.parametrize("case,expect", [("film_bloch_ddi_open", "real, mirror-symmetric in b and c"),
+                                         ("skewed_two_basis_ddi_periodic_images", "complex")])
+def test_ddi_spectrum_format(engine, case, expect):
+    """The kernel spectrum is stored compressed only where its symmetries hold: an open
+    orthogonal film is even in r (real spectrum, mirror-symmetric); a skewed two-atom basis
+    is not."""
+    from spirit_amd import hip, simulation
+    print_case("ddi format", case)
+    e = engine(GRAD_CASES[case])
+    simulation.start(e.p, simulation.METHOD_LLG, 2, single_shot=True)
+    info = hip.ddi_info(e.p)
+    simulation.stop(e.p)
+    print(info)
+    assert ("spectrum " + expect + ";") in info
