@@ -77,6 +77,7 @@ def run_gpu(args, wl):
     t0 = time.perf_counter()
     simulation.n_shot(p, args.steps)  # returns after the stream has drained
     t1 = time.perf_counter()
+    info = hip.ddi_info(p)
     conv_ms, conv_n = hip.profile_get(p, "ddi_conv")
     stage_ms, stage_n = hip.profile_get(p, "llg_stage")
     print("bench: %d steps in %.3f s" % (args.steps, t1 - t0), file=sys.stderr, flush=True)
@@ -86,7 +87,25 @@ def run_gpu(args, wl):
     simulation.stop(p)
     state.delete(p)
     return dict(seconds=t1 - t0, nos=nos, conv_ms=conv_ms, conv_n=conv_n, stage_ms=stage_ms, stage_n=stage_n,
-                norm_err=norm_err, setup_s=setup_s)
+                norm_err=norm_err, setup_s=setup_s, ddi_info=info)
+
+
+PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1"}
+
+
+def pass_c_bytes(dims, info):
+    """HBM bytes one convolution-pass launch must move: the c < Nc planes of the spin
+    spectrum read and written (96 H_PN) plus one read of the kernel spectrum as stored
+    (8 B per real entry, 16 B per complex one; 6 tensor components)."""
+    import re
+    Na, Nb, Nc = dims
+    Pb = 2 * Nb if Nb > 1 else 1
+    Ha = (2 * Na if Na > 1 else 1) // 2 + 1
+    m = re.search(r"stored (\d+)x(\d+)x(\d+)", info)
+    kb, ha, kc = (int(x) for x in m.groups())
+    entry = 8 if "spectrum real" in info else 16
+    kind = int(re.search(r"pass C kernel (\d+)", info).group(1))
+    return 96 * Ha * Pb * Nc + 6 * entry * kb * ha * kc, PASS_C_KERNELS.get(kind, str(kind))
 
 
 def cpu_baseline(wl, threads, steps):
@@ -161,10 +180,13 @@ def main():
     }
     if g["conv_n"] > 0:
         t = g["conv_ms"] / g["conv_n"] * 1e-3
-        ach = conv_bytes / t / 1e9
+        c_bytes, c_kernel = pass_c_bytes(wl["dims"], g["ddi_info"])
+        ach = c_bytes / t / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass_c_conv",
-                           "alg_bytes_per_launch": conv_bytes, "avg_launch_ms": t * 1e3}
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": c_kernel,
+                           "alg_bytes_per_launch": c_bytes, "dense_spectrum_bytes_per_launch": conv_bytes,
+                           "avg_launch_ms": t * 1e3}
+        out["config"]["ddi_plan"] = g["ddi_info"]
     else:
         out["roofline"] = None
     out["kernel_ms"] = {"ddi_conv_avg": g["conv_ms"] / max(1, g["conv_n"]),

@@ -1376,8 +1376,8 @@ try
         return 0;
     const DdiDev & d = plan->dev();
     char msg[256];
-    std::snprintf( msg, sizeof( msg ), "padded %dx%dx%d; spectrum %s; stored %dx%dx%d", d.Pa, d.Pb, d.Pc,
-                   plan->spectrum_format(), d.KB, d.Ha, d.KC );
+    std::snprintf( msg, sizeof( msg ), "padded %dx%dx%d; spectrum %s; stored %dx%dx%d; pass C kernel %d tile %d (%.3f ms)",
+                   d.Pa, d.Pb, d.Pc, plan->spectrum_format(), d.KB, d.Ha, d.KC, d.c_kind, d.c_tk, plan->pass_c_ms() );
     if( buffer && length > 0 )
     {
         std::strncpy( buffer, msg, (size_t)length - 1 );

@@ -3,6 +3,9 @@
 
 #include "device_types.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace spirit_hip
 {
 
@@ -26,6 +29,15 @@ struct DdiDev
     int KB, KC;
     int sig_b[6], sig_c[6];
     int lds_max; // dynamic LDS bytes a block may use on this device (host-side launch choice)
+    int c_kind;  // pass C kernel (DdiPassC), chosen by timing when the plan is built
+    int c_tk;    // column tile of DDI_C_P2_LDS
+};
+
+enum DdiPassC
+{
+    DDI_C_GENERIC = 0, // mixed-radix LDS kernel (any length)
+    DDI_C_P2_LDS  = 1, // power of two, all 3*nb lines of c_tk columns held in LDS
+    DDI_C_P2_REG  = 2, // power of two, one basis atom, real spectrum: components held in registers
 };
 
 struct DipoleArgs
@@ -46,6 +58,9 @@ struct DhatSymmetry
 };
 
 int fft_lines_per_block( const FftLen & L );
+
+// Pass C kernels that can run this plan, as (c_kind, c_tk) pairs (at least one).
+std::vector<std::pair<int, int>> ddi_pass_c_candidates( const DdiDev & d );
 
 // nullptr if every pass of the plan fits the device's LDS, else the reason it does not.
 const char * ddi_plan_problem( const DdiDev & d );

@@ -18,6 +18,8 @@
 #include "fft_pow2.hpp"
 
 #include <algorithm>
+#include <utility>
+#include <vector>
 
 namespace spirit_hip
 {
@@ -587,44 +589,182 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_inv( DdiDev d, double * __restri
     }
 }
 
-// Pass B: LPB consecutive columns (ka) of one (arr, c) slab, inputs b < b_in, outputs b < b_out.
+// Column tiles of the pass B / pass C kernels: CW consecutive ka columns, N/8 threads per
+// column; thread (l = tid % CW, t = tid / CW) holds the values t + (N/8) k of column l, so a
+// wave's global accesses are CW-wide contiguous row pieces (128 B for CW = 8).
+template<int N>
+struct P2Col
+{
+    static constexpr int T   = N / 8;
+    static constexpr int CWB = ( 512 / T ) > 64 ? 64 : ( 512 / T ); // pass B: <= 512 threads
+    static constexpr int CWC = ( 256 / T ) > 32 ? 32 : ( 256 / T ); // pass C (one basis atom): <= 256 threads
+};
+
+// Pass B: columns along b of one (arr, c) slab, inputs b < b_in (zero above), outputs b < b_out.
+// Inputs go straight from HBM to registers and outputs back; LDS only carries the transposes
+// between FFT stages.
 template<int N, bool INV>
 __global__ void __launch_bounds__( 512 ) k_p2_b( DdiDev d, double2 * __restrict__ X, int c_ext, int b_in, int b_out )
 {
     extern __shared__ double2 lds[];
-    constexpr int T = P2<N>::T, LPB = P2Blk<N>::LPB, LS = P2<N>::LS;
-    const int n_tiles = ( d.Ha + LPB - 1 ) / LPB;
+    constexpr int T = P2<N>::T, CW = P2Col<N>::CWB, LS = P2<N>::LS;
+    const int n_tiles = ( d.Ha + CW - 1 ) / CW;
     long long bid     = blockIdx.x;
     const int tile    = (int)( bid % n_tiles );
     bid /= n_tiles;
     const int c    = (int)( bid % c_ext );
     const int arr  = (int)( bid / c_ext );
-    const int ka0  = tile * LPB;
-    double2 * slab = X + ( (long long)arr * c_ext + c ) * (long long)N * d.Hs;
-    for( int e = threadIdx.x; e < LPB * b_in; e += blockDim.x )
-    {
-        const int l = e % LPB, b = e / LPB;
-        if( ka0 + l < d.Ha )
-            lds[l * LS + pad8( b )] = slab[(long long)b * d.Hs + ka0 + l];
-    }
-    __syncthreads();
-    const int l = threadIdx.x / T, t = threadIdx.x % T;
-    double2 * L = lds + l * LS;
+    const int l    = threadIdx.x % CW;
+    const int t    = threadIdx.x / CW;
+    const int ka   = tile * CW + l;
+    const bool col = ka < d.Ha;
+    double2 * col_ptr = X + ( (long long)arr * c_ext + c ) * (long long)N * d.Hs + ka;
     double2 v[8];
 #pragma unroll
     for( int k = 0; k < 8; ++k )
     {
         const int b = t + T * k;
-        v[k]        = b < b_in ? L[pad8( b )] : make_double2( 0, 0 );
+        v[k]        = ( col && b < b_in ) ? col_ptr[(long long)b * d.Hs] : make_double2( 0, 0 );
     }
-    __syncthreads();
-    p2_fft<N, INV>( v, L, d.Lb.tw2, t );
-    for( int e = threadIdx.x; e < LPB * b_out; e += blockDim.x )
+    p2_fft_reg<N, INV>( v, lds + l * LS, d.Lb.tw2, t );
+    if( col )
     {
-        const int ll = e % LPB, b = e / LPB;
-        if( ka0 + ll < d.Ha )
-            slab[(long long)b * d.Hs + ka0 + ll] = lds[ll * LS + pad8( b )];
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+        {
+            const int b = t + T * k;
+            if( b < b_out )
+                col_ptr[(long long)b * d.Hs] = v[k];
+        }
     }
+}
+
+// Pass C for one basis atom: CW columns (ka) of one kb plane; each thread holds the values
+// c = t + (N/8) k of all three components of its column, so after the three forward
+// transforms the spectrum multiply (3x3, symmetric) runs in registers on the same index set
+// the inverse transforms take as input.
+template<int N, bool REAL>
+__global__ void __launch_bounds__( 256, 2 ) k_p2_c1( DdiDev d, double2 * __restrict__ X, int order_pairs )
+{
+    extern __shared__ double2 lds[];
+    constexpr int T = P2<N>::T, CW = P2Col<N>::CWC, LS = P2<N>::LS;
+    const int n_tiles = ( d.Ha + CW - 1 ) / CW;
+    // Blocks are dealt to the 8 XCDs round-robin; renumber so that each XCD runs a
+    // contiguous range of logical blocks.  With order_pairs the planes kb and Pb-kb,
+    // which read the same stored kernel-spectrum plane, follow each other.
+    const int nblk = gridDim.x;
+    const int full = ( nblk / 8 ) * 8;
+    int lb         = blockIdx.x;
+    if( lb < full )
+        lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
+    const int tile = lb % n_tiles;
+    const int o    = lb / n_tiles;
+    int kb         = o;
+    if( order_pairs && o > 0 )
+    {
+        const int p = ( o + 1 ) / 2;
+        kb          = ( o & 1 ) ? p : d.Pb - p;
+    }
+    const int l           = threadIdx.x % CW;
+    const int t           = threadIdx.x / CW;
+    const int ka          = tile * CW + l;
+    const bool col        = ka < d.Ha;
+    const int Nc          = d.Nc;
+    const long long plane = (long long)d.Pb * d.Hs;
+    double2 * base        = X + (long long)kb * d.Hs + ka;
+    // stage twiddles live in LDS behind the transpose lines (read per use, not held in registers)
+    double2 * tw = lds + CW * LS;
+    for( int e = threadIdx.x; e < P2<N>::NTW; e += blockDim.x )
+        tw[e] = d.Lc.tw2[e];
+    __syncthreads();
+    double2 * L = lds + l * LS;
+    // Components are loaded, transformed and stored one after the other (scheduling barriers
+    // keep the next component's loads from being hoisted), so at most two components plus one
+    // transform's working set are live in registers.
+    double2 v0[8], v1[8], v2[8];
+    auto load = [&]( double2( &v )[8], int q ) {
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+        {
+            const int c = t + T * k;
+            v[k]        = ( col && c < Nc ) ? base[(long long)( q * Nc + c ) * plane] : make_double2( 0, 0 );
+        }
+    };
+    auto store = [&]( const double2( &v )[8], int q ) {
+        if( col )
+        {
+#pragma unroll
+            for( int k = 0; k < 8; ++k )
+            {
+                const int c = t + T * k;
+                if( c < Nc )
+                    base[(long long)( q * Nc + c ) * plane] = v[k];
+            }
+        }
+    };
+    load( v0, 0 );
+    p2_fft_reg<N, false>( v0, L, tw, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    load( v1, 1 );
+    p2_fft_reg<N, false>( v1, L, tw, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    load( v2, 2 );
+    p2_fft_reg<N, false>( v2, L, tw, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    if( col )
+    {
+        // kernel spectrum at (kb, ka, kc = t + T k); mirror images in b and c carry the
+        // per-component signs
+        const bool mb = d.sym_b && kb > d.Pb / 2;
+        const int kbs = mb ? d.Pb - kb : kb;
+        const long long row = ( (long long)kbs * d.Ha + ka ) * d.KC;
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+        {
+            const int kc  = t + T * k;
+            double2 D[6];
+            if constexpr( REAL )
+            {
+                const bool mc    = d.sym_c && kc > d.Pc / 2;
+                const int kcs    = mc ? d.Pc - kc : kc;
+                const double * p = d.Dh_r + ( row + kcs ) * 6;
+#pragma unroll
+                for( int q = 0; q < 6; ++q )
+                {
+                    const int sg = ( mb ? d.sig_b[q] : 1 ) * ( mc ? d.sig_c[q] : 1 );
+                    D[q]         = make_double2( sg < 0 ? -p[q] : p[q], 0.0 );
+                }
+            }
+            else
+                dhat_load( d, kb, ka, kc, 0, D );
+            double2 sv[3] = { v0[k], v1[k], v2[k] }, r[3];
+            if constexpr( REAL )
+            {
+                // real symmetric D: r_q = sum_p D_qp s_p, component-wise on re and im
+                const double dxx = D[0].x, dxy = D[1].x, dxz = D[2].x, dyy = D[3].x, dyz = D[4].x, dzz = D[5].x;
+                r[0] = make_double2( dxx * sv[0].x + dxy * sv[1].x + dxz * sv[2].x,
+                                     dxx * sv[0].y + dxy * sv[1].y + dxz * sv[2].y );
+                r[1] = make_double2( dxy * sv[0].x + dyy * sv[1].x + dyz * sv[2].x,
+                                     dxy * sv[0].y + dyy * sv[1].y + dyz * sv[2].y );
+                r[2] = make_double2( dxz * sv[0].x + dyz * sv[1].x + dzz * sv[2].x,
+                                     dxz * sv[0].y + dyz * sv[1].y + dzz * sv[2].y );
+            }
+            else
+                sym3_mul( D, sv, r );
+            v0[k] = r[0], v1[k] = r[1], v2[k] = r[2];
+            // bound the kernel-spectrum loads in flight (register pressure)
+            if( k & 1 )
+                __builtin_amdgcn_sched_barrier( 0 );
+        }
+    }
+    p2_fft_reg<N, true>( v0, L, tw, t );
+    store( v0, 0 );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p2_fft_reg<N, true>( v1, L, tw, t );
+    store( v1, 1 );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p2_fft_reg<N, true>( v2, L, tw, t );
+    store( v2, 2 );
 }
 
 // Pass C: tk columns (ka) of one kb plane, all 3*nb arrays (3 nb tk lines of T threads).
@@ -754,6 +894,45 @@ static bool p2_ab_ok( const DdiDev & d, const FftLen & L )
     return is_p2( L ) && p2_ab_lds( L.n ) <= (size_t)d.lds_max;
 }
 
+// entries of the power-of-two stage-twiddle table
+static int p2_ntw( int n )
+{
+    int lg = 0;
+    while( ( 1 << lg ) < n )
+        ++lg;
+    const int n8 = lg / 3, rem = lg % 3, nst = n8 + ( rem ? 1 : 0 );
+    int total = 0, Ns = 8;
+    for( int s = 1; s < nst; ++s, Ns *= 8 )
+        total += Ns * ( ( s < n8 ? 8 : ( rem == 1 ? 2 : 4 ) ) - 1 );
+    return total > 0 ? total : 1;
+}
+
+// pass B: CW*(n/8) threads, CW lines of LDS
+static int p2_b_cw( int n )
+{
+    const int T = n / 8;
+    return ( 512 / T ) > 64 ? 64 : ( 512 / T );
+}
+static size_t p2_b_lds( int n )
+{
+    return sizeof( double2 ) * p2_b_cw( n ) * p2_ls( n );
+}
+static bool p2_b_ok( const DdiDev & d )
+{
+    return is_p2( d.Lb ) && p2_b_lds( d.Lb.n ) <= (size_t)d.lds_max;
+}
+// pass C for one basis atom (k_p2_c1): CW*(n/8) <= 256 threads
+static int p2_c1_cw( int n )
+{
+    const int T = n / 8;
+    return ( 256 / T ) > 32 ? 32 : ( 256 / T );
+}
+static bool p2_c1_ok( const DdiDev & d )
+{
+    return d.nb == 1 && d.dh_real && is_p2( d.Lc ) && d.Lc.n <= 2048
+           && sizeof( double2 ) * ( p2_c1_cw( d.Lc.n ) * p2_ls( d.Lc.n ) + p2_ntw( d.Lc.n ) ) <= (size_t)d.lds_max;
+}
+
 // pass C column tile: 3*nb*tk*(N/8) threads <= 1024, LDS <= min(lds_max, 120 KiB), tk <= 16; 0 if none fits
 static int p2_c_tk( const DdiDev & d )
 {
@@ -773,14 +952,31 @@ static int generic_c_tk( const DdiDev & d )
     return tk < 1 ? 1 : tk;
 }
 
+std::vector<std::pair<int, int>> ddi_pass_c_candidates( const DdiDev & d )
+{
+    std::vector<std::pair<int, int>> c;
+    if( p2_c1_ok( d ) )
+        c.push_back( { DDI_C_P2_REG, 0 } );
+    const int tk = p2_c_tk( d );
+    if( tk > 0 )
+    {
+        c.push_back( { DDI_C_P2_LDS, tk } );
+        if( tk / 2 > 0 )
+            c.push_back( { DDI_C_P2_LDS, tk / 2 } );
+    }
+    if( c.empty() )
+        c.push_back( { DDI_C_GENERIC, 0 } );
+    return c;
+}
+
 const char * ddi_plan_problem( const DdiDev & d )
 {
     const size_t lim = (size_t)d.lds_max;
     if( !p2_ab_ok( d, d.La ) && lds_bytes( fft_lines_per_block( d.La ), d.La.n ) > lim )
         return "padded length along a exceeds the LDS of one block";
-    if( !p2_ab_ok( d, d.Lb ) && lds_bytes( fft_lines_per_block( d.Lb ), d.Lb.n ) > lim )
+    if( !p2_b_ok( d ) && lds_bytes( fft_lines_per_block( d.Lb ), d.Lb.n ) > lim )
         return "padded length along b exceeds the LDS of one block";
-    if( p2_c_tk( d ) == 0 && lds_bytes( 3 * d.nb * generic_c_tk( d ), d.Lc.n ) > lim )
+    if( !p2_c1_ok( d ) && p2_c_tk( d ) == 0 && lds_bytes( 3 * d.nb * generic_c_tk( d ), d.Lc.n ) > lim )
         return "padded length along c times 3*n_cell_atoms exceeds the LDS of one block";
     if( lds_bytes( 6 * std::max( 1, fft_lines_per_block( d.Lc ) / 6 ), d.Lc.n ) > lim )
         return "padded length along c exceeds the LDS of the kernel-spectrum set-up";
@@ -806,11 +1002,24 @@ static void p2_launch_a_inv( const DdiDev & d, double * h, int nos, hipStream_t 
 template<int N, bool INV>
 static void p2_launch_b( const DdiDev & d, hipStream_t st )
 {
-    const int LPB         = P2Blk<N>::LPB;
-    const int tiles       = ( d.Ha + LPB - 1 ) / LPB;
+    constexpr int CW      = P2Col<N>::CWB;
+    const int tiles       = ( d.Ha + CW - 1 ) / CW;
     const unsigned blocks = (unsigned)( (long long)tiles * d.Nc * 3 * d.nb );
-    hipLaunchKernelGGL( ( k_p2_b<N, INV> ), dim3( blocks ), dim3( P2Blk<N>::THR ), p2_ab_lds( N ), st, d, d.X, d.Nc,
+    hipLaunchKernelGGL( ( k_p2_b<N, INV> ), dim3( blocks ), dim3( CW * P2<N>::T ), p2_b_lds( N ), st, d, d.X, d.Nc,
                         INV ? N : d.Nb, INV ? d.Nb : N );
+}
+template<int N>
+static void p2_launch_c1( const DdiDev & d, hipStream_t st )
+{
+    if constexpr( N <= 2048 )
+    {
+        constexpr int CW      = P2Col<N>::CWC;
+        const int tiles       = ( d.Ha + CW - 1 ) / CW;
+        const unsigned blocks = (unsigned)( (long long)tiles * d.Pb );
+        const size_t lds = sizeof( double2 ) * ( CW * p2_ls( N ) + P2<N>::NTW );
+        hipLaunchKernelGGL( ( k_p2_c1<N, true> ), dim3( blocks ), dim3( CW * P2<N>::T ), lds, st, d, d.X,
+                            d.sym_b ? 1 : 0 );
+    }
 }
 template<int N>
 static void p2_launch_c( const DdiDev & d, int tk, hipStream_t st )
@@ -858,7 +1067,7 @@ void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int n
                             spins, nos, n_arr, d.Na, d.Nb, d.Nc, d.Pb, d.Nc, d.Hs, d.X, lpb );
     }
     // pass B
-    if( p2_ab_ok( d, d.Lb ) )
+    if( p2_b_ok( d ) )
     {
 #define CALL_B( n ) p2_launch_b<n, false>( d, st )
         P2_DISPATCH( d.Pb, CALL_B );
@@ -875,10 +1084,15 @@ void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int n
     // pass C
     if( evc0 )
         (void)hipEventRecord( evc0, st );
-    const int tkc = p2_c_tk( d );
-    if( tkc > 0 )
+    if( d.c_kind == DDI_C_P2_REG && p2_c1_ok( d ) )
     {
-#define CALL_C( n ) p2_launch_c<n>( d, tkc, st )
+#define CALL_C1( n ) p2_launch_c1<n>( d, st )
+        P2_DISPATCH( d.Pc, CALL_C1 );
+#undef CALL_C1
+    }
+    else if( d.c_kind == DDI_C_P2_LDS && d.c_tk > 0 && d.c_tk <= p2_c_tk( d ) )
+    {
+#define CALL_C( n ) p2_launch_c<n>( d, d.c_tk, st )
         P2_DISPATCH( d.Pc, CALL_C );
 #undef CALL_C
     }
@@ -893,7 +1107,7 @@ void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int n
     if( evc1 )
         (void)hipEventRecord( evc1, st );
     // pass B^-1
-    if( p2_ab_ok( d, d.Lb ) )
+    if( p2_b_ok( d ) )
     {
 #define CALL_BI( n ) p2_launch_b<n, true>( d, st )
         P2_DISPATCH( d.Pb, CALL_BI );

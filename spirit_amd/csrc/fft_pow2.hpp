@@ -27,6 +27,15 @@ __device__ __forceinline__ constexpr int pad8( int i )
     return i + ( i >> 3 );
 }
 
+constexpr int ntw_sum( int n8, int rem )
+{
+    int total = 0, Ns = 8;
+    const int nst = n8 + ( rem ? 1 : 0 );
+    for( int s = 1; s < nst; ++s, Ns *= 8 )
+        total += Ns * ( ( s < n8 ? 8 : ( rem == 1 ? 2 : 4 ) ) - 1 );
+    return total > 0 ? total : 1;
+}
+
 template<int N>
 struct P2
 {
@@ -37,6 +46,8 @@ struct P2
     static constexpr int REM = LOG % 3;
     static constexpr int NST = N8 + ( REM ? 1 : 0 );
     static constexpr int LS  = N + N / 8 + 1; // padded LDS line stride (complex values)
+    // entries of the stage-twiddle table: sum over stages s >= 1 of 8^s * (R_s - 1)
+    static constexpr int NTW = ntw_sum( N8, REM );
 };
 
 template<int N, int S>
@@ -47,9 +58,12 @@ struct P2Stage
     static constexpr int TWOFF = ipow8c( S ) - 8; // valid for S >= 1
 };
 
-// Runs stages S.. of the transform.  On entry v holds this thread's inputs of stage S;
-// on exit the transform is in L[pad8(i)] in natural order (block-synchronised).
-template<int N, bool INV, int S>
+// Runs stages S.. of the transform.  On entry v holds this thread's inputs of stage S.
+// KEEP = false: on exit the transform is in L[pad8(i)] in natural order (block-synchronised).
+// KEEP = true:  the last stage leaves its outputs in registers, v[k] = X[t + T*k] (the
+//               same layout as the inputs), and L is free again for the next transform
+//               once every thread has passed the barrier that precedes the last stage.
+template<int N, bool INV, int S, bool KEEP>
 __device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restrict__ L,
                                           const double2 * __restrict__ tw, int t )
 {
@@ -58,6 +72,8 @@ __device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restric
     constexpr int Ns   = St::Ns;
     constexpr int M    = 8 / R;
     constexpr int T    = P2<N>::T;
+    constexpr bool LST = ( S + 1 == P2<N>::NST );
+    double2 out[8];
 #pragma unroll
     for( int m = 0; m < M; ++m )
     {
@@ -79,27 +95,57 @@ __device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restric
             }
         }
         dft_small<R, INV>( u, nullptr, N );
-        const int idxD = ( j / Ns ) * Ns * R + jm;
+        if constexpr( LST && KEEP )
+        {
+            // last stage: Ns = N/R, so idxD + r*Ns = j + r*N/R = t + T*(m + M*r)
 #pragma unroll
-        for( int r = 0; r < R; ++r )
-            L[pad8( idxD + r * Ns )] = u[r];
+            for( int r = 0; r < R; ++r )
+                out[m + M * r] = u[r];
+        }
+        else
+        {
+            const int idxD = ( j / Ns ) * Ns * R + jm;
+#pragma unroll
+            for( int r = 0; r < R; ++r )
+                L[pad8( idxD + r * Ns )] = u[r];
+        }
     }
-    __syncthreads();
-    if constexpr( S + 1 < P2<N>::NST )
+    if constexpr( LST && KEEP )
     {
 #pragma unroll
         for( int k = 0; k < 8; ++k )
-            v[k] = L[pad8( t + T * k )];
+            v[k] = out[k];
+        return;
+    }
+    else
+    {
         __syncthreads();
-        p2_stage<N, INV, S + 1>( v, L, tw, t );
+        if constexpr( !LST )
+        {
+#pragma unroll
+            for( int k = 0; k < 8; ++k )
+                v[k] = L[pad8( t + T * k )];
+            __syncthreads();
+            p2_stage<N, INV, S + 1, KEEP>( v, L, tw, t );
+        }
     }
 }
 
+// Transform ending in LDS (natural order, L[pad8(i)]).
 template<int N, bool INV>
 __device__ __forceinline__ void p2_fft( double2 ( &v )[8], double2 * __restrict__ L, const double2 * __restrict__ tw,
                                         int t )
 {
-    p2_stage<N, INV, 0>( v, L, tw, t );
+    p2_stage<N, INV, 0, false>( v, L, tw, t );
+}
+
+// Transform ending in registers: v[k] = X[t + T*k].  Every thread of the block must call it
+// (it synchronises the block between stages); L may be reused right after it returns.
+template<int N, bool INV>
+__device__ __forceinline__ void p2_fft_reg( double2 ( &v )[8], double2 * __restrict__ L,
+                                            const double2 * __restrict__ tw, int t )
+{
+    p2_stage<N, INV, 0, true>( v, L, tw, t );
 }
 
 } // namespace spirit_hip

@@ -1,6 +1,7 @@
 // Device-resident LLG runner and DDI plan.
 #include "gpu_engine.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -242,12 +243,61 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
     ddi_launch_compress( d, full.as<double2>(), Dh.as<void>(), st );
     HIPCHECK( hipGetLastError() );
     HIPCHECK( hipStreamSynchronize( st ) );
-    char msg[256];
+    full.free();
+    choose_pass_c( st );
+    char msg[320];
     std::snprintf( msg, sizeof( msg ),
-                   "DDI plan %dx%dx%d padded, kernel spectrum %s (|Im|/max %.2e, b-mirror %.2e, c-mirror %.2e)", d.Pa,
-                   d.Pb, d.Pc, spectrum_format(), sym.max_imag / sym.max_abs, sym.max_asym_b / sym.max_abs,
-                   sym.max_asym_c / sym.max_abs );
+                   "DDI plan %dx%dx%d padded, kernel spectrum %s (|Im|/max %.2e, b-mirror %.2e, c-mirror %.2e), "
+                   "pass C kernel %d/%d (%.3f ms)",
+                   d.Pa, d.Pb, d.Pc, spectrum_format(), sym.max_imag / sym.max_abs, sym.max_asym_b / sym.max_abs,
+                   sym.max_asym_c / sym.max_abs, d.c_kind, d.c_tk, c_ms );
     log_message( 5, msg );
+}
+
+// Times every pass C kernel that can run the plan on a zero configuration (the run time
+// does not depend on the values) and keeps the fastest.
+void DdiPlan::choose_pass_c( hipStream_t st )
+{
+    auto cand = ddi_pass_c_candidates( d );
+    d.c_kind  = cand[0].first;
+    d.c_tk    = cand[0].second;
+    c_ms      = 0;
+    if( cand.size() == 1 )
+        return;
+    const size_t nos = (size_t)d.Na * d.Nb * d.Nc * d.nb;
+    DeviceBuffer s0( sizeof( double ) * 3 * nos ), h0( sizeof( double ) * 3 * nos );
+    HIPCHECK( hipMemsetAsync( s0.as<void>(), 0, s0.size(), st ) );
+    hipEvent_t e0, e1;
+    HIPCHECK( hipEventCreate( &e0 ) );
+    HIPCHECK( hipEventCreate( &e1 ) );
+    double best = 1e30;
+    std::pair<int, int> pick = cand[0];
+    for( auto & c : cand )
+    {
+        d.c_kind = c.first;
+        d.c_tk   = c.second;
+        ddi_launch_field( d, s0.as<double>(), h0.as<double>(), (int)nos, st );
+        double t = 1e30;
+        for( int rep = 0; rep < 3; ++rep )
+        {
+            ddi_launch_field( d, s0.as<double>(), h0.as<double>(), (int)nos, st, e0, e1 );
+            HIPCHECK( hipGetLastError() );
+            HIPCHECK( hipEventSynchronize( e1 ) );
+            float ms = 0;
+            HIPCHECK( hipEventElapsedTime( &ms, e0, e1 ) );
+            t = std::min( t, (double)ms );
+        }
+        if( t < best )
+        {
+            best = t;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    d.c_kind = pick.first;
+    d.c_tk   = pick.second;
+    c_ms     = best;
 }
 
 void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0,

@@ -73,9 +73,17 @@ public:
         return sym;
     }
 
+    // milliseconds of the chosen pass C kernel when the plan was built (0 if there was no choice)
+    double pass_c_ms() const
+    {
+        return c_ms;
+    }
+
 private:
+    void choose_pass_c( hipStream_t st );
     DdiDev d{};
     DhatSymmetry sym{};
+    double c_ms = 0;
     FftLenHost La, Lb, Lc;
     DeviceBuffer X, Dh;
 };
