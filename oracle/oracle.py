@@ -75,6 +75,22 @@ class OrcLLG(ctypes.Structure):
     ]
 
 
+class OrcGNEB(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("spring_constant", ctypes.c_double),
+        ("spring_force_ratio", ctypes.c_double),
+        ("path_shortening_constant", ctypes.c_double),
+        ("moving_endpoints", ctypes.c_int),
+        ("translating_endpoints", ctypes.c_int),
+        ("escape_first", ctypes.c_int),
+        ("delta_Rx_left", ctypes.c_double),
+        ("delta_Rx_right", ctypes.c_double),
+        ("n_E_interpolations", ctypes.c_int),
+        ("force_convergence", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -116,6 +132,11 @@ def lib():
         L.orc_fft.argtypes = [P(d), i, i]
         L.orc_rotate.argtypes = [P(d), P(d), d, P(d)]
         L.orc_sib_transform.argtypes = [P(d), P(d), P(d)]
+        L.orc_gneb_run.argtypes = [vp, i, P(OrcGNEB), i, P(i), P(d), ctypes.c_long, i, P(d), P(d), P(d), P(d), P(d),
+                                   P(d)]
+        L.orc_gneb_run.restype = ctypes.c_long
+        L.orc_cubic_hermite.argtypes = [P(d), P(d), P(d), i, i, P(d), P(d)]
+        L.orc_homogeneous_rotation.argtypes = [P(d), i, i, i]
         _lib = L
     return _lib
 
@@ -316,6 +337,45 @@ class System:
                               ctypes.byref(e), ctypes.byref(t), _dp(eff))
         del keep
         return s, e.value, t.value, eff
+
+    def gneb(self, chain, solver, n_iterations, image_type=None, dt=1e-3, spring_constant=1.0,
+             spring_force_ratio=0.0, path_shortening_constant=0.0, moving_endpoints=False,
+             translating_endpoints=False, escape_first=False, delta_Rx=(1.0, 1.0), n_E_interpolations=10,
+             force_convergence=1e-12, check_convergence=True):
+        """Method_GNEB run on a (noi, nos, 3) chain; returns (chain, dict of chain data, iterations)."""
+        c = np.array(chain, dtype=np.float64).reshape(-1, self.nos, 3).copy()
+        noi = c.shape[0]
+        it = np.zeros(noi, dtype=np.int32) if image_type is None else np.ascontiguousarray(image_type, np.int32)
+        par = OrcGNEB()
+        par.dt, par.spring_constant, par.spring_force_ratio = dt, spring_constant, spring_force_ratio
+        par.path_shortening_constant = path_shortening_constant
+        par.moving_endpoints, par.translating_endpoints = int(moving_endpoints), int(translating_endpoints)
+        par.escape_first = int(escape_first)
+        par.delta_Rx_left, par.delta_Rx_right = delta_Rx
+        par.n_E_interpolations, par.force_convergence = n_E_interpolations, force_convergence
+        E, Rx, mt = np.zeros(noi), np.zeros(noi), np.zeros(noi)
+        eff = np.zeros((noi, self.nos, 3))
+        npts = noi + (noi - 1) * n_E_interpolations
+        Rxi, Ei = np.zeros(npts), np.zeros(npts)
+        n = lib().orc_gneb_run(self.h, int(solver), ctypes.byref(par), noi, _ip(it), _dp(c), int(n_iterations),
+                               1 if check_convergence else 0, _dp(E), _dp(Rx), _dp(mt), _dp(eff), _dp(Rxi), _dp(Ei))
+        data = dict(energies=E, Rx=Rx, max_torque=mt, eff=eff, Rx_interpolated=Rxi, E_interpolated=Ei)
+        return c, data, n
+
+
+def cubic_hermite(x, p, m, n_interp):
+    x, p, m = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, p, m))
+    n = len(x)
+    npts = n + (n - 1) * n_interp
+    ox, op = np.zeros(npts), np.zeros(npts)
+    lib().orc_cubic_hermite(_dp(x), _dp(p), _dp(m), n, n_interp, _dp(ox), _dp(op))
+    return ox, op
+
+
+def homogeneous_rotation(chain, idx_1, idx_2):
+    c = np.array(chain, dtype=np.float64).copy()
+    lib().orc_homogeneous_rotation(_dp(c), c.shape[1], idx_1, idx_2)
+    return c
 
 
 def neighbours_in_shells(cfg, n_shells, redundant=True):

@@ -1369,3 +1369,749 @@ void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spi
         }
     }
 }
+
+/* ======================================================================
+ * GNEB (core/src/engine/Method_GNEB.cpp, Manifoldmath.cpp, Solver_*.hpp)
+ * ====================================================================== */
+
+/* Vectormath::dot over a field of n3/3 vectors (serial sum in index order) */
+static double vf_dot( const double * a, const double * b, long n3 )
+{
+    double s = 0;
+    for( long i = 0; i < n3; i += 3 )
+        s += a[i] * b[i] + a[i + 1] * b[i + 1] + a[i + 2] * b[i + 2];
+    return s;
+}
+/* Manifoldmath::normalize, Manifoldmath.cpp:220-229: vf *= 1/|vf| */
+static void vf_normalize( double * a, long n3 )
+{
+    double sc = 1.0 / sqrt( vf_dot( a, a, n3 ) );
+    for( long t = 0; t < n3; ++t )
+        a[t] *= sc;
+}
+/* Manifoldmath::project_tangential, Manifoldmath.cpp:58-63 */
+static void vf_project_tangential( double * v1, const double * v2, long n3 )
+{
+    for( long i = 0; i < n3; i += 3 )
+    {
+        double d = v1[i] * v2[i] + v1[i + 1] * v2[i + 1] + v1[i + 2] * v2[i + 2];
+        for( int q = 0; q < 3; ++q )
+            v1[i + q] -= d * v2[i + q];
+    }
+}
+/* Vectormath::angle, Vectormath.cpp:434-441 */
+static double v3_angle( const double * a, const double * b )
+{
+    double r = v3_dot( a, b );
+    r        = fmax( -1.0, fmin( 1.0, r ) );
+    return acos( r );
+}
+/* Manifoldmath::dist_geodesic, Manifoldmath.cpp:65-73 */
+static double dist_geodesic( const double * a, const double * b, long n3 )
+{
+    double d = 0;
+    for( long i = 0; i < n3; i += 3 )
+    {
+        double x = v3_angle( a + i, b + i );
+        d += x * x;
+    }
+    return sqrt( d );
+}
+
+/* Geodesic_Tangent, Manifoldmath.cpp:78-108 */
+static void geodesic_tangent( double * tangent, const double * minus, const double * plus, const double * mid, long n3 )
+{
+    const double ex[3] = { 1, 0, 0 }, ey[3] = { 0, 1, 0 };
+    const double eps   = 1e-15;
+    for( long i = 0; i < n3; i += 3 )
+    {
+        double axis[3];
+        v3_cross( plus + i, minus + i, axis );
+        if( fabs( v3_dot( minus + i, plus + i ) + 1 ) < eps )
+        {
+            const double * a = ( fabs( v3_dot( mid + i, ex ) - 1 ) > eps ) ? ex : ey;
+            axis[0] = a[0], axis[1] = a[1], axis[2] = a[2];
+        }
+        v3_cross( mid + i, axis, tangent + i );
+    }
+    vf_normalize( tangent, n3 );
+}
+
+/* Tangents, Manifoldmath.cpp:113-209 */
+static void gneb_tangents( double * const * conf, const double * E, double ** tangents, int noi, long n3, double * tp,
+                           double * tm )
+{
+    for( int img = 0; img < noi; ++img )
+    {
+        const double * image = conf[img];
+        double * tau         = tangents[img];
+        if( img == 0 )
+            geodesic_tangent( tau, image, conf[1], image, n3 );
+        else if( img == noi - 1 )
+            geodesic_tangent( tau, conf[img - 1], image, image, n3 );
+        else
+        {
+            const double * plus  = conf[img + 1];
+            const double * minus = conf[img - 1];
+            double E_mid = E[img], E_plus = E[img + 1], E_minus = E[img - 1];
+            for( long t = 0; t < n3; ++t )
+            {
+                tp[t] = 1 * plus[t];
+                tp[t] += -1 * image[t];
+                tm[t] = 1 * image[t];
+                tm[t] += -1 * minus[t];
+            }
+            if( ( E_plus < E_mid && E_mid > E_minus ) || ( E_plus > E_mid && E_mid < E_minus ) )
+            {
+                double E_max = fmax( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
+                double E_min = fmin( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
+                double cp = E_plus > E_minus ? E_max : E_min;
+                double cm = E_plus > E_minus ? E_min : E_max;
+                for( long t = 0; t < n3; ++t )
+                {
+                    tau[t] = cp * tp[t];
+                    tau[t] += cm * tm[t];
+                }
+            }
+            else if( E_plus > E_mid && E_mid > E_minus )
+            {
+                for( long t = 0; t < n3; ++t )
+                    tau[t] = 1 * tp[t];
+            }
+            else if( E_plus < E_mid && E_mid < E_minus )
+            {
+                for( long t = 0; t < n3; ++t )
+                    tau[t] = tm[t];
+            }
+            else
+            {
+                for( long t = 0; t < n3; ++t )
+                {
+                    tau[t] = 1 * tp[t];
+                    tau[t] += 1 * tm[t];
+                }
+            }
+            vf_project_tangential( tau, image, n3 );
+            vf_normalize( tau, n3 );
+        }
+    }
+}
+
+/* Cubic_Hermite_Spline::Interpolate, core/src/utility/Cubic_Hermite_Spline.cpp:10-47.
+   x, p, m of length n; out_x, out_p of length n + (n-1)*n_interp. */
+void orc_cubic_hermite( const double * x, const double * p, const double * m, int n, int n_interp, double * out_x,
+                        double * out_p )
+{
+    for( int i = 0; i < n - 1; ++i )
+    {
+        double x0 = x[i], x1 = x[i + 1], p0 = p[i], p1 = p[i + 1], m0 = m[i], m1 = m[i + 1];
+        for( int j = 0; j < n_interp + 1; ++j )
+        {
+            double t   = j / (double)( n_interp + 1 );
+            double h00 = 2 * pow( t, 3 ) - 3 * pow( t, 2 ) + 1;
+            double h10 = -2 * pow( t, 3 ) + 3 * pow( t, 2 );
+            double h01 = pow( t, 3 ) - 2 * pow( t, 2 ) + t;
+            double h11 = pow( t, 3 ) - pow( t, 2 );
+            int idx    = i * ( n_interp + 1 ) + j;
+            out_x[idx] = x0 + t * ( x1 - x0 );
+            out_p[idx] = h00 * p0 + h10 * p1 + h01 * m0 * ( x0 - x1 ) + h11 * m1 * ( x0 - x1 );
+        }
+    }
+    int last    = n + ( n - 1 ) * n_interp - 1;
+    out_x[last] = x[n - 1];
+    out_p[last] = p[n - 1];
+}
+
+/* Rotation matrix about a unit axis (Rodrigues), as Eigen::AngleAxis::toRotationMatrix */
+static void rotation_matrix( const double * axis, double angle, double R[3][3] )
+{
+    double s = sin( angle ), c = cos( angle );
+    double sa[3] = { s * axis[0], s * axis[1], s * axis[2] };
+    double ca[3] = { ( 1 - c ) * axis[0], ( 1 - c ) * axis[1], ( 1 - c ) * axis[2] };
+    double t;
+    t       = ca[0] * axis[1];
+    R[0][1] = t - sa[2];
+    R[1][0] = t + sa[2];
+    t       = ca[0] * axis[2];
+    R[0][2] = t + sa[1];
+    R[2][0] = t - sa[1];
+    t       = ca[1] * axis[2];
+    R[1][2] = t - sa[0];
+    R[2][1] = t + sa[0];
+    for( int q = 0; q < 3; ++q )
+        R[q][q] = ca[q] * axis[q] + c;
+}
+
+typedef struct gneb_ws
+{
+    int noi;
+    long n3;
+    double **F_gradient, **F_total, **F_spring, **tangents, **eff;
+    double *f_shrink, *F_tl, *F_tr, *tp, *tm, *lengths, *dE_dRx;
+    double * Rx;
+    double * E;
+    int stop; /* geodesic distance zero */
+} gneb_ws;
+
+static double ** alloc_fields( int noi, long n3 )
+{
+    double ** f = (double **)malloc( sizeof( double * ) * noi );
+    for( int i = 0; i < noi; ++i )
+        f[i] = (double *)calloc( (size_t)n3, sizeof( double ) );
+    return f;
+}
+static void free_fields( double ** f, int noi )
+{
+    if( !f )
+        return;
+    for( int i = 0; i < noi; ++i )
+        free( f[i] );
+    free( f );
+}
+
+/* Method_GNEB::Calculate_Force, Method_GNEB.cpp:88-357 */
+static void gneb_calc_force( orc_sys * s, const orc_gneb * par, const int * image_type, gneb_ws * w,
+                             double * const * conf, double * const * chain_spins, double ** forces )
+{
+    int noi  = w->noi;
+    long n3  = w->n3;
+    long nos = n3 / 3;
+    for( int img = 0; img < noi; ++img )
+    {
+        orc_gradient_and_energy( s, conf[img], w->eff[img], &w->E[img] );
+        for( long t = 0; t < n3; ++t )
+        {
+            w->eff[img][t] *= -1;
+            w->F_gradient[img][t] = w->eff[img][t];
+        }
+        if( img > 0 )
+        {
+            w->Rx[img] = w->Rx[img - 1] + dist_geodesic( conf[img], conf[img - 1], n3 );
+            if( w->Rx[img] - w->Rx[img - 1] < 1e-10 )
+            {
+                w->stop = 1;
+                return;
+            }
+        }
+    }
+    gneb_tangents( conf, w->E, w->tangents, noi, n3, w->tp, w->tm );
+
+    for( int i = 0; i < noi; ++i )
+        w->lengths[i] = 0;
+    if( par->spring_force_ratio > 0 )
+    {
+        double ratio_E  = fmin( 1.0, par->spring_force_ratio );
+        double ratio_Rx = 1 - ratio_E;
+        for( int i = 0; i < noi; ++i )
+            w->dE_dRx[i] = vf_dot( w->eff[i], w->tangents[i], n3 );
+        int n_interp = 20;
+        int npts     = noi + ( noi - 1 ) * n_interp;
+        double * ix  = (double *)malloc( sizeof( double ) * (size_t)( npts > 0 ? npts : 1 ) );
+        double * ip  = (double *)malloc( sizeof( double ) * (size_t)( npts > 0 ? npts : 1 ) );
+        orc_cubic_hermite( w->Rx, w->E, w->dE_dRx, noi, n_interp, ix, ip );
+        int imax = 0, imin = 0, emax = 0, emin = 0;
+        for( int k = 1; k < npts; ++k )
+        {
+            if( ix[k] > ix[imax] )
+                imax = k;
+            if( ix[k] < ix[imin] )
+                imin = k;
+            if( ip[k] > ip[emax] )
+                emax = k;
+            if( ip[k] < ip[emin] )
+                emin = k;
+        }
+        double range_Rx = ix[imax] - ix[imin];
+        double range_E  = ip[emax] - ip[emin];
+        for( int idx_image = 1; idx_image < noi; ++idx_image )
+        {
+            for( int i = 1; i <= n_interp; ++i )
+            {
+                int idx    = ( idx_image - 1 ) * ( n_interp + 1 ) + i;
+                double dRx = ratio_Rx * ( ix[idx] - ix[idx - 1] ) / range_Rx;
+                double dE  = ratio_E * ( ip[idx] - ip[idx - 1] ) / range_E;
+                w->lengths[idx_image] += sqrt( dRx * dRx + dE * dE );
+            }
+            w->lengths[idx_image] *= range_Rx;
+        }
+        free( ix );
+        free( ip );
+    }
+
+    for( int img = 1; img < noi - 1; ++img )
+    {
+        const double * image = conf[img];
+        double * Fg          = w->F_gradient[img];
+        double * tau         = w->tangents[img];
+        double * Ft          = w->F_total[img];
+        vf_project_tangential( Fg, image, n3 );
+        if( image_type[img] == 1 ) /* climbing: invert_parallel, Manifoldmath.cpp:41-47 */
+        {
+            double x = vf_dot( Fg, tau, n3 );
+            for( long t = 0; t < n3; ++t )
+                Fg[t] -= 2 * x * tau[t];
+            memcpy( Ft, Fg, sizeof( double ) * n3 );
+        }
+        else if( image_type[img] == 2 ) /* falling */
+            memcpy( Ft, Fg, sizeof( double ) * n3 );
+        else if( image_type[img] == 0 ) /* normal */
+        {
+            double x = vf_dot( Fg, tau, n3 ); /* project_orthogonal, Manifoldmath.cpp:32-39 */
+            for( long t = 0; t < n3; ++t )
+                Fg[t] -= x * tau[t];
+            if( par->path_shortening_constant > 0 )
+            {
+                double * tp = w->tp;
+                double * tm = w->tm;
+                for( long t = 0; t < n3; ++t )
+                {
+                    tp[t] = 1 * chain_spins[img + 1][t];
+                    tp[t] += -1 * chain_spins[img][t];
+                    tm[t] = 1 * chain_spins[img][t];
+                    tm[t] += -1 * chain_spins[img - 1][t];
+                }
+                vf_normalize( tp, n3 );
+                vf_normalize( tm, n3 );
+                for( long t = 0; t < n3; ++t )
+                {
+                    w->f_shrink[t] = 1 * tp[t];
+                    w->f_shrink[t] += -1 * tm[t];
+                }
+                for( long t = 0; t < n3; ++t )
+                    tp[t] = 1 * Fg[t];
+                double gradnorm = sqrt( vf_dot( tp, tp, n3 ) );
+                for( long t = 0; t < n3; ++t )
+                    tp[t] *= 1.0 / gradnorm;
+                double x1 = vf_dot( w->f_shrink, tp, n3 );
+                for( long t = 0; t < n3; ++t )
+                    w->f_shrink[t] -= x1 * tp[t];
+                double x2 = vf_dot( w->f_shrink, tau, n3 );
+                for( long t = 0; t < n3; ++t )
+                    w->f_shrink[t] -= x2 * tau[t];
+                vf_normalize( w->f_shrink, n3 );
+                double scalefactor = fmax( gradnorm, nos * par->path_shortening_constant );
+                for( long t = 0; t < n3; ++t )
+                    w->f_shrink[t] *= scalefactor;
+            }
+            double d;
+            if( par->spring_force_ratio > 0 )
+                d = par->spring_constant * ( w->lengths[img + 1] - w->lengths[img] );
+            else
+                d = par->spring_constant * ( w->Rx[img + 1] - 2 * w->Rx[img] + w->Rx[img - 1] );
+            for( long t = 0; t < n3; ++t )
+                w->F_spring[img][t] = d * tau[t];
+            for( long t = 0; t < n3; ++t )
+            {
+                Ft[t] = 1 * Fg[t];
+                Ft[t] += 1 * w->F_spring[img][t];
+            }
+            if( par->path_shortening_constant > 0 )
+                for( long t = 0; t < n3; ++t )
+                    Ft[t] += 1 * w->f_shrink[t];
+        }
+        else
+            memset( Ft, 0, sizeof( double ) * n3 );
+        for( long t = 0; t < n3; ++t )
+            forces[img][t] = 1 * Ft[t];
+    }
+
+    if( par->moving_endpoints )
+    {
+        vf_project_tangential( w->F_gradient[0], conf[0], n3 );
+        vf_project_tangential( w->F_gradient[noi - 1], conf[noi - 1], n3 );
+        if( par->translating_endpoints )
+        {
+            const double * sl = chain_spins[0];
+            const double * sr = chain_spins[noi - 1];
+            const double * gl = w->F_gradient[0];
+            const double * gr = w->F_gradient[noi - 1];
+            for( long i = 0; i < n3; i += 3 )
+            {
+                double axis[3];
+                v3_cross( sl + i, sr + i, axis );
+                double angle = acos( v3_dot( sl + i, sr + i ) );
+                v3_normalize( axis );
+                double R[3][3];
+                rotation_matrix( axis, angle, R );
+                if( fabs( v3_dot( sl + i, sr + i ) ) >= 1.0 )
+                    for( int a = 0; a < 3; ++a )
+                        for( int b = 0; b < 3; ++b )
+                            R[a][b] = a == b ? 1.0 : 0.0;
+                for( int a = 0; a < 3; ++a )
+                {
+                    double rr = R[a][0] * gr[i] + R[a][1] * gr[i + 1] + R[a][2] * gr[i + 2];
+                    double rl = R[0][a] * gl[i] + R[1][a] * gl[i + 1] + R[2][a] * gl[i + 2];
+                    w->F_tl[i + a] = -0.5 * ( gl[i + a] + rr );
+                    w->F_tr[i + a] = -0.5 * ( rl + gr[i + a] );
+                }
+            }
+            double pl = vf_dot( w->F_tl, w->tangents[0], n3 );
+            for( long t = 0; t < n3; ++t )
+                w->F_tl[t] = pl * w->tangents[0][t];
+            double pr = vf_dot( w->F_tr, w->tangents[noi - 1], n3 );
+            for( long t = 0; t < n3; ++t )
+                w->F_tr[t] = pr * w->tangents[noi - 1][t];
+        }
+        double rotational_coeff = 1.0;
+        if( par->escape_first )
+        {
+            double proj_left  = vf_dot( w->F_gradient[0], w->tangents[0], n3 );
+            double proj_right = vf_dot( w->F_gradient[noi - 1], w->tangents[noi - 1], n3 );
+            if( proj_left > proj_right )
+                rotational_coeff = 0.0;
+        }
+        for( int e = 0; e < 2; ++e )
+        {
+            int img           = e == 0 ? 0 : noi - 1;
+            double delta_Rx0  = e == 0 ? par->delta_Rx_left : par->delta_Rx_right;
+            double delta_Rx   = e == 0 ? w->Rx[1] - w->Rx[0] : w->Rx[noi - 1] - w->Rx[noi - 2];
+            double k          = ( e == 0 ? 1.0 : -1.0 ) * par->spring_constant;
+            double projection = vf_dot( w->F_gradient[img], w->tangents[img], n3 );
+            double tc         = k * ( delta_Rx - delta_Rx0 );
+            const double * Ftr = e == 0 ? w->F_tl : w->F_tr;
+            for( long t = 0; t < n3; ++t )
+            {
+                forces[img][t] = rotational_coeff * ( w->F_gradient[img][t] - projection * w->tangents[img][t] )
+                                 + tc * w->tangents[img][t] + Ftr[t];
+                w->F_total[img][t] = forces[img][t];
+            }
+        }
+    }
+}
+
+/* Method_GNEB::Calculate_Force_Virtual, Method_GNEB.cpp:360-391 */
+static void gneb_force_virtual( const orc_gneb * par, int noi, long n3, double * const * conf, double ** forces,
+                                double ** fv )
+{
+    double dtg = par->dt * ORC_GAMMA / ORC_MU_B;
+    for( int i = 0; i < noi; ++i )
+    {
+        if( !par->moving_endpoints && ( i == 0 || i == noi - 1 ) )
+            continue;
+        for( long t = 0; t < n3; t += 3 )
+        {
+            double c[3];
+            v3_cross( conf[i] + t, forces[i] + t, c );
+            for( int q = 0; q < 3; ++q )
+                fv[i][t + q] = dtg * c[q];
+        }
+    }
+}
+
+long orc_gneb_run( orc_sys * s, int solver, const orc_gneb * par, int noi, const int * image_type, double * spins,
+                   long n_iterations, int check_convergence, double * energies, double * Rx, double * max_torque_all,
+                   double * eff_field, double * Rx_interp, double * E_interp )
+{
+    if( noi < 2 )
+        return 0;
+    long n3 = 3L * s->nos;
+    gneb_ws w;
+    memset( &w, 0, sizeof( w ) );
+    w.noi        = noi;
+    w.n3         = n3;
+    w.F_gradient = alloc_fields( noi, n3 );
+    w.F_total    = alloc_fields( noi, n3 );
+    w.F_spring   = alloc_fields( noi, n3 );
+    w.tangents   = alloc_fields( noi, n3 );
+    w.eff        = alloc_fields( noi, n3 );
+    w.f_shrink   = (double *)calloc( (size_t)n3, sizeof( double ) );
+    w.F_tl       = (double *)calloc( (size_t)n3, sizeof( double ) );
+    w.F_tr       = (double *)calloc( (size_t)n3, sizeof( double ) );
+    w.tp         = (double *)calloc( (size_t)n3, sizeof( double ) );
+    w.tm         = (double *)calloc( (size_t)n3, sizeof( double ) );
+    w.lengths    = (double *)calloc( (size_t)noi, sizeof( double ) );
+    w.dE_dRx     = (double *)calloc( (size_t)noi, sizeof( double ) );
+    w.Rx         = (double *)calloc( (size_t)noi, sizeof( double ) );
+    w.E          = (double *)calloc( (size_t)noi, sizeof( double ) );
+
+    double ** conf  = (double **)malloc( sizeof( double * ) * noi );
+    for( int i = 0; i < noi; ++i )
+        conf[i] = spins + (size_t)i * n3;
+    double ** forces = alloc_fields( noi, n3 );
+    double ** fv     = alloc_fields( noi, n3 );
+    double ** pred   = alloc_fields( noi, n3 );
+    double ** fpred  = alloc_fields( noi, n3 );
+    double ** fvpred = alloc_fields( noi, n3 );
+    double ** tmp    = alloc_fields( noi, n3 );
+    double ** vel    = alloc_fields( noi, n3 );
+    double ** fprev  = alloc_fields( noi, n3 );
+    double ** k1 = alloc_fields( noi, n3 ), **k2 = alloc_fields( noi, n3 ), **k3 = alloc_fields( noi, n3 );
+    double * mt = (double *)calloc( (size_t)noi, sizeof( double ) );
+    double * proj = (double *)malloc( sizeof( double ) * (size_t)noi );
+    double * fn2  = (double *)malloc( sizeof( double ) * (size_t)noi );
+    const double m = 1.0;
+
+    double max_torque = par->force_convergence + 1.0;
+    long iteration    = 0;
+    for( iteration = 0; iteration < n_iterations; ++iteration )
+    {
+        if( check_convergence && max_torque < par->force_convergence )
+            break;
+        if( solver == ORC_SOLVER_VP )
+        {
+            /* Solver_VP.hpp:29-114 */
+            for( int i = 0; i < noi; ++i )
+                memcpy( fprev[i], forces[i], sizeof( double ) * n3 );
+            gneb_calc_force( s, par, image_type, &w, conf, conf, forces );
+            if( w.stop )
+                break;
+            gneb_force_virtual( par, noi, n3, conf, forces, fv );
+            double proj_full = 0, fn2_full = 0;
+            for( int i = 0; i < noi; ++i )
+            {
+                for( long t = 0; t < n3; ++t )
+                    vel[i][t] += 0.5 / m * ( fprev[i][t] + forces[i][t] );
+                proj[i] = vf_dot( vel[i], forces[i], n3 );
+                fn2[i]  = vf_dot( forces[i], forces[i], n3 );
+            }
+            for( int i = 0; i < noi; ++i )
+            {
+                proj_full += proj[i];
+                fn2_full += fn2[i];
+            }
+            for( int i = 0; i < noi; ++i )
+            {
+                double dt    = par->dt;
+                double ratio = proj_full / fn2_full;
+                if( proj_full <= 0 )
+                    memset( vel[i], 0, sizeof( double ) * n3 );
+                else
+                    for( long t = 0; t < n3; ++t )
+                        vel[i][t] = forces[i][t] * ratio;
+                for( long t = 0; t < n3; t += 3 )
+                {
+                    double c[3];
+                    for( int q = 0; q < 3; ++q )
+                        c[q] = conf[i][t + q] + dt * vel[i][t + q] + 0.5 / m * dt * forces[i][t + q];
+                    v3_normalize( c );
+                    for( int q = 0; q < 3; ++q )
+                        conf[i][t + q] = c[q];
+                }
+            }
+        }
+        else if( solver == ORC_SOLVER_HEUN || solver == ORC_SOLVER_DEPONDT || solver == ORC_SOLVER_SIB )
+        {
+            gneb_calc_force( s, par, image_type, &w, conf, conf, forces );
+            if( w.stop )
+                break;
+            gneb_force_virtual( par, noi, n3, conf, forces, fv );
+            for( int i = 0; i < noi; ++i )
+                for( long t = 0; t < n3; t += 3 )
+                {
+                    double * sp = pred[i] + t;
+                    const double * x = conf[i] + t;
+                    const double * f = fv[i] + t;
+                    if( solver == ORC_SOLVER_HEUN )
+                    {
+                        neg_cross( x, f, tmp[i] + t );
+                        for( int q = 0; q < 3; ++q )
+                            sp[q] = 1 * x[q];
+                        for( int q = 0; q < 3; ++q )
+                            sp[q] += 1 * tmp[i][t + q];
+                        v3_normalize( sp );
+                    }
+                    else if( solver == ORC_SOLVER_DEPONDT )
+                    {
+                        double axis[3] = { 1 * f[0], 1 * f[1], 1 * f[2] };
+                        double angle   = sqrt( v3_sqnorm( f ) );
+                        v3_normalize( axis );
+                        orc_rotate( x, axis, angle, sp );
+                    }
+                    else
+                    {
+                        orc_sib_transform( x, f, sp );
+                        for( int q = 0; q < 3; ++q )
+                            sp[q] += 1 * x[q];
+                        for( int q = 0; q < 3; ++q )
+                            sp[q] *= 0.5;
+                    }
+                }
+            gneb_calc_force( s, par, image_type, &w, pred, conf, fpred );
+            if( w.stop )
+                break;
+            gneb_force_virtual( par, noi, n3, pred, fpred, fvpred );
+            for( int i = 0; i < noi; ++i )
+                for( long t = 0; t < n3; t += 3 )
+                {
+                    double * x = conf[i] + t;
+                    if( solver == ORC_SOLVER_HEUN )
+                    {
+                        double * c = tmp[i] + t;
+                        double t1[3];
+                        for( int q = 0; q < 3; ++q )
+                            c[q] *= 0.5;
+                        for( int q = 0; q < 3; ++q )
+                            c[q] += 1 * x[q];
+                        neg_cross( pred[i] + t, fvpred[i] + t, t1 );
+                        for( int q = 0; q < 3; ++q )
+                            c[q] += 0.5 * t1[q];
+                        v3_normalize( c );
+                        for( int q = 0; q < 3; ++q )
+                            x[q] = c[q];
+                    }
+                    else if( solver == ORC_SOLVER_DEPONDT )
+                    {
+                        double h[3];
+                        for( int q = 0; q < 3; ++q )
+                            h[q] = 0.5 * fv[i][t + q];
+                        for( int q = 0; q < 3; ++q )
+                            h[q] += 0.5 * fvpred[i][t + q];
+                        double angle = sqrt( v3_sqnorm( h ) );
+                        v3_normalize( h );
+                        orc_rotate( x, h, angle, x );
+                    }
+                    else
+                        orc_sib_transform( x, fvpred[i] + t, x );
+                }
+        }
+        else if( solver == ORC_SOLVER_RK4 )
+        {
+            /* Solver_RK4.hpp:41-147 over all images */
+            double ** ks[3] = { k1, k2, k3 };
+            const double cpred[3] = { 0.5, 0.5, 1.0 };
+            gneb_calc_force( s, par, image_type, &w, conf, conf, forces );
+            if( w.stop )
+                break;
+            gneb_force_virtual( par, noi, n3, conf, forces, fv );
+            double ** c_in  = conf;
+            double ** fv_in = fv;
+            for( int stage = 0; stage < 3 && !w.stop; ++stage )
+            {
+                for( int i = 0; i < noi; ++i )
+                    for( long t = 0; t < n3; t += 3 )
+                    {
+                        neg_cross( c_in[i] + t, fv_in[i] + t, ks[stage][i] + t );
+                        double p[3];
+                        for( int q = 0; q < 3; ++q )
+                            p[q] = 1 * conf[i][t + q];
+                        for( int q = 0; q < 3; ++q )
+                            p[q] += cpred[stage] * ks[stage][i][t + q];
+                        v3_normalize( p );
+                        for( int q = 0; q < 3; ++q )
+                            tmp[i][t + q] = p[q];
+                    }
+                for( int i = 0; i < noi; ++i )
+                    memcpy( pred[i], tmp[i], sizeof( double ) * n3 );
+                gneb_calc_force( s, par, image_type, &w, pred, conf, fpred );
+                gneb_force_virtual( par, noi, n3, pred, fpred, fvpred );
+                c_in  = pred;
+                fv_in = fvpred;
+            }
+            if( w.stop )
+                break;
+            for( int i = 0; i < noi; ++i )
+                for( long t = 0; t < n3; t += 3 )
+                {
+                    double k4[3], c[3];
+                    neg_cross( pred[i] + t, fvpred[i] + t, k4 );
+                    for( int q = 0; q < 3; ++q )
+                        c[q] = 1 * conf[i][t + q];
+                    for( int q = 0; q < 3; ++q )
+                        c[q] += 1.0 / 6.0 * k1[i][t + q];
+                    for( int q = 0; q < 3; ++q )
+                        c[q] += 1.0 / 3.0 * k2[i][t + q];
+                    for( int q = 0; q < 3; ++q )
+                        c[q] += 1.0 / 3.0 * k3[i][t + q];
+                    for( int q = 0; q < 3; ++q )
+                        c[q] += 1.0 / 6.0 * k4[q];
+                    v3_normalize( c );
+                    for( int q = 0; q < 3; ++q )
+                        conf[i][t + q] = c[q];
+                }
+        }
+        /* Hook_Post_Iteration, Method_GNEB.cpp:411-456 */
+        max_torque = 0;
+        for( int img = 0; img < noi; ++img )
+        {
+            vf_project_tangential( w.F_total[img], conf[img], n3 );
+            double mx = 0;
+            for( long t = 0; t < n3; t += 3 )
+                mx = fmax( mx, v3_sqnorm( w.F_total[img] + t ) );
+            mt[img] = sqrt( mx );
+            if( mt[img] > max_torque )
+                max_torque = mt[img];
+            vf_project_tangential( forces[img], conf[img], n3 );
+        }
+    }
+
+    if( energies )
+        memcpy( energies, w.E, sizeof( double ) * noi );
+    if( Rx )
+        memcpy( Rx, w.Rx, sizeof( double ) * noi );
+    if( max_torque_all )
+        memcpy( max_torque_all, mt, sizeof( double ) * noi );
+    if( eff_field )
+        for( int i = 0; i < noi; ++i )
+            memcpy( eff_field + (size_t)i * n3, w.eff[i], sizeof( double ) * n3 );
+    if( Rx_interp && E_interp )
+    {
+        for( int i = 0; i < noi; ++i )
+            w.dE_dRx[i] = vf_dot( w.eff[i], w.tangents[i], n3 );
+        orc_cubic_hermite( w.Rx, w.E, w.dE_dRx, noi, par->n_E_interpolations, Rx_interp, E_interp );
+    }
+    long done = w.stop ? -( iteration + 1 ) : iteration;
+
+    free_fields( w.F_gradient, noi );
+    free_fields( w.F_total, noi );
+    free_fields( w.F_spring, noi );
+    free_fields( w.tangents, noi );
+    free_fields( w.eff, noi );
+    free( w.f_shrink );
+    free( w.F_tl );
+    free( w.F_tr );
+    free( w.tp );
+    free( w.tm );
+    free( w.lengths );
+    free( w.dE_dRx );
+    free( w.Rx );
+    free( w.E );
+    free( conf );
+    free_fields( forces, noi );
+    free_fields( fv, noi );
+    free_fields( pred, noi );
+    free_fields( fpred, noi );
+    free_fields( fvpred, noi );
+    free_fields( tmp, noi );
+    free_fields( vel, noi );
+    free_fields( fprev, noi );
+    free_fields( k1, noi );
+    free_fields( k2, noi );
+    free_fields( k3, noi );
+    free( mt );
+    free( proj );
+    free( fn2 );
+    return done;
+}
+
+/* Configuration_Chain::Homogeneous_Rotation, core/src/utility/Configuration_Chain.cpp:28-75.
+   chain: noi images of nos spins (AoS); images idx_1 < img < idx_2 are overwritten. */
+void orc_homogeneous_rotation( double * chain, int nos, int idx_1, int idx_2 )
+{
+    const double ex[3] = { 1, 0, 0 }, ey[3] = { 0, 1, 0 };
+    long n3 = 3L * nos;
+    for( int i = 0; i < nos; ++i )
+    {
+        const double * s1 = chain + idx_1 * n3 + 3 * i;
+        const double * s2 = chain + idx_2 * n3 + 3 * i;
+        double rot_angle  = v3_angle( s1, s2 );
+        double axis[3];
+        v3_cross( s1, s2, axis );
+        v3_normalize( axis );
+        if( fabs( rot_angle - ORC_PI ) < 1e-4 )
+        {
+            const double * a = ( fabs( v3_dot( s1, ex ) ) - 1 > 1e-4 ) ? ex : ey;
+            axis[0] = a[0], axis[1] = a[1], axis[2] = a[2];
+        }
+        for( int img = idx_1 + 1; img < idx_2; ++img )
+        {
+            double * o = chain + img * n3 + 3 * i;
+            if( rot_angle > 1e-8 )
+            {
+                double angle = rot_angle * (double)( img - idx_1 ) / (double)( idx_2 - idx_1 );
+                orc_rotate( s1, axis, angle, o );
+            }
+            else
+                o[0] = s1[0], o[1] = s1[1], o[2] = s1[2];
+        }
+    }
+}

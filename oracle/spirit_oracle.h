@@ -151,6 +151,41 @@ void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spi
    counter-based generator that replaces the reference's sequential mt19937 stream. */
 void orc_normal3( uint64_t seed, uint64_t step, uint64_t site, double out[3] );
 
+/* ---------------- GNEB ---------------- */
+typedef struct orc_gneb
+{
+    double dt;                      /* LLG time step of the images [ps] (Method_GNEB.cpp:379, Solver_VP.hpp:92) */
+    double spring_constant;
+    double spring_force_ratio;      /* 0: Rx springs; > 0: energy-weighted path lengths */
+    double path_shortening_constant;
+    int moving_endpoints;
+    int translating_endpoints;
+    int escape_first;
+    double delta_Rx_left, delta_Rx_right;
+    int n_E_interpolations;
+    double force_convergence;
+} orc_gneb;
+
+/* Method_GNEB with solver VP / SIB / Depondt / Heun / RK4 (Method_GNEB.cpp:88-456,
+   Solver_*.hpp, Method_Solver.hpp:224-239, Method.cpp:57-113 with n_iterations_amortize = 1).
+   spins: noi images of nos spins (AoS), advanced in place.  image_type: 0 normal,
+   1 climbing, 2 falling, 3 stationary (Spin_System_Chain.hpp:13-19).  A fresh solver state
+   (zero velocities) is used, as each Simulation_GNEB_Start constructs a new method.
+   Outputs (nullable): energies/Rx/max torque per image and effective fields of the last
+   force evaluation; interpolated Rx/E (noi + (noi-1)*n_E_interpolations points).
+   Returns the number of iterations done, or -(iterations+1) if the run stopped because
+   two neighbouring images coincide. */
+long orc_gneb_run( orc_sys * s, int solver, const orc_gneb * par, int noi, const int * image_type, double * spins,
+                   long n_iterations, int check_convergence, double * energies, double * Rx, double * max_torque_all,
+                   double * eff_field, double * Rx_interp, double * E_interp );
+
+/* Cubic_Hermite_Spline::Interpolate, core/src/utility/Cubic_Hermite_Spline.cpp:10-47 */
+void orc_cubic_hermite( const double * x, const double * p, const double * m, int n, int n_interp, double * out_x,
+                        double * out_p );
+
+/* Configuration_Chain::Homogeneous_Rotation, core/src/utility/Configuration_Chain.cpp:28-75 */
+void orc_homogeneous_rotation( double * chain, int nos, int idx_1, int idx_2 );
+
 /* ---------------- FFT (used by the DDI convolution) ---------------- */
 /* In-place complex DFT of length n (interleaved re,im), sign -1 forward, +1 backward, unnormalised. */
 void orc_fft( double * data, int n, int sign );

@@ -100,3 +100,32 @@ def update_data(p_state, idx_chain=-1):
 
 def setup_data(p_state, idx_chain=-1):
     _Setup_Data(ctypes.c_void_p(p_state), _i(idx_chain))
+
+
+_Get_Rx_Interpolated = _fn("Chain_Get_Rx_Interpolated", [_vp, ctypes.POINTER(_f), _i])
+_Get_Energy_Interpolated = _fn("Chain_Get_Energy_Interpolated", [_vp, ctypes.POINTER(_f), _i])
+
+
+def _n_interpolated(p_state, idx_chain):
+    from spirit_amd.parameters import gneb
+    noi = get_noi(p_state, idx_chain)
+    return noi + (noi - 1) * gneb.get_n_energy_interpolations(p_state, idx_chain)
+
+
+def get_reaction_coordinate(p_state, idx_chain=-1):
+    """Reaction coordinate Rx of every image (geodesic path length)."""
+    return get_rx(p_state, idx_chain)
+
+
+def get_reaction_coordinate_interpolated(p_state, idx_chain=-1):
+    n = _n_interpolated(p_state, idx_chain)
+    arr = (_f * n)()
+    _Get_Rx_Interpolated(ctypes.c_void_p(p_state), arr, _i(idx_chain))
+    return [float(x) for x in arr]
+
+
+def get_energy_interpolated(p_state, idx_chain=-1):
+    n = _n_interpolated(p_state, idx_chain)
+    arr = (_f * n)()
+    _Get_Energy_Interpolated(ctypes.c_void_p(p_state), arr, _i(idx_chain))
+    return [float(x) for x in arr]

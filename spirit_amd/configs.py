@@ -66,7 +66,9 @@ def config_text(cfg):
     lines.append("ddi_radius %r" % float(cfg.get("ddi_radius", 0.0)))
     for k in ("llg_seed", "llg_dt", "llg_damping", "llg_temperature", "llg_force_convergence", "llg_n_iterations",
               "llg_n_iterations_log", "llg_n_iterations_amortize", "gneb_spring_constant", "gneb_force_convergence",
-              "gneb_n_iterations", "gneb_n_iterations_log"):
+              "gneb_n_iterations", "gneb_n_iterations_log", "gneb_moving_endpoints", "gneb_translating_endpoints",
+              "gneb_equilibrium_delta_Rx_left", "gneb_equilibrium_delta_Rx_right", "gneb_escape_first",
+              "gneb_n_energy_interpolations"):
         if k in cfg:
             lines.append("%s %s" % (k, repr(cfg[k]) if isinstance(cfg[k], float) else cfg[k]))
     return "\n".join(lines) + "\n"

@@ -43,5 +43,6 @@ bool stop_file_present();
 void gneb_n_shot( State * state, int N );
 void gneb_finalize( State * state );
 void gneb_sync_host( State * state );
+void gneb_notify_changed( State * state ); // GNEB parameters / image types changed on the host
 
 } // namespace spirit_hip

@@ -654,7 +654,18 @@ catch( ... )
 // ===========================================================================
 // Parameters_GNEB.h  (core/src/Spirit/Parameters_GNEB.cpp)
 // ===========================================================================
-#define GNEB_PARAMS auto & p = *state->chain->gneb
+// every setter reaches a running single-shot GNEB method when it returns
+struct GnebNotify
+{
+    State * s;
+    ~GnebNotify()
+    {
+        gneb_notify_changed( s );
+    }
+};
+#define GNEB_PARAMS                                                                                                    \
+    auto & p = *state->chain->gneb;                                                                                    \
+    GnebNotify gneb_notify_on_return{ state }
 
 extern "C" void Parameters_GNEB_Set_Output_Tag( State * state, const char * tag, int idx_chain ) noexcept
 {
@@ -726,26 +737,32 @@ extern "C" void Parameters_GNEB_Set_Climbing_Falling( State * state, int image_t
 try
 {
     API_IMAGE();
+    chain->image_type.resize( chain->noi(), GNEB_NORMAL );
     chain->image_type[idx_image] = image_type;
+    gneb_notify_changed( state );
 }
 catch( ... )
 {
     handle_exception( idx_image, idx_chain );
 }
 
-// Parameters_GNEB.cpp: the highest-energy inner image climbs
+// Parameters_GNEB.cpp:338-363: local energy maxima climb, minima fall, stationary images stay
 extern "C" void Parameters_GNEB_Set_Image_Type_Automatically( State * state, int idx_chain ) noexcept
 try
 {
     auto & c = *state->chain;
+    c.image_type.resize( c.noi(), GNEB_NORMAL );
     for( int img = 1; img < c.noi() - 1; ++img )
     {
         double E0 = c.images[img - 1]->E, E1 = c.images[img]->E, E2 = c.images[img + 1]->E;
         if( E0 < E1 && E1 > E2 )
             c.image_type[img] = GNEB_CLIMBING;
-        else
+        else if( E0 > E1 && E1 < E2 )
+            c.image_type[img] = GNEB_FALLING;
+        else if( c.image_type[img] != GNEB_STATIONARY )
             c.image_type[img] = GNEB_NORMAL;
     }
+    gneb_notify_changed( state );
 }
 catch( ... )
 {

@@ -392,7 +392,7 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
 // ---------------------------------------------------------------------------
 // LLG runner
 // ---------------------------------------------------------------------------
-static void aos_to_soa( const double * aos, double * soa, long long n )
+void aos_to_soa( const double * aos, double * soa, long long n )
 {
     for( long long i = 0; i < n; ++i )
     {
@@ -401,7 +401,7 @@ static void aos_to_soa( const double * aos, double * soa, long long n )
         soa[2 * n + i] = aos[3 * i + 2];
     }
 }
-static void soa_to_aos( const double * soa, double * aos, long long n )
+void soa_to_aos( const double * soa, double * aos, long long n )
 {
     for( long long i = 0; i < n; ++i )
     {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "ddi.hpp"
+#include "gneb.hpp"
 #include "host_core.hpp"
 #include "llg.hpp"
 
@@ -12,6 +13,10 @@
 
 namespace spirit_hip
 {
+
+// one field of n vectors: AoS [x0,y0,z0,x1,...] <-> SoA [x...][y...][z...]
+void aos_to_soa( const double * aos, double * soa, long long n );
+void soa_to_aos( const double * soa, double * aos, long long n );
 
 void hip_check( hipError_t e, const char * what );
 #define HIPCHECK( x ) ::spirit_hip::hip_check( ( x ), #x )
@@ -179,6 +184,22 @@ public:
         return img >= 0 && img < (int)ddi.size() ? ddi[img].get() : nullptr;
     }
 
+    // For methods built on this runner's images (GNEB): the device image table, the DDI
+    // field buffer (nullptr without DDI) and the DDI field of a batched configuration x.
+    const ImageDev * dev_images() const
+    {
+        return b_imgs.as<ImageDev>();
+    }
+    double * dev_ddi_field()
+    {
+        return any_ddi ? b_h.as<double>() : nullptr;
+    }
+    void compute_ddi_field( const double * x )
+    {
+        if( any_ddi )
+            ddi_field( x );
+    }
+
 private:
     hipEvent_t next_event();
     void collect_profile();
@@ -205,6 +226,55 @@ private:
     std::vector<double> obs_host;
     bool any_ddi = false;
     long noise_steps = 0, noise_pos = 0;
+};
+
+// Runs Method_GNEB (core/src/engine/Method_GNEB.cpp) on a chain of images with equal spin
+// counts: every force evaluation, tangent, spring and solver update of all images runs on
+// the GPU, with the chain-level scalars kept on the device.
+class GNEBRunner
+{
+public:
+    GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParameters & par, const std::vector<int> & types,
+                int solver );
+    ~GNEBRunner();
+
+    // Enqueue up to n iterations (Method::Iterate with n_iterations_amortize = 1); with
+    // check_convergence the run stops on the device once the chain's max torque is below the
+    // force convergence.  Returns the number of iterations done.
+    long run( long n, bool check_convergence );
+    // 0 running, 1 converged, 2 stopped because two neighbouring images coincide
+    int status();
+
+    struct ChainData
+    {
+        std::vector<double> E, Rx, dE_dRx, torque;
+        double max_torque = 0;
+    };
+    // energies, reaction coordinates, slopes and torques of the last force evaluation
+    ChainData chain_data();
+    void download_spins();                      // to the images' host mirrors
+    void download_effective_field( int img, double * aos );
+    void upload_spins();
+    void set_types( const std::vector<int> & types );
+    void set_parameters( const GNEBParameters & par );
+    double * dev_spins()
+    {
+        return field->dev_spins();
+    }
+    hipStream_t stream() const
+    {
+        return field->stream;
+    }
+
+private:
+    void calc_force( double * x, bool vp );
+    void iteration();
+    std::unique_ptr<LLGRunner> field; // images, Hamiltonians, DDI plans, the chain's spins
+    std::vector<ImageSpec> specs;
+    int solver = 0, noi = 0, nos = 0, bpi = 0;
+    GnebArgs A{};
+    DeviceBuffer b_eff, b_Fg, b_tau, b_Ft, b_ftr, b_fv, b_fv0, b_v, b_fprev, b_pred, b_tmp, b_part, b_ch, b_types,
+        b_stop, b_iter;
 };
 
 } // namespace spirit_hip

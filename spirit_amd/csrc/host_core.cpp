@@ -734,7 +734,11 @@ GNEBParameters gneb_parameters_from_config( const std::string & path )
     f.read( p.n_iterations_amortize, "gneb_n_iterations_amortize" );
     f.read( p.n_E_interpolations, "gneb_n_energy_interpolations" );
     f.read( p.moving_endpoints, "gneb_moving_endpoints" );
+    f.read( p.equilibrium_delta_Rx_left, "gneb_equilibrium_delta_Rx_left" );
+    f.read( p.equilibrium_delta_Rx_right, "gneb_equilibrium_delta_Rx_right" );
     f.read( p.translating_endpoints, "gneb_translating_endpoints" );
+    // not read by the reference's parser (Configparser.cpp:1031-1057); accepted here so that
+    // Method_GNEB's escape_first branch can be switched on
     f.read( p.escape_first, "gneb_escape_first" );
     f.read( p.spring_force_ratio, "gneb_spring_force_ratio" );
     f.read( p.path_shortening_constant, "gneb_path_shortening_constant" );

@@ -1,0 +1,351 @@
+// Per-spin device helpers shared by the LLG and GNEB kernels (FP64): vector algebra with
+// the reference's (Eigen's) evaluation order, the counter-based thermal generator, the
+// neighbour index, the fused local gradient and the block reductions.
+//
+// Restated reference code (paths relative to the reference checkout):
+//   Hamiltonian_Heisenberg::Gradient_and_Energy  core/src/engine/Hamiltonian_Heisenberg.cpp:704-864
+//   Vectormath::idx_from_pair                    core/include/engine/Vectormath.hpp:437-528
+//   Vectormath::rotate, sib_transform            core/src/engine/Vectormath.cpp:474-485,
+//                                                core/src/engine/Solver_Kernels.cpp:15-48
+//   Method_LLG::Calculate_Force_Virtual          core/src/engine/Method_LLG.cpp:132-226
+//   Method_LLG::Prepare_Thermal_Field            core/src/engine/Method_LLG.cpp:66-110
+#pragma once
+
+#include "llg.hpp"
+
+namespace spirit_hip
+{
+
+// ---------------------------------------------------------------------------
+// small vector helpers with the reference's (Eigen's) evaluation order
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dot3( const double * a, const double * b )
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ __forceinline__ void cross3( const double * a, const double * b, double * o )
+{
+    double x = a[1] * b[2] - a[2] * b[1];
+    double y = a[2] * b[0] - a[0] * b[2];
+    double z = a[0] * b[1] - a[1] * b[0];
+    o[0] = x, o[1] = y, o[2] = z;
+}
+__device__ __forceinline__ void normalize3( double * a )
+{
+    double z = dot3( a, a );
+    if( z > 0 )
+    {
+        double n = sqrt( z );
+        a[0] /= n;
+        a[1] /= n;
+        a[2] /= n;
+    }
+}
+// Rodrigues rotation, Vectormath.cpp:474-479
+__device__ __forceinline__ void rotate3( const double * v, const double * axis, double angle, double * out )
+{
+    double sn, c;
+    sincos( angle, &sn, &c );
+    double cr[3];
+    cross3( axis, v, cr );
+    double d = dot3( axis, v );
+    double o[3];
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        o[q] = v[q] * c + cr[q] * sn + axis[q] * d * ( 1 - c );
+    out[0] = o[0], out[1] = o[1], out[2] = o[2];
+}
+// sib_transform, Solver_Kernels.cpp:15-48
+__device__ __forceinline__ void sib3( const double * s, const double * f, double * out )
+{
+    double A[3] = { 0.5 * f[0], 0.5 * f[1], 0.5 * f[2] };
+    double an   = sqrt( dot3( A, A ) );
+    double detAi = 1.0 / ( 1 + an * an );
+    double c[3], a2[3];
+    cross3( s, A, c );
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        a2[q] = s[q] - c[q];
+    double o0 = ( a2[0] * ( A[0] * A[0] + 1 ) + a2[1] * ( A[0] * A[1] - A[2] ) + a2[2] * ( A[0] * A[2] + A[1] ) ) * detAi;
+    double o1 = ( a2[0] * ( A[1] * A[0] + A[2] ) + a2[1] * ( A[1] * A[1] + 1 ) + a2[2] * ( A[1] * A[2] - A[0] ) ) * detAi;
+    double o2 = ( a2[0] * ( A[2] * A[0] - A[1] ) + a2[1] * ( A[2] * A[1] + A[0] ) + a2[2] * ( A[2] * A[2] + 1 ) ) * detAi;
+    out[0] = o0, out[1] = o1, out[2] = o2;
+}
+__device__ __forceinline__ void neg_cross3( const double * a, const double * b, double * o )
+{
+    double c[3];
+    cross3( a, b, c );
+    o[0] = -1 * c[0], o[1] = -1 * c[1], o[2] = -1 * c[2];
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) + Box-Muller: three N(0,1) per (step, site)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox( uint32_t c[4], uint32_t k0, uint32_t k1 )
+{
+#pragma unroll
+    for( int r = 0; r < 10; ++r )
+    {
+        uint32_t hi0 = __umulhi( 0xD2511F53u, c[0] ), lo0 = 0xD2511F53u * c[0];
+        uint32_t hi1 = __umulhi( 0xCD9E8D57u, c[2] ), lo1 = 0xCD9E8D57u * c[2];
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0, c[1] = lo1, c[2] = n2, c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+__device__ __forceinline__ double u53( uint32_t a, uint32_t b )
+{
+    return ( (double)( a >> 5 ) * 67108864.0 + (double)( b >> 6 ) ) * ( 1.0 / 9007199254740992.0 );
+}
+__device__ __forceinline__ void normal3( uint64_t seed, uint64_t step, uint64_t site, double * z )
+{
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)( seed >> 32 );
+    uint32_t c0[4] = { (uint32_t)site, (uint32_t)( site >> 32 ), (uint32_t)step, (uint32_t)( step >> 32 ) };
+    uint32_t c1[4] = { c0[0], c0[1] | 0x80000000u, c0[2], c0[3] };
+    philox( c0, k0, k1 );
+    philox( c1, k0, k1 );
+    double u1 = 1.0 - u53( c0[0], c0[1] );
+    double u2 = u53( c0[2], c0[3] );
+    double u3 = 1.0 - u53( c1[0], c1[1] );
+    double u4 = u53( c1[2], c1[3] );
+    double r1 = sqrt( -2.0 * log( u1 ) );
+    double r2 = sqrt( -2.0 * log( u3 ) );
+    z[0]      = r1 * cos( 2.0 * Pi * u2 );
+    z[1]      = r1 * sin( 2.0 * Pi * u2 );
+    z[2]      = r2 * cos( 2.0 * Pi * u4 );
+}
+
+// ---------------------------------------------------------------------------
+// neighbour index, Vectormath.hpp:437-528 (defects off)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int pair_neighbour( const HamDev & H, int a, int b, int c, const PairDev & p )
+{
+    const int Na = H.Na, Nb = H.Nb, Nc = H.Nc;
+    if( abs( p.da ) > Na || abs( p.db ) > Nb || abs( p.dc ) > Nc )
+        return -1;
+    int ja = a + p.da, jb = b + p.db, jc = c + p.dc;
+    if( H.bc[0] || ( 0 <= ja && ja < Na ) )
+    {
+        if( ja < 0 )
+            ja += Na;
+        if( ja >= Na )
+            ja -= Na;
+    }
+    else
+        return -1;
+    if( H.bc[1] || ( 0 <= jb && jb < Nb ) )
+    {
+        if( jb < 0 )
+            jb += Nb;
+        if( jb >= Nb )
+            jb -= Nb;
+    }
+    else
+        return -1;
+    if( H.bc[2] || ( 0 <= jc && jc < Nc ) )
+    {
+        if( jc < 0 )
+            jc += Nc;
+        if( jc >= Nc )
+            jc -= Nc;
+    }
+    else
+        return -1;
+    return p.j + H.nb * ( ja + Na * ( jb + Nb * jc ) );
+}
+
+// Gradient_and_Energy for one spin: g (all terms) and its energy share e
+// (1/2 g.s over aniso+exchange+DMI+DDI, plus the cubic and Zeeman energies).
+__device__ __forceinline__ void local_gradient( const HamDev & H, const double * __restrict__ x, int nos, int i,
+                                                const double * __restrict__ hddi, double * g, double & e )
+{
+    const int ib   = i % H.nb;
+    const int cell = i / H.nb;
+    const int a    = cell % H.Na;
+    const int rest = cell / H.Na;
+    const int b    = rest % H.Nb;
+    const int c    = rest / H.Nb;
+    const double s[3] = { x[i], x[nos + i], x[2 * nos + i] };
+    g[0] = g[1] = g[2] = 0;
+    // anisotropy, :785-800
+    for( int k = 0; k < H.n_aniso; ++k )
+    {
+        if( H.aniso_idx[k] != ib )
+            continue;
+        const double * n = H.aniso_n[k];
+        double d         = dot3( n, s );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            g[q] -= 2.0 * H.aniso_K[k] * n[q] * d;
+    }
+    if( H.has_site_K )
+    {
+        double d = dot3( H.site_n, s );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            g[q] -= 2.0 * H.site_K[i] * H.site_n[q] * d;
+    }
+    // exchange, :822-842 (gather over the redundant table)
+    for( int k = 0; k < H.n_exch; ++k )
+    {
+        const PairDev & p = H.exch[k];
+        if( p.i != ib )
+            continue;
+        int j = pair_neighbour( H, a, b, c, p );
+        if( j >= 0 )
+        {
+            g[0] -= p.J * x[j];
+            g[1] -= p.J * x[nos + j];
+            g[2] -= p.J * x[2 * nos + j];
+        }
+    }
+    // DMI, :844-864
+    for( int k = 0; k < H.n_dmi; ++k )
+    {
+        const PairDev & p = H.dmi[k];
+        if( p.i != ib )
+            continue;
+        int j = pair_neighbour( H, a, b, c, p );
+        if( j >= 0 )
+        {
+            const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
+            double cr[3];
+            cross3( sj, p.n, cr );
+#pragma unroll
+            for( int q = 0; q < 3; ++q )
+                g[q] -= p.D * cr[q];
+        }
+    }
+    // DDI field (precomputed by the FFT passes), :995-1013
+    if( hddi )
+    {
+        g[0] -= hddi[i];
+        g[1] -= hddi[nos + i];
+        g[2] -= hddi[2 * nos + i];
+    }
+    e = 0.5 * dot3( g, s );
+    // cubic anisotropy, :802-820 and :342-358
+    for( int k = 0; k < H.n_cubic; ++k )
+    {
+        if( H.cubic_idx[k] != ib )
+            continue;
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            g[q] -= 2.0 * H.cubic_K[k] * pow( s[q], 3.0 );
+        e -= H.cubic_K[k] / 2 * ( pow( s[0], 4.0 ) + pow( s[1], 4.0 ) + pow( s[2], 4.0 ) );
+    }
+    // Zeeman, :768-783 and :745-751
+    if( H.has_zeeman )
+    {
+        const double mu = H.mu[ib];
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            g[q] -= mu * H.B * H.Bn[q];
+        double ext[3] = { H.Bn[0] * H.B, H.Bn[1] * H.B, H.Bn[2] * H.B };
+        e += -mu * dot3( ext, s );
+    }
+}
+
+// Method_LLG::Calculate_Force_Virtual (no STT), Method_LLG.cpp:132-226
+__device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, double mu, const double * s,
+                                               const double * F, const double * xi, double * fv )
+{
+    const double a = P.damping;
+    if( P.direct_minimization || solver == SOLVER_VP )
+    {
+        const double dtg = P.dt * gamma_e / mu_B;
+        double c[3];
+        cross3( s, F, c );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            fv[q] = dtg * c[q];
+        return;
+    }
+    const double dtg = P.dt * gamma_e / mu_B / ( 1 + a * a );
+    double c[3], v[3];
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        v[q] = dtg * F[q];
+    cross3( s, F, c );
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        v[q] += dtg * a * c[q];
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        v[q] /= mu;
+    if( P.thermal )
+    {
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            v[q] += 1 * xi[q];
+        cross3( s, xi, c );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            v[q] += a * c[q];
+    }
+    fv[0] = v[0], fv[1] = v[1], fv[2] = v[2];
+}
+
+// thermal field xi_i = eps * sqrt(T/mu_i) * N(0,1), Method_LLG.cpp:66-110
+__device__ __forceinline__ void thermal_field( const LLGDev & P, double mu, uint64_t step, int i,
+                                               const double * __restrict__ noise, double * xi )
+{
+    double z[3];
+    if( noise )
+    {
+        z[0] = noise[3LL * i + 0];
+        z[1] = noise[3LL * i + 1];
+        z[2] = noise[3LL * i + 2];
+    }
+    else
+        normal3( P.seed, step, (uint64_t)i, z );
+    const double a   = P.damping;
+    const double eps = sqrt( 2 * a * P.dt * gamma_e / mu_B * k_B ) / ( 1 + a * a );
+    const double f   = eps * sqrt( P.temperature / mu );
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        xi[q] = f * z[q];
+}
+
+// block-wide reduction of (sum e, max t) into partials[blockIdx.x]
+__device__ __forceinline__ void block_reduce_store( double e, double t, double * part_E, double * part_T )
+{
+    __shared__ double sE[LLG_THREADS / 64], sT[LLG_THREADS / 64];
+#pragma unroll
+    for( int off = 32; off > 0; off >>= 1 )
+    {
+        e += __shfl_down( e, off, 64 );
+        t = fmax( t, __shfl_down( t, off, 64 ) );
+    }
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if( lane == 0 )
+    {
+        sE[w] = e;
+        sT[w] = t;
+    }
+    __syncthreads();
+    if( threadIdx.x == 0 )
+    {
+        double E = sE[0], T = sT[0];
+        for( int k = 1; k < LLG_THREADS / 64; ++k )
+        {
+            E += sE[k];
+            T = fmax( T, sT[k] );
+        }
+        part_E[blockIdx.x] = E;
+        part_T[blockIdx.x] = T;
+    }
+}
+
+__device__ __forceinline__ void load3( const double * f, int nos, int i, double * v )
+{
+    v[0] = f[i], v[1] = f[nos + i], v[2] = f[2 * nos + i];
+}
+__device__ __forceinline__ void store3( double * f, int nos, int i, const double * v )
+{
+    f[i] = v[0], f[nos + i] = v[1], f[2 * nos + i] = v[2];
+}
+
+
+} // namespace spirit_hip

@@ -81,7 +81,6 @@ struct MethodLLG
 };
 
 // A running GNEB method on the chain, core/src/engine/Method_GNEB.cpp
-class GNEBRunner;
 struct MethodGNEB
 {
     std::unique_ptr<GNEBRunner> runner;
@@ -91,7 +90,8 @@ struct MethodGNEB
     long n_iterations_log = 0;
     double ips            = 0;
     std::vector<double> max_torque_all;
-    double max_torque = 0;
+    double max_torque  = 0;
+    bool device_newer  = false; // device spins newer than the host mirrors
     std::chrono::steady_clock::time_point t_start;
     std::string solver_name;
 };

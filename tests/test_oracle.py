@@ -252,6 +252,93 @@ def test_solvers_minimum_energy(oracle):
     assert abs(float(np.float32(mz)) - G["mz_minimum"]) < G["tolerance"]
 
 
+def _skyrmion_minimum(oracle):
+    G = GOLD["solvers"]
+    cfg = G["cfg"]
+    S = oracle.System(cfg)
+    pos = S.positions()
+    center = 0.5 * (pos.min(axis=0) + pos.max(axis=0))
+    s = _skyrmion(pos, center, G["skyrmion"])
+    done, torque = 0, 1.0
+    while torque >= cfg["llg_force_convergence"] and done < 400000:
+        s, e, torque, _ = S.llg(s, 2, 2000, dt=cfg["llg_dt"], damping=cfg["llg_damping"], direct_minimization=True)
+        done += 2000
+    return S, s
+
+
+@pytest.mark.parametrize("solver", [0, 3, 2])
+def test_gneb_saddle_point_golden(oracle, solver):
+    """core/test/test_solvers.cpp:74-133 (VP, Heun, Depondt): skyrmion collapse over 9 images -- homogeneous
+    rotation to +z, 2e4 GNEB iterations, climbing image set automatically, GNEB to convergence;
+    the highest interior image has the reference's energy and magnetisation (float, 1e-5)."""
+    G = GOLD["solvers"]
+    cfg = G["cfg"]
+    S, s_min = _skyrmion_minimum(oracle)
+    noi = G["n_images"]
+    chain = np.zeros((noi, S.nos, 3))
+    chain[0] = s_min
+    chain[noi - 1] = [0.0, 0.0, 1.0]
+    chain = oracle.homogeneous_rotation(chain, 0, noi - 1)
+    kw = dict(dt=cfg["llg_dt"], spring_constant=cfg["gneb_spring_constant"],
+              force_convergence=cfg["gneb_force_convergence"])
+    chain, data, n1 = S.gneb(chain, solver, 20000, **kw)
+    assert n1 > 0
+    E = data["energies"]
+    types = np.zeros(noi, dtype=np.int32)
+    for i in range(1, noi - 1):  # Parameters_GNEB_Set_Image_Type_Automatically
+        if E[i - 1] < E[i] > E[i + 1]:
+            types[i] = 1
+        elif E[i - 1] > E[i] < E[i + 1]:
+            types[i] = 2
+    assert types.sum() >= 1
+    chain, data, n2 = S.gneb(chain, solver, 20000, image_type=types, **kw)
+    assert 0 < n2 < 20000 and data["max_torque"].max() < cfg["gneb_force_convergence"]
+    E = data["energies"]
+    i_max = 1 + int(np.argmax(E[1:noi - 1]))
+    assert abs(float(np.float32(E[i_max])) - G["energy_saddle"]) < G["tolerance"]
+    mz = 2.0 * chain[i_max][:, 2].mean()
+    assert abs(float(np.float32(mz)) - G["mz_saddle"]) < G["tolerance"]
+    # the climbing image is the one that ended highest, its force is inverted along the path
+    assert types[i_max] == 1
+
+
+def test_cubic_hermite_spline(oracle):
+    """Cubic_Hermite_Spline::Interpolate reproduces a cubic exactly from its values and
+    slopes (tangent sign convention m * (x0 - x1)), and passes through the data points."""
+    f = lambda x: 0.3 * x ** 3 - x ** 2 + 2 * x - 1
+    df = lambda x: 0.9 * x ** 2 - 2 * x + 2
+    x = np.array([0.0, 0.7, 1.5, 2.2])
+    # the reference's slope term is h01*m0*(x0-x1): a curve with slopes -f'(x) reproduces f
+    ox, op = oracle.cubic_hermite(x, f(x), -df(x), 5)
+    assert len(ox) == 4 + 3 * 5
+    np.testing.assert_allclose(op, f(ox), atol=1e-12)
+    np.testing.assert_allclose(ox[::6], x, atol=0)
+
+
+def test_homogeneous_rotation(oracle):
+    """Configuration_Chain::Homogeneous_Rotation: interior images lie on the great circle from
+    image 0 to the last image, at equal angular steps, and stay unit vectors."""
+    rng = np.random.default_rng(3)
+    nos, noi = 50, 7
+    chain = np.zeros((noi, nos, 3))
+    chain[0] = oracle.random_spins(nos, 1)
+    chain[-1] = oracle.random_spins(nos, 2)
+    chain[-1][0] = chain[0][0]  # a parallel pair stays untouched
+    out = oracle.homogeneous_rotation(chain, 0, noi - 1)
+    np.testing.assert_allclose(np.linalg.norm(out, axis=2), 1.0, atol=1e-14)
+    for i in range(nos):
+        a, b = out[0][i], out[-1][i]
+        total = math.acos(np.clip(a @ b, -1, 1))
+        n = np.cross(a, b)
+        for img in range(1, noi - 1):
+            v = out[img][i]
+            if total < 1e-8:
+                np.testing.assert_array_equal(v, a)
+                continue
+            assert abs(math.acos(np.clip(a @ v, -1, 1)) - total * img / (noi - 1)) < 1e-12
+            assert abs(v @ n) < 1e-12 * max(1.0, np.linalg.norm(n))
+
+
 def _skyrmion(pos, center, p):
     """Configurations::Skyrmion (Configurations.cpp:218-270) on a +z background, r_cut = r."""
     s = np.tile([0.0, 0.0, 1.0], (pos.shape[0], 1))
