@@ -5,6 +5,19 @@
 #define SPIRIT_HIP_EXTENSIONS_H
 #include "Spirit_Defines.h"
 
+#include "Simulation.h"
+
+/* For ensembles of independent replicas (uneven lattice sizes, per-replica fields and
+ * temperatures): runs Simulation_LLG_Start(solver_type, n_iterations, n_iterations_log) on the
+ * active image of each of the n_states states, all advanced together -- one launch per solver
+ * stage for every replica.  Each replica keeps its own Hamiltonian, LLG parameters, convergence
+ * test and iteration count; the results land in each state as after Simulation_LLG_Start.
+ * info (nullable) receives n_states run summaries. */
+SPIRIT_API void Spirit_HIP_Simulation_LLG_Start_Batch( State ** states, int n_states, int solver_type,
+                                                      int n_iterations SPIRIT_DEFAULT( -1 ),
+                                                      int n_iterations_log SPIRIT_DEFAULT( -1 ),
+                                                      Simulation_Run_Info * info SPIRIT_DEFAULT( 0 ) ) SPIRIT_NOEXCEPT;
+
 /* For benchmarks and long single-shot loops: mode 1 (default) refreshes the host
  * spin mirror after every Simulation_SingleShot / Simulation_N_Shot call; mode 0
  * keeps the spins on the GPU until a System_* / Quantity_* getter, Simulation_Stop

@@ -429,6 +429,126 @@ catch( ... )
     handle_exception( idx_image, idx_chain );
 }
 
+// Independent LLG runs of the active images of several states, batched into one runner: every
+// stage of every image is one launch, each image keeps its own Hamiltonian, parameters,
+// convergence test and iteration count (as if Simulation_LLG_Start ran on each in turn).
+extern "C" void Spirit_HIP_Simulation_LLG_Start_Batch( State ** states, int n_states, int solver_type,
+                                                      int n_iterations, int n_iterations_log,
+                                                      Simulation_Run_Info * info ) noexcept
+try
+{
+    if( !states || n_states <= 0 )
+        throw api_error( 2, "Spirit_HIP_Simulation_LLG_Start_Batch: no states" );
+    struct Entry
+    {
+        std::shared_ptr<Image> image;
+        long n_iterations = 0, iteration = 0;
+        bool active       = true;
+    };
+    std::vector<Entry> E( n_states );
+    for( int k = 0; k < n_states; ++k )
+    {
+        int idx_image = -1, idx_chain = -1;
+        std::shared_ptr<Chain> chain;
+        from_indices( states[k], idx_image, idx_chain, E[k].image, chain );
+        if( E[k].image->iteration_allowed || chain->iteration_allowed )
+            throw api_error( 3, "Spirit_HIP_Simulation_LLG_Start_Batch: state " + std::to_string( k )
+                                    + " already runs a simulation" );
+        for( int j = 0; j < k; ++j )
+            if( E[j].image == E[k].image )
+                throw api_error( 3, "Spirit_HIP_Simulation_LLG_Start_Batch: an image appears twice" );
+    }
+    std::vector<ImageSpec> specs;
+    long am = 1;
+    for( int k = 0; k < n_states; ++k )
+    {
+        auto & img = *E[k].image;
+        std::lock_guard<std::recursive_mutex> lk( img.mtx );
+        img.iteration_allowed  = true;
+        img.singleshot_allowed = false;
+        if( n_iterations > 0 )
+            img.llg->n_iterations = n_iterations;
+        if( n_iterations_log > 0 )
+            img.llg->n_iterations_log = n_iterations_log;
+        E[k].n_iterations = std::max( 1L, img.llg->n_iterations );
+        if( k == 0 )
+            am = std::max( 1L, std::min( img.llg->n_iterations_amortize, E[k].n_iterations ) );
+        specs.push_back( ImageSpec{ img.geometry, img.hamiltonian, *img.llg, img.spins.data() } );
+    }
+    std::unique_ptr<LLGRunner> runner;
+    auto release = [&]() {
+        for( auto & e : E )
+        {
+            e.image->iteration_allowed  = false;
+            e.image->singleshot_allowed = false;
+        }
+    };
+    try
+    {
+        runner.reset( new LLGRunner( specs, solver_type ) );
+        runner->initial_observe();
+    }
+    catch( ... )
+    {
+        release();
+        throw;
+    }
+    auto t_start = std::chrono::steady_clock::now();
+    for( int k = 0; k < n_states; ++k )
+    {
+        E[k].image->E          = runner->energy( k );
+        E[k].image->max_torque = runner->max_torque( k );
+    }
+    // Method::Iterate per image; convergence is tested on the device every am steps
+    while( true )
+    {
+        long batch = 0;
+        for( int k = 0; k < n_states; ++k )
+        {
+            auto & img = *E[k].image;
+            const bool walltime
+                = img.llg->max_walltime_sec > 0
+                  && std::chrono::duration<double>( std::chrono::steady_clock::now() - t_start ).count()
+                         > (double)img.llg->max_walltime_sec;
+            E[k].active = E[k].active && E[k].iteration < E[k].n_iterations && img.iteration_allowed
+                          && !stop_file_present() && !( img.max_torque < img.llg->force_convergence ) && !walltime;
+            runner->set_image_stopped( k, !E[k].active );
+            if( E[k].active )
+            {
+                long left = ( E[k].n_iterations - E[k].iteration + am - 1 ) / am * am;
+                batch     = batch == 0 ? left : std::min( batch, left );
+            }
+        }
+        if( batch == 0 )
+            break;
+        batch = std::min( batch, std::max( 1L, 2000 / am ) * am );
+        runner->run( batch, (int)am, true, 0.0 );
+        for( int k = 0; k < n_states; ++k )
+        {
+            E[k].iteration += runner->steps_last_run( k );
+            E[k].image->E          = runner->energy( k );
+            E[k].image->max_torque = runner->max_torque( k );
+        }
+    }
+    runner->download_spins();
+    for( int k = 0; k < n_states; ++k )
+        runner->download_effective_field( k, E[k].image->eff_field.data() );
+    release();
+    double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t_start ).count();
+    if( info )
+        for( int k = 0; k < n_states; ++k )
+        {
+            info[k].max_torque       = (float)E[k].image->max_torque;
+            info[k].total_iterations = (int)E[k].iteration;
+            info[k].total_walltime   = (int)ms;
+            info[k].total_ips        = ms > 0 ? float( E[k].iteration / ms * 1000.0 ) : 0.f;
+        }
+}
+catch( ... )
+{
+    handle_exception( -1, -1 );
+}
+
 extern "C" void Simulation_MC_Start( State * state, int n_iterations, int n_iterations_log, bool singleshot,
                                      Simulation_Run_Info * info, int idx_image, int idx_chain ) noexcept
 {

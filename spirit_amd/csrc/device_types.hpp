@@ -78,6 +78,7 @@ struct LLGDev
     int direct_minimization;
     int thermal; // temperature > 0
     uint64_t seed;
+    double force_convergence; // the image stops once its max torque is below this
 };
 
 // One image of a batched launch: where its fields live and how it is parameterised.

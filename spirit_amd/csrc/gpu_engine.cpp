@@ -420,6 +420,7 @@ static LLGDev make_llg_dev( const LLGParameters & p )
     L.direct_minimization = p.direct_minimization ? 1 : 0;
     L.thermal             = p.temperature > 0 ? 1 : 0;
     L.seed                = p.seed;
+    L.force_convergence   = p.force_convergence;
     return L;
 }
 
@@ -481,7 +482,14 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
     b_pE.resize( sizeof( double ) * n_blocks );
     b_pT.resize( sizeof( double ) * n_blocks );
     b_stop.resize( sizeof( int ) );
-    b_stopstep.resize( sizeof( long long ) );
+    b_stopstep.resize( sizeof( long long ) * specs.size() );
+    b_img_stop.resize( sizeof( int ) * specs.size() );
+    b_nstop.resize( sizeof( int ) );
+    HIPCHECK( hipMemset( b_img_stop.as<void>(), 0, sizeof( int ) * specs.size() ) );
+    HIPCHECK( hipMemset( b_nstop.as<void>(), 0, sizeof( int ) ) );
+    HIPCHECK( hipMemset( b_stop.as<void>(), 0, sizeof( int ) ) );
+    img_stopped.assign( specs.size(), 0 );
+    steps_last.assign( specs.size(), 0 );
     b_obs.resize( sizeof( double ) * 2 * specs.size() );
     HIPCHECK( hipMemset( b_obs.as<void>(), 0, sizeof( double ) * 2 * specs.size() ) );
     HIPCHECK( hipMemset( b_eff.as<void>(), 0, f3 ) );
@@ -662,7 +670,7 @@ void LLGRunner::ddi_field( const double * x, bool prof_events )
 static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuffer & s, DeviceBuffer & sp,
                               DeviceBuffer & sp2, DeviceBuffer & aux, DeviceBuffer & aux2, DeviceBuffer & h,
                               DeviceBuffer & eff, DeviceBuffer & efft, DeviceBuffer & pE, DeviceBuffer & pT,
-                              DeviceBuffer & stop )
+                              DeviceBuffer & stop, DeviceBuffer & img_stop )
 {
     StepArgs A{};
     A.imgs    = imgs.as<ImageDev>();
@@ -677,15 +685,18 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
     A.eff_tmp = efft.as<double>();
     A.part_E  = pE.as<double>();
     A.part_T  = pT.as<double>();
-    A.stop    = stop.as<int>();
+    A.stop     = stop.as<int>();
+    A.img_stop = img_stop.as<int>();
     return A;
 }
+
+#define STEP_ARGS()                                                                                                    \
+    base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop )
 
 void LLGRunner::step( bool obs, bool count, bool check, double conv )
 {
     (void)count;
-    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
-                               b_stop );
+    StepArgs A = STEP_ARGS();
     A.step  = step_counter;
     A.obs   = obs ? 1 : 0;
     A.noise = nullptr;
@@ -714,7 +725,8 @@ void LLGRunner::step( bool obs, bool count, bool check, double conv )
             llg_launch_stage( solver, st, A, n_blocks, stream );
     }
     if( obs )
-        llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_stopstep.as<long long>(),
+        llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_img_stop.as<int>(), b_nstop.as<int>(),
+                             b_stopstep.as<long long>(),
                              (long long)step_counter, conv, check ? 1 : 0, stream );
     ++step_counter;
 }
@@ -723,7 +735,18 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
 {
     if( n <= 0 )
         return 0;
-    HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
+    // the host's view of which images have stopped; the device adds those that converge
+    const int n_img = (int)specs.size();
+    int n_stopped   = 0;
+    for( int k = 0; k < n_img; ++k )
+        n_stopped += img_stopped[k] ? 1 : 0;
+    stop_host = n_stopped == n_img ? 1 : 0;
+    nstop_host = n_stopped;
+    HIPCHECK( hipMemcpyAsync( b_img_stop.as<void>(), img_stopped.data(), sizeof( int ) * n_img, hipMemcpyHostToDevice,
+                              stream ) );
+    HIPCHECK( hipMemcpyAsync( b_nstop.as<void>(), &nstop_host, sizeof( int ), hipMemcpyHostToDevice, stream ) );
+    HIPCHECK( hipMemcpyAsync( b_stop.as<void>(), &stop_host, sizeof( int ), hipMemcpyHostToDevice, stream ) );
+    const std::vector<int> was_stopped = img_stopped;
     const uint64_t first = step_counter;
     auto t0              = std::chrono::steady_clock::now();
     for( long it = 0; it < n; ++it )
@@ -732,11 +755,11 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
         step( obs, true, check_convergence && obs, force_convergence );
     }
     HIPCHECK( hipGetLastError() );
-    int stopped         = 0;
-    long long stop_step = 0;
-    HIPCHECK( hipMemcpyAsync( &stopped, b_stop.as<void>(), sizeof( int ), hipMemcpyDeviceToHost, stream ) );
-    HIPCHECK( hipMemcpyAsync( &stop_step, b_stopstep.as<void>(), sizeof( long long ), hipMemcpyDeviceToHost,
+    std::vector<long long> stop_step( n_img, 0 );
+    HIPCHECK( hipMemcpyAsync( img_stopped.data(), b_img_stop.as<void>(), sizeof( int ) * n_img, hipMemcpyDeviceToHost,
                               stream ) );
+    HIPCHECK( hipMemcpyAsync( stop_step.data(), b_stopstep.as<void>(), sizeof( long long ) * n_img,
+                              hipMemcpyDeviceToHost, stream ) );
     HIPCHECK( hipMemcpyAsync( obs_host.data(), b_obs.as<void>(), sizeof( double ) * obs_host.size(),
                               hipMemcpyDeviceToHost, stream ) );
     HIPCHECK( hipStreamSynchronize( stream ) );
@@ -744,24 +767,30 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
     last_step_ms = std::chrono::duration<double, std::milli>( t1 - t0 ).count() / (double)n;
     if( !pending.empty() )
         collect_profile();
-    long done    = n;
-    if( stopped )
+    long done = 0;
+    for( int k = 0; k < n_img; ++k )
     {
-        done         = (long)( stop_step - (long long)first + 1 );
-        step_counter = first + (uint64_t)done;
+        if( was_stopped[k] )
+            steps_last[k] = 0;
+        else if( img_stopped[k] )
+            steps_last[k] = (long)( stop_step[k] - (long long)first + 1 );
+        else
+            steps_last[k] = n;
+        done = std::max( done, steps_last[k] );
     }
+    step_counter = first + (uint64_t)done;
     return done;
 }
 
 void LLGRunner::initial_observe()
 {
-    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
-                               b_stop );
+    StepArgs A = STEP_ARGS();
     if( any_ddi )
         ddi_field( A.s );
     HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
     llg_launch_observe( A, solver, n_blocks, stream );
-    llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_stopstep.as<long long>(), -1, 0.0, 0, stream );
+    llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_img_stop.as<int>(), b_nstop.as<int>(),
+                         b_stopstep.as<long long>(), -1, 0.0, 0, stream );
     HIPCHECK( hipGetLastError() );
     HIPCHECK( hipMemcpyAsync( obs_host.data(), b_obs.as<void>(), sizeof( double ) * obs_host.size(),
                               hipMemcpyDeviceToHost, stream ) );
@@ -770,8 +799,7 @@ void LLGRunner::initial_observe()
 
 double LLGRunner::gradient( int img, double * grad_aos )
 {
-    StepArgs A = base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT,
-                               b_stop );
+    StepArgs A = STEP_ARGS();
     if( any_ddi )
         ddi_field( A.s );
     DeviceBuffer g( sizeof( double ) * 3 * (size_t)n_total );

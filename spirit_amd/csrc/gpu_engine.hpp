@@ -129,9 +129,23 @@ public:
     void refresh_hamiltonian(); // re-upload after parameter changes
 
     // Enqueue n steps.  observe_every > 0 computes energy/torque every that many steps
-    // (always on the last step); with check_convergence the run stops on the device once the
-    // max torque drops below force_convergence.  Returns the number of steps performed.
+    // (always on the last step); with check_convergence an image stops on the device once its
+    // max torque drops below force_convergence (the others continue).  Returns the largest
+    // number of steps any image performed; steps_last_run(img) gives each image's.
     long run( long n, int observe_every, bool check_convergence, double force_convergence );
+    long steps_last_run( int img ) const
+    {
+        return steps_last[img];
+    }
+    // an image that has stopped (converged, or ended by its caller) takes no further steps
+    bool image_stopped( int img ) const
+    {
+        return img_stopped[img] != 0;
+    }
+    void set_image_stopped( int img, bool stopped )
+    {
+        img_stopped[img] = stopped ? 1 : 0;
+    }
     // Energy, max torque and effective field of the current spins (no step taken).
     void initial_observe();
 
@@ -222,8 +236,11 @@ private:
     std::vector<std::unique_ptr<HamUpload>> hams;
     std::vector<std::unique_ptr<DdiPlan>> ddi;
     DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
-        b_noise, b_stopstep;
+        b_noise, b_stopstep, b_img_stop, b_nstop;
     std::vector<double> obs_host;
+    std::vector<int> img_stopped; // host view of the per-image stop flags
+    std::vector<long> steps_last; // steps each image advanced in the last run
+    int stop_host = 0, nstop_host = 0;
     bool any_ddi = false;
     long noise_steps = 0, noise_pos = 0;
 };

@@ -25,7 +25,8 @@ struct StepArgs
     const double * noise;  // injected N(0,1) thermal noise of this step, AoS per spin (nullable)
     double * part_E;       // per-block energy partial sums
     double * part_T;       // per-block max squared torque
-    int * stop;            // set once the run has converged; every stage returns early
+    int * stop;            // set once every image has stopped; every stage returns early
+    const int * img_stop;  // per image: set once that image has stopped (converged or ended)
     uint64_t step;         // global step index (thermal-noise counter)
     int obs;               // compute observables (energy, torque, effective field)
     int pad;
@@ -34,7 +35,11 @@ struct StepArgs
 void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, hipStream_t st );
 void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, int n_blocks, hipStream_t st );
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
-void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, long long * stop_step, long long step_index,
-                          double force_convergence, int check_convergence, hipStream_t st );
+// Per-image energy / max torque into obs[2*img], [2*img+1]; with check_convergence an image whose
+// torque is below force_convergence stops (img_stop, stop_step[img]); the run-wide stop flag is set
+// when n_stopped reaches n_images.
+void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, int * img_stop, int * n_stopped,
+                          long long * stop_step, long long step_index, double force_convergence, int check_convergence,
+                          hipStream_t st );
 
 } // namespace spirit_hip

@@ -29,7 +29,10 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
     if( *A.stop )
         return;
     const int blk           = blockIdx.x;
-    const ImageDev * __restrict__ I = A.imgs + A.blk_img[blk];
+    const int img           = A.blk_img[blk];
+    if( A.img_stop[img] )
+        return;
+    const ImageDev * __restrict__ I = A.imgs + img;
     const HamDev & H        = I->ham;
     const int nos           = H.nos;
     const long long base    = 3 * I->off;
@@ -308,52 +311,49 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_gradient( StepArgs A, const d
     block_reduce_store( e_i, 0.0, A.part_E, A.part_T );
 }
 
-// Per-image reduction of the partials; convergence flag (Method_Solver.hpp:232-239)
-// with the index of the step at which the run converged.
-__global__ void k_finalize( const ImageDev * __restrict__ imgs, int n_images, const double * __restrict__ part_E,
-                            const double * __restrict__ part_T, double * __restrict__ obs, int * __restrict__ stop,
-                            long long * __restrict__ stop_step, long long step_index, double force_convergence,
-                            int check_convergence )
+// Per-image reduction of the partials (one block per image, fixed order); convergence of
+// each image (Method_Solver.hpp:232-239) with the index of the step at which it converged.
+// Once every image has stopped, the run-wide flag stops all further stages.
+__global__ void __launch_bounds__( 256 )
+    k_finalize( const ImageDev * __restrict__ imgs, int n_images, const double * __restrict__ part_E,
+                const double * __restrict__ part_T, double * __restrict__ obs, int * __restrict__ stop,
+                int * __restrict__ img_stop, int * __restrict__ n_stopped, long long * __restrict__ stop_step,
+                long long step_index, double force_convergence, int check_convergence )
 {
     __shared__ double sE[256], sT[256];
-    if( *stop )
+    const int img = blockIdx.x;
+    if( *stop || img_stop[img] )
         return;
-    double Tall = 0;
-    for( int img = 0; img < n_images; ++img )
+    const ImageDev & I = imgs[img];
+    double e = 0, t = 0;
+    for( int k = threadIdx.x; k < I.n_blocks; k += blockDim.x )
     {
-        const ImageDev & I = imgs[img];
-        double e = 0, t = 0;
-        for( int k = threadIdx.x; k < I.n_blocks; k += blockDim.x )
+        e += part_E[I.block_begin + k];
+        t = fmax( t, part_T[I.block_begin + k] );
+    }
+    sE[threadIdx.x] = e;
+    sT[threadIdx.x] = t;
+    __syncthreads();
+    for( int w = blockDim.x / 2; w > 0; w >>= 1 )
+    {
+        if( threadIdx.x < w )
         {
-            e += part_E[I.block_begin + k];
-            t = fmax( t, part_T[I.block_begin + k] );
+            sE[threadIdx.x] += sE[threadIdx.x + w];
+            sT[threadIdx.x] = fmax( sT[threadIdx.x], sT[threadIdx.x + w] );
         }
-        sE[threadIdx.x] = e;
-        sT[threadIdx.x] = t;
-        __syncthreads();
-        for( int w = blockDim.x / 2; w > 0; w >>= 1 )
-        {
-            if( threadIdx.x < w )
-            {
-                sE[threadIdx.x] += sE[threadIdx.x + w];
-                sT[threadIdx.x] = fmax( sT[threadIdx.x], sT[threadIdx.x + w] );
-            }
-            __syncthreads();
-        }
-        if( threadIdx.x == 0 )
-        {
-            obs[2 * img]     = sE[0];
-            obs[2 * img + 1] = sqrt( sT[0] );
-        }
-        Tall = fmax( Tall, sT[0] );
         __syncthreads();
     }
     if( threadIdx.x == 0 )
     {
-        if( check_convergence && sqrt( Tall ) < force_convergence )
+        const double torque = sqrt( sT[0] );
+        obs[2 * img]        = sE[0];
+        obs[2 * img + 1]    = torque;
+        if( check_convergence && torque < I.llg.force_convergence )
         {
-            *stop      = 1;
-            *stop_step = step_index;
+            img_stop[img]  = 1;
+            stop_step[img] = step_index;
+            if( atomicAdd( n_stopped, 1 ) + 1 == n_images )
+                *stop = 1;
         }
     }
 }
@@ -408,11 +408,12 @@ void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, i
     hipLaunchKernelGGL( k_gradient, dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A, x, grad );
 }
 
-void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, long long * stop_step, long long step_index,
-                          double force_convergence, int check_convergence, hipStream_t st )
+void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, int * img_stop, int * n_stopped,
+                          long long * stop_step, long long step_index, double force_convergence, int check_convergence,
+                          hipStream_t st )
 {
-    hipLaunchKernelGGL( k_finalize, dim3( 1 ), dim3( 256 ), 0, st, A.imgs, n_images, A.part_E, A.part_T, obs, A.stop,
-                        stop_step, step_index, force_convergence, check_convergence );
+    hipLaunchKernelGGL( k_finalize, dim3( n_images ), dim3( 256 ), 0, st, A.imgs, n_images, A.part_E, A.part_T, obs,
+                        A.stop, img_stop, n_stopped, stop_step, step_index, force_convergence, check_convergence );
 }
 
 } // namespace spirit_hip

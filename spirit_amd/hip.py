@@ -28,6 +28,20 @@ _Profile_Enable = _fn("Spirit_HIP_Profile_Enable", [_vp, ctypes.c_bool, _i, _i])
 _Profile_Get = _fn("Spirit_HIP_Profile_Get", [_vp, ctypes.c_char_p, _PD, ctypes.POINTER(_i), _i, _i])
 _Profile_Reset = _fn("Spirit_HIP_Profile_Reset", [_vp, _i, _i])
 _DDI_Info = _fn("Spirit_HIP_DDI_Info", [_vp, ctypes.c_char_p, _i, _i, _i], _i)
+_LLG_Start_Batch = _fn("Spirit_HIP_Simulation_LLG_Start_Batch",
+                       [ctypes.POINTER(_vp), _i, _i, _i, _i, ctypes.c_void_p])
+
+
+def llg_start_batch(states, solver_type, n_iterations=-1, n_iterations_log=-1):
+    """Simulation_LLG_Start on the active image of every state in `states`, all replicas advanced
+    together on the GPU.  Returns one simulation_run_info per state."""
+    from spirit_amd import simulation
+    n = len(states)
+    arr = (_vp * n)(*[ctypes.c_void_p(p).value for p in states])
+    infos = (simulation.simulation_run_info * n)()
+    spiritlib.wrap_function(_LLG_Start_Batch, [arr, _i(n), _i(solver_type), _i(n_iterations), _i(n_iterations_log),
+                                               ctypes.cast(infos, ctypes.c_void_p)])
+    return list(infos)
 
 
 def profile_enable(p_state, enable=True, idx_image=-1, idx_chain=-1):

