@@ -64,6 +64,42 @@ SPIRIT_API void Spirit_HIP_Profile_Reset( State * state, int idx_image SPIRIT_DE
 SPIRIT_API int Spirit_HIP_DDI_Info( State * state, char * buffer, int length, int idx_image SPIRIT_DEFAULT( -1 ),
                                     int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* For parameter fitting and inverse design: a differentiable LLG run of one image (the
+ * image's geometry, Hamiltonian and LLG parameters are copied at creation).  solver_type is
+ * Solver_Depondt or Solver_Heun; n_steps steps of LLG dynamics (T >= 0, thermal noise from the
+ * engine's counter-based generator); checkpoints every checkpoint_every steps bound the memory
+ * of the backward pass, which recomputes each segment.
+ * Parameters: J per exchange shell, D per DMI shell, B [T] (field magnitude), K per uniaxial
+ * anisotropy entry, K_site per spin (along the per-site anisotropy axis, default z).
+ * Spin arrays are AoS (nos*3); with on_device = 1 they are device pointers, else host pointers.
+ * Parameter arrays and their gradients are host arrays; any may be NULL.  Functions returning
+ * int return 0 on success (errors are logged). */
+typedef struct Spirit_HIP_Adjoint Spirit_HIP_Adjoint;
+SPIRIT_API Spirit_HIP_Adjoint * Spirit_HIP_Adjoint_Create( State * state, int solver_type, int n_steps,
+                                                           int checkpoint_every, int idx_image SPIRIT_DEFAULT( -1 ),
+                                                           int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+SPIRIT_API void Spirit_HIP_Adjoint_Destroy( Spirit_HIP_Adjoint * handle ) SPIRIT_NOEXCEPT;
+SPIRIT_API int Spirit_HIP_Adjoint_Get_Sizes( Spirit_HIP_Adjoint * handle, int * nos, int * n_exchange_shells,
+                                             int * n_dmi_shells, int * n_anisotropy ) SPIRIT_NOEXCEPT;
+SPIRIT_API int Spirit_HIP_Adjoint_Set_Parameters( Spirit_HIP_Adjoint * handle, const double * J, const double * D,
+                                                  const double * B, const double * K,
+                                                  const double * K_site ) SPIRIT_NOEXCEPT;
+SPIRIT_API int Spirit_HIP_Adjoint_Get_Parameters( Spirit_HIP_Adjoint * handle, double * J, double * D, double * B,
+                                                  double * K, double * K_site ) SPIRIT_NOEXCEPT;
+/* s_N of the run from s_0 (and the checkpoints the backward pass needs). */
+SPIRIT_API int Spirit_HIP_Adjoint_Forward( Spirit_HIP_Adjoint * handle, const double * s0, double * sN,
+                                           int on_device ) SPIRIT_NOEXCEPT;
+/* The discrete adjoint of the last forward run: from dL/ds_N, dL/ds_0 and the parameter
+ * gradients dJ, dD, dB, dK, dK_site. */
+SPIRIT_API int Spirit_HIP_Adjoint_Backward( Spirit_HIP_Adjoint * handle, const double * dL_dsN, double * dL_ds0,
+                                            double * dJ, double * dD, double * dB, double * dK, double * dK_site,
+                                            int on_device ) SPIRIT_NOEXCEPT;
+/* Tangent-linear run: s_N and its directional derivative ds_N along (ds0, dJ, dD, dB, dK, dK_site). */
+SPIRIT_API int Spirit_HIP_Adjoint_Tangent( Spirit_HIP_Adjoint * handle, const double * s0, const double * ds0,
+                                           const double * dJ, const double * dD, const double * dB, const double * dK,
+                                           const double * dK_site, double * sN, double * dsN,
+                                           int on_device ) SPIRIT_NOEXCEPT;
+
 /* Name of the GPU the engine runs on (empty if none is visible). */
 SPIRIT_API const char * Spirit_HIP_Device_Name( void ) SPIRIT_NOEXCEPT;
 

@@ -1,0 +1,216 @@
+"""Differentiable LLG runs on the GPU (engine extension, include/Spirit/Spirit_HIP.h).
+
+``Session`` wraps one Spirit_HIP_Adjoint handle: a fixed-length Depondt or Heun run of one
+image whose final spins can be differentiated with respect to the initial spins and the
+Hamiltonian parameters -- J per exchange shell, D per DMI shell, the field magnitude B [T],
+K per uniaxial anisotropy entry and K_i per site -- by a hand-written discrete adjoint
+(checkpoint-and-recompute), with a tangent-linear run for directional derivatives.
+
+``LLGFunction`` exposes the run as a ``torch.autograd.Function``::
+
+    sess = Session(p_state, solver=SOLVER_DEPONDT, n_steps=500, checkpoint_every=50)
+    J, D, B, K_site = sess.parameter_tensors(requires_grad=True)
+    sN = LLGFunction.apply(sess, s0, J, D, B, K_site)
+    loss = ((sN - target) ** 2).sum(dim=1).mean()
+    loss.backward()            # J.grad, D.grad, B.grad, K_site.grad, s0.grad
+
+Spin tensors are (nos, 3) float64; on a GPU tensor the engine reads and writes device
+memory directly.
+"""
+import ctypes
+
+import numpy as np
+
+from spirit_amd import spiritlib
+
+_spirit = spiritlib.load_spirit_library()
+_vp, _i = ctypes.c_void_p, ctypes.c_int
+_PD = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int)
+
+SOLVER_DEPONDT = 2
+SOLVER_HEUN = 3
+
+
+def _fn(name, argtypes, restype=None):
+    f = getattr(_spirit, name)
+    f.argtypes = argtypes
+    f.restype = restype
+    return f
+
+
+_Create = _fn("Spirit_HIP_Adjoint_Create", [_vp, _i, _i, _i, _i, _i], _vp)
+_Destroy = _fn("Spirit_HIP_Adjoint_Destroy", [_vp])
+_Get_Sizes = _fn("Spirit_HIP_Adjoint_Get_Sizes", [_vp, _PI, _PI, _PI, _PI], _i)
+_Set_Parameters = _fn("Spirit_HIP_Adjoint_Set_Parameters", [_vp, _PD, _PD, _PD, _PD, _PD], _i)
+_Get_Parameters = _fn("Spirit_HIP_Adjoint_Get_Parameters", [_vp, _PD, _PD, _PD, _PD, _PD], _i)
+_Forward = _fn("Spirit_HIP_Adjoint_Forward", [_vp, _vp, _vp, _i], _i)
+_Backward = _fn("Spirit_HIP_Adjoint_Backward", [_vp, _vp, _vp, _PD, _PD, _PD, _PD, _PD, _i], _i)
+_Tangent = _fn("Spirit_HIP_Adjoint_Tangent", [_vp, _vp, _vp, _PD, _PD, _PD, _PD, _PD, _vp, _vp, _i], _i)
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_PD)
+
+
+def _f64(a, n=None):
+    if a is None:
+        return None
+    if hasattr(a, "detach"):
+        a = a.detach().cpu().numpy()
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+class Session:
+    """A differentiable run of the active image of ``p_state`` (its Hamiltonian and LLG
+    parameters are copied when the session is created)."""
+
+    def __init__(self, p_state, solver=SOLVER_DEPONDT, n_steps=100, checkpoint_every=50, idx_image=-1,
+                 idx_chain=-1):
+        self.h = _Create(ctypes.c_void_p(p_state), solver, n_steps, checkpoint_every, idx_image, idx_chain)
+        if not self.h:
+            raise RuntimeError("Spirit_HIP_Adjoint_Create failed (see the log)")
+        nos, ne, nd, na = _i(), _i(), _i(), _i()
+        _Get_Sizes(self.h, ctypes.byref(nos), ctypes.byref(ne), ctypes.byref(nd), ctypes.byref(na))
+        self.nos, self.n_exchange, self.n_dmi, self.n_aniso = nos.value, ne.value, nd.value, na.value
+        self.n_steps = n_steps
+
+    def close(self):
+        if self.h:
+            _Destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters ---------------------------------------------------------
+    def set_parameters(self, J=None, D=None, B=None, K=None, K_site=None):
+        J, D, K = _f64(J, self.n_exchange), _f64(D, self.n_dmi), _f64(K, self.n_aniso)
+        B, Ks = _f64(B, 1), _f64(K_site, self.nos)
+        if _Set_Parameters(self.h, _dp(J), _dp(D), _dp(B), _dp(K), _dp(Ks)) != 0:
+            raise RuntimeError("Spirit_HIP_Adjoint_Set_Parameters failed")
+
+    def get_parameters(self):
+        J, D, B = np.zeros(self.n_exchange), np.zeros(self.n_dmi), np.zeros(1)
+        K, Ks = np.zeros(self.n_aniso), np.zeros(self.nos)
+        _Get_Parameters(self.h, _dp(J), _dp(D), _dp(B), _dp(K), _dp(Ks))
+        return dict(J=J, D=D, B=B, K=K, K_site=Ks)
+
+    def parameter_tensors(self, requires_grad=True, device=None):
+        import torch
+        p = self.get_parameters()
+        return tuple(torch.tensor(p[k], dtype=torch.float64, device=device, requires_grad=requires_grad)
+                     for k in ("J", "D", "B", "K_site"))
+
+    # -- runs -----------------------------------------------------------------
+    @staticmethod
+    def _io(x, nos):
+        """(pointer, on_device, keep-alive) of a (nos, 3) float64 array or tensor."""
+        if hasattr(x, "data_ptr"):
+            import torch
+            t = x.detach().to(torch.float64).contiguous()
+            if t.numel() != 3 * nos:
+                raise ValueError("expected %d spins" % nos)
+            if t.is_cuda:
+                torch.cuda.synchronize()
+                return ctypes.c_void_p(t.data_ptr()), 1, t
+            a = t.numpy()
+        else:
+            a = np.asarray(x, dtype=np.float64)
+        a = np.ascontiguousarray(a.reshape(nos, 3))
+        return ctypes.c_void_p(a.ctypes.data), 0, a
+
+    @staticmethod
+    def _ptr(a):
+        return ctypes.c_void_p(a.data_ptr()) if hasattr(a, "data_ptr") else ctypes.c_void_p(a.ctypes.data)
+
+    def _out_for(self, x, dev):
+        """an output (nos, 3) array on the same side (host / GPU) as the input actually used"""
+        if dev:
+            import torch
+            return torch.empty((self.nos, 3), dtype=torch.float64, device=x.device)
+        out = np.empty((self.nos, 3))
+        return out
+
+    def forward(self, s0):
+        """Final spins after n_steps from s0 (records the checkpoints for backward)."""
+        p, dev, keep = self._io(s0, self.nos)
+        out = self._out_for(s0, dev)
+        if _Forward(self.h, p, self._ptr(out), dev) != 0:
+            raise RuntimeError("Spirit_HIP_Adjoint_Forward failed")
+        del keep
+        return out
+
+    def backward(self, dL_dsN):
+        """dL/ds0 and the parameter gradients of the last forward run, from dL/ds_N."""
+        p, dev, keep = self._io(dL_dsN, self.nos)
+        ds0 = self._out_for(dL_dsN, dev)
+        pd = self._ptr(ds0)
+        dJ, dD, dB = np.zeros(self.n_exchange), np.zeros(self.n_dmi), np.zeros(1)
+        dK, dKs = np.zeros(self.n_aniso), np.zeros(self.nos)
+        if _Backward(self.h, p, pd, _dp(dJ), _dp(dD), _dp(dB), _dp(dK), _dp(dKs), dev) != 0:
+            raise RuntimeError("Spirit_HIP_Adjoint_Backward failed")
+        del keep
+        return dict(s0=ds0, J=dJ, D=dD, B=dB, K=dK, K_site=dKs)
+
+    def tangent(self, s0, ds0, dJ=None, dD=None, dB=None, dK=None, dK_site=None):
+        """(s_N, ds_N): the run and its derivative along (ds0, dJ, dD, dB, dK, dK_site)."""
+        p, dev, keep = self._io(s0, self.nos)
+        pd, dev2, keep2 = self._io(ds0, self.nos)
+        if dev != dev2:
+            raise ValueError("s0 and ds0 must both live on the host or both on the GPU")
+        sN, dsN = self._out_for(s0, dev), self._out_for(s0, dev)
+        ptr = self._ptr
+        dJ, dD, dK = _f64(dJ, self.n_exchange), _f64(dD, self.n_dmi), _f64(dK, self.n_aniso)
+        dB, dKs = _f64(dB, 1), _f64(dK_site, self.nos)
+        if _Tangent(self.h, p, pd, _dp(dJ), _dp(dD), _dp(dB), _dp(dK), _dp(dKs), ptr(sN), ptr(dsN), dev) != 0:
+            raise RuntimeError("Spirit_HIP_Adjoint_Tangent failed")
+        del keep, keep2
+        return sN, dsN
+
+
+def _torch_function():
+    import torch
+
+    class LLGFunction(torch.autograd.Function):
+        """s_N = LLG(s0; J, D, B, K_site) through the engine's differentiable run."""
+
+        @staticmethod
+        def forward(ctx, session, s0, J, D, B, K_site):
+            session.set_parameters(J=J, D=D, B=B, K_site=K_site)
+            sN = session.forward(s0)
+            if not torch.is_tensor(sN):
+                sN = torch.from_numpy(sN)
+            ctx.session = session
+            ctx.devices = (s0.device, J.device)
+            return sN
+
+        @staticmethod
+        def backward(ctx, grad_sN):
+            g = ctx.session.backward(grad_sN.contiguous())
+            ds0 = g["s0"] if torch.is_tensor(g["s0"]) else torch.from_numpy(g["s0"])
+            dev = ctx.devices[1]
+            t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)
+            return None, ds0.to(ctx.devices[0]), t(g["J"]), t(g["D"]), t(g["B"]), t(g["K_site"])
+
+    return LLGFunction
+
+
+class _Lazy:
+    _f = None
+
+    def __getattr__(self, name):
+        if _Lazy._f is None:
+            _Lazy._f = _torch_function()
+        return getattr(_Lazy._f, name)
+
+
+LLGFunction = _Lazy()

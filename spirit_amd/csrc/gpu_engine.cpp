@@ -349,7 +349,8 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
         const Pair & p = h.exch_pairs[k];
         PairDev q{};
         q.i = p.i, q.j = p.j, q.da = p.t[0], q.db = p.t[1], q.dc = p.t[2];
-        q.J = h.exch_J[k];
+        q.J   = h.exch_J[k];
+        q.pad = k < h.exch_shell.size() ? h.exch_shell[k] : 0;
         pv.push_back( q );
     }
     for( size_t k = 0; k < h.dmi_pairs.size(); ++k )
@@ -357,7 +358,8 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
         const Pair & p = h.dmi_pairs[k];
         PairDev q{};
         q.i = p.i, q.j = p.j, q.da = p.t[0], q.db = p.t[1], q.dc = p.t[2];
-        q.D = h.dmi_D[k];
+        q.D   = h.dmi_D[k];
+        q.pad = k < h.dmi_shell.size() ? h.dmi_shell[k] : 0;
         for( int c = 0; c < 3; ++c )
             q.n[c] = h.dmi_n[k][c];
         pv.push_back( q );
@@ -780,6 +782,30 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
     }
     step_counter = first + (uint64_t)done;
     return done;
+}
+
+void LLGRunner::tape_step( uint64_t step, double * s_in, double * s_out, double * F1, double * fv1, double * F2,
+                           double * fv2 )
+{
+    if( solver != SOLVER_DEPONDT && solver != SOLVER_HEUN )
+        throw std::runtime_error( "tape_step: Depondt or Heun only" );
+    StepArgs A = STEP_ARGS();
+    A.s        = s_in;
+    A.s_out    = s_out;
+    A.step     = step;
+    A.obs      = 0;
+    A.noise    = nullptr;
+    HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
+    HIPCHECK( hipMemsetAsync( b_img_stop.as<void>(), 0, sizeof( int ) * specs.size(), stream ) );
+    for( int st = 0; st < 2; ++st )
+    {
+        const double * x = st == 0 ? s_in : A.sp;
+        if( any_ddi )
+            ddi_field( x );
+        A.tape_F  = st == 0 ? F1 : F2;
+        A.tape_fv = st == 0 ? fv1 : fv2;
+        llg_launch_stage( solver, st, A, n_blocks, stream );
+    }
 }
 
 void LLGRunner::initial_observe()

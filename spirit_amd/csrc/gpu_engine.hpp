@@ -204,6 +204,11 @@ public:
     {
         return b_imgs.as<ImageDev>();
     }
+    // host copy of image k's device description (its HamDev holds device pointers)
+    const ImageDev & image_host( int k ) const
+    {
+        return imgs_host[k];
+    }
     double * dev_ddi_field()
     {
         return any_ddi ? b_h.as<double>() : nullptr;
@@ -213,6 +218,11 @@ public:
         if( any_ddi )
             ddi_field( x );
     }
+    // One Depondt / Heun step of a single-image runner from s_in to s_out with step index
+    // `step`, recording the forces and virtual forces of both evaluations (adjoint tape).
+    // The arithmetic is that of run(): same kernels, same thermal-noise counters.
+    void tape_step( uint64_t step, double * s_in, double * s_out, double * F1, double * fv1, double * F2,
+                    double * fv2 );
 
 private:
     hipEvent_t next_event();
@@ -243,6 +253,64 @@ private:
     int stop_host = 0, nstop_host = 0;
     bool any_ddi = false;
     long noise_steps = 0, noise_pos = 0;
+};
+
+// Differentiable LLG run of one image (Depondt or Heun, fixed number of steps): forward with
+// checkpoints, the discrete adjoint (dL/ds0 and dL/d parameters from dL/ds_N) by recomputing
+// each checkpoint segment with a tape, and a tangent-linear (forward-mode) run.  Spin fields
+// crossing this interface are device SoA arrays [x...][y...][z...].
+class AdjointRunner
+{
+public:
+    AdjointRunner( const Geometry & g, const Hamiltonian & h, const LLGParameters & llg, int solver, int n_steps,
+                   int checkpoint_every );
+    ~AdjointRunner();
+    int n_exchange_shells() const
+    {
+        return n_exch_shells;
+    }
+    int n_dmi_shells() const
+    {
+        return n_dmi_sh;
+    }
+    int n_aniso() const
+    {
+        return (int)ham->aniso_idx.size();
+    }
+    int nos() const
+    {
+        return geom->nos;
+    }
+    // parameter values (nullptr keeps): J per exchange shell, D per DMI shell, B [T], K per basis
+    // anisotropy entry, K_i per site (along the per-site axis)
+    void set_parameters( const double * J, const double * D, const double * B, const double * K,
+                         const double * K_site );
+    void get_parameters( double * J, double * D, double * B, double * K, double * K_site ) const;
+    void forward( const double * s0, double * sN );
+    // dL/ds_N -> dL/ds_0 and parameter gradients (host arrays; dK_site is a device array or nullptr)
+    void backward( const double * dL_dsN, double * dL_ds0, double * dJ, double * dD, double * dB, double * dK,
+                   double * dK_site );
+    // tangent-linear run along (ds0, dJ, dD, dB, dK, dK_site): sN and dsN
+    void tangent( const double * s0, const double * ds0, const double * dJ, const double * dD, const double * dB,
+                  const double * dK, const double * dK_site, double * sN, double * dsN );
+    hipStream_t stream() const
+    {
+        return run->stream;
+    }
+
+private:
+    void upload_variants();
+    std::shared_ptr<Geometry> geom;
+    std::shared_ptr<Hamiltonian> ham; // private copy: the parameters being differentiated
+    LLGParameters llg;
+    int solver, n_steps, ckpt;
+    int n_exch_shells = 0, n_dmi_sh = 0;
+    std::vector<double> host_spins;
+    std::unique_ptr<LLGRunner> run;
+    HamUpload hess_up, delta_up;
+    DeviceBuffer b_ckpt, b_tape_s, b_F1, b_fv1, b_F2, b_fv2, b_sp, b_lam, b_ms, b_msp, b_mfv1, b_mg, b_pg, b_hd,
+        b_dsp, b_ds, b_dsn, b_site_zero;
+    bool tape_ok = false;
 };
 
 // Runs Method_GNEB (core/src/engine/Method_GNEB.cpp) on a chain of images with equal spin

@@ -273,35 +273,38 @@ void Hamiltonian::update_interactions( const Geometry & g )
 {
     exch_pairs.clear();
     exch_J.clear();
+    exch_shell.clear();
     if( !exchange_shell_J.empty() )
     {
-        std::vector<int> shells;
-        neighbours_in_shells( g, (int)exchange_shell_J.size(), true, exch_pairs, shells );
+        neighbours_in_shells( g, (int)exchange_shell_J.size(), true, exch_pairs, exch_shell );
         for( size_t k = 0; k < exch_pairs.size(); ++k )
-            exch_J.push_back( exchange_shell_J[shells[k]] );
+            exch_J.push_back( exchange_shell_J[exch_shell[k]] );
     }
     else
     {
         exch_pairs = exch_pairs_in;
         exch_J     = exch_J_in;
         for( size_t k = 0; k < exch_pairs_in.size(); ++k )
+            exch_shell.push_back( (int)k );
+        for( size_t k = 0; k < exch_pairs_in.size(); ++k )
         {
             const Pair & p = exch_pairs_in[k];
             exch_pairs.push_back( Pair{ p.j, p.i, { -p.t[0], -p.t[1], -p.t[2] } } );
             exch_J.push_back( exch_J_in[k] );
+            exch_shell.push_back( (int)k );
         }
     }
     dmi_pairs.clear();
     dmi_D.clear();
     dmi_n.clear();
+    dmi_shell.clear();
     if( !dmi_shell_D.empty() )
     {
-        std::vector<int> shells;
-        neighbours_in_shells( g, (int)dmi_shell_D.size(), true, dmi_pairs, shells );
+        neighbours_in_shells( g, (int)dmi_shell_D.size(), true, dmi_pairs, dmi_shell );
         for( size_t k = 0; k < dmi_pairs.size(); ++k )
         {
             dmi_n.push_back( dmi_normal_from_pair( g, dmi_pairs[k], dmi_chirality ) );
-            dmi_D.push_back( dmi_shell_D[shells[k]] );
+            dmi_D.push_back( dmi_shell_D[dmi_shell[k]] );
         }
     }
     else
@@ -310,11 +313,14 @@ void Hamiltonian::update_interactions( const Geometry & g )
         dmi_D     = dmi_D_in;
         dmi_n     = dmi_n_in;
         for( size_t k = 0; k < dmi_pairs_in.size(); ++k )
+            dmi_shell.push_back( (int)k );
+        for( size_t k = 0; k < dmi_pairs_in.size(); ++k )
         {
             const Pair & p = dmi_pairs_in[k];
             dmi_pairs.push_back( Pair{ p.j, p.i, { -p.t[0], -p.t[1], -p.t[2] } } );
             dmi_D.push_back( dmi_D_in[k] );
             dmi_n.push_back( Vec3{ -dmi_n_in[k][0], -dmi_n_in[k][1], -dmi_n_in[k][2] } );
+            dmi_shell.push_back( (int)k );
         }
     }
     ddi_bc = bc;

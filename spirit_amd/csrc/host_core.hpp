@@ -101,9 +101,11 @@ struct Hamiltonian
     // derived redundant tables, Update_Interactions (Hamiltonian_Heisenberg.cpp:101-198)
     std::vector<Pair> exch_pairs;
     std::vector<double> exch_J;
+    std::vector<int> exch_shell; // shell of each pair (input pair index for pair lists)
     std::vector<Pair> dmi_pairs;
     std::vector<double> dmi_D;
     std::vector<Vec3> dmi_n;
+    std::vector<int> dmi_shell;
 
     int ddi_method = DDI_NONE;
     std::array<int, 3> ddi_images{ 4, 4, 4 };

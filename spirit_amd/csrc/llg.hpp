@@ -27,6 +27,9 @@ struct StepArgs
     double * part_T;       // per-block max squared torque
     int * stop;            // set once every image has stopped; every stage returns early
     const int * img_stop;  // per image: set once that image has stopped (converged or ended)
+    double * s_out;        // where the last stage writes the new spins (nullptr: in place into s)
+    double * tape_F;       // this stage's force F = -g (nullable; adjoint tape)
+    double * tape_fv;      // this stage's virtual force (nullable; adjoint tape)
     uint64_t step;         // global step index (thermal-noise counter)
     int obs;               // compute observables (energy, torque, effective field)
     int pad;

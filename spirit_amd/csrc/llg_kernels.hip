@@ -64,6 +64,11 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
             thermal_field( I->llg, mu, A.step, i, A.noise ? A.noise + base : nullptr, xi );
         double fv[3];
         virtual_force( I->llg, SOLVER, mu, si, F, xi, fv );
+        if( A.tape_F )
+        {
+            store3( A.tape_F + base, nos, i, F );
+            store3( A.tape_fv + base, nos, i, fv );
+        }
         if( STAGE == 0 && A.obs )
             store3( A.eff_tmp + base, nos, i, F );
 
@@ -102,7 +107,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                 double angle = sqrt( dot3( t, t ) );
                 normalize3( t );
                 rotate3( s0, t, angle, snew );
-                store3( A.s + base, nos, i, snew );
+                store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
             }
         }
         else if( SOLVER == SOLVER_HEUN )
@@ -138,7 +143,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                     t[q] += 0.5 * t1[q];
                 normalize3( t );
                 snew[0] = t[0], snew[1] = t[1], snew[2] = t[2];
-                store3( A.s + base, nos, i, snew );
+                store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
             }
         }
         else if( SOLVER == SOLVER_SIB )
@@ -162,7 +167,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                 if( A.obs )
                     load3( A.aux + base, nos, i, fv1 );
                 sib3( s0, fv, snew );
-                store3( A.s + base, nos, i, snew );
+                store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
             }
         }
         else if( SOLVER == SOLVER_RK4 )
@@ -217,7 +222,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                     acc[q] += 1.0 / 6.0 * k[q];
                 normalize3( acc );
                 snew[0] = acc[0], snew[1] = acc[1], snew[2] = acc[2];
-                store3( A.s + base, nos, i, snew );
+                store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
                 if( A.obs )
                     load3( A.aux2 + base, nos, i, fv1 );
             }
