@@ -11,6 +11,13 @@ each with the FFT convolution) on the spins already resident in HBM.
 Prints one JSON line: spin-steps/s, the roofline of the dominant kernel (the DDI
 convolution pass, timed live with HIP events on the engine's stream) and the CPU
 oracle timed on the host cores on a bounded sample of the same workload.
+
+Two more legs report into the same line: "batched" -- 256 independent stochastic
+replicas on L x L x 1 square lattices (L from {32..256}, per-replica B and T,
+Depondt, 500 steps) through one Spirit_HIP_Simulation_LLG_Start_Batch against the
+same replicas run one Simulation_LLG_Start after another; "adjoint" -- 500 steps of
+the default workload forward with checkpoints every 50 steps plus the discrete
+adjoint of a mean-square loss (gradients for J, D, B and every K_i).
 """
 import argparse
 import json
@@ -90,6 +97,104 @@ def run_gpu(args, wl):
                 norm_err=norm_err, setup_s=setup_s, ddi_info=info)
 
 
+def run_batched(n_steps, n_replicas):
+    """The uneven ensemble (n_replicas independent stochastic replicas, L x L x 1 with L from
+    {32..256}, per-replica B and T, Depondt dt=1e-3 ps, n_steps steps): every replica run by its
+    own Simulation_LLG_Start in turn, then all replicas by one Spirit_HIP_Simulation_LLG_Start_Batch.
+    Both timings are the API calls end to end (device set-up, run, spins back in the states)."""
+    from spirit_amd import configs, hip, simulation, state, system
+    cfgs = configs.ensemble(n_replicas)
+    tmp = tempfile.mkdtemp()
+    states, spins0 = [], []
+    t0 = time.perf_counter()
+    for k, cfg in enumerate(cfgs):
+        path = os.path.join(tmp, "r%d.cfg" % k)
+        configs.write_config(cfg, path)
+        p = state.setup(path, True)
+        if not p:
+            raise RuntimeError("State_Setup failed")
+        states.append(p)
+        spins0.append(configs.random_spins(system.get_nos(p), 30000 + k))
+    print("bench: ensemble of %d replicas set up in %.1f s" % (n_replicas, time.perf_counter() - t0), file=sys.stderr,
+          flush=True)
+    nos = sum(len(s) for s in spins0)
+
+    def reset():
+        for p, s in zip(states, spins0):
+            system.get_spin_directions(p)[:] = s
+
+    # warm-up of both paths on a few steps (module loading, first allocations)
+    reset()
+    simulation.start(states[0], simulation.METHOD_LLG, 2, n_iterations=2)
+    hip.llg_start_batch(states[:2], 2, n_iterations=2)
+    reset()
+    t0 = time.perf_counter()
+    for p in states:
+        simulation.start(p, simulation.METHOD_LLG, 2, n_iterations=n_steps)
+    t_seq = time.perf_counter() - t0
+    print("bench: sequential ensemble %.3f s" % t_seq, file=sys.stderr, flush=True)
+    reset()
+    t0 = time.perf_counter()
+    infos = hip.llg_start_batch(states, 2, n_iterations=n_steps)
+    t_bat = time.perf_counter() - t0
+    print("bench: batched ensemble %.3f s" % t_bat, file=sys.stderr, flush=True)
+    done = all(i.total_iterations == n_steps for i in infos)
+    for p in states:
+        state.delete(p)
+    return {"workload": "ensemble%d_depondt" % n_replicas, "replicas": n_replicas, "spins": nos, "steps": n_steps,
+            "value": nos * n_steps / t_bat, "sequential_value": nos * n_steps / t_seq, "unit": "spin-steps/s",
+            "speedup_vs_sequential": t_seq / t_bat, "batched_s": t_bat, "sequential_s": t_seq,
+            "all_replicas_completed": done}
+
+
+def run_adjoint(n_steps, ckpt, plain_ms_per_step):
+    """The differentiable Depondt+DDI run on 256x256x128 (per-site K_i, B=0.2 T, dt 5e-4 ps,
+    alpha 0.3, T=0): forward with a checkpoint every `ckpt` steps, then the discrete adjoint of
+    L = mean |s_N - t|^2 to a seeded target t, spins and dL/ds on the GPU."""
+    import torch
+    from spirit_amd import adjoint, configs, hip, state
+    cfg = dict(configs.CUBE_256)
+    nos = 256 * 256 * 128
+    path = os.path.join(tempfile.mkdtemp(), "adjoint.cfg")
+    configs.write_config(cfg, path)
+    p = state.setup(path, True)
+    if not p:
+        raise RuntimeError("State_Setup failed")
+    hip.set_site_anisotropy(p, configs.site_anisotropy(nos, 20006), (0.0, 0.0, 1.0))
+    t0 = time.perf_counter()
+    sess = adjoint.Session(p, solver=2, n_steps=n_steps, checkpoint_every=ckpt)
+    print("bench: adjoint session set up in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    dev = torch.device("cuda")
+    s0 = torch.tensor(configs.random_spins(nos, 20006), device=dev)
+    target = torch.tensor(configs.random_spins(nos, 20007), device=dev)
+    times = []
+    for rep in range(2):  # the first pass is the warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sN = sess.forward(s0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lam = 2.0 * (sN - target) / nos
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        g = sess.backward(lam)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        times.append((t1 - t0, t3 - t2))
+        print("bench: adjoint pass %d: forward %.3f s, backward %.3f s" % (rep, t1 - t0, t3 - t2), file=sys.stderr,
+              flush=True)
+    fwd, bwd = times[-1]
+    loss = float(((sN - target) ** 2).sum(dim=1).mean())
+    sess.close()
+    state.delete(p)
+    plain = plain_ms_per_step * 1e-3 * n_steps if plain_ms_per_step else None
+    return {"workload": "cube256_depondt_ddi_adjoint", "steps": n_steps, "checkpoint_every": ckpt, "spins": nos,
+            "value": nos * n_steps / (fwd + bwd), "unit": "spin-steps/s (forward + backward)",
+            "forward_s": fwd, "backward_s": bwd, "backward_over_forward": bwd / fwd,
+            "gradient_over_plain_forward": (fwd + bwd) / plain if plain else None,
+            "loss": loss, "dL_dJ": float(g["J"][0]), "dL_dD": float(g["D"][0]), "dL_dB": float(g["B"][0])}
+
+
 PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1"}
 
 
@@ -150,7 +255,14 @@ def main():
     ap.add_argument("--workload", default="cube256_depondt_ddi", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="plain,batched,adjoint",
+                    help="comma-separated: plain (always run), batched (uneven ensemble), adjoint")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=50)
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
               file=sys.stderr)
@@ -192,6 +304,11 @@ def main():
     out["kernel_ms"] = {"ddi_conv_avg": g["conv_ms"] / max(1, g["conv_n"]),
                         "llg_stage_avg": g["stage_ms"] / max(1, g["stage_n"])}
     out["max_norm_error"] = g["norm_err"]
+    if "batched" in legs:
+        out["batched"] = run_batched(args.ensemble_steps, args.replicas)
+    if "adjoint" in legs:
+        out["adjoint"] = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint,
+                                     ms_per_step if args.workload == "cube256_depondt_ddi" else None)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         try:

@@ -974,6 +974,60 @@ void orc_gradient( orc_sys * s, const double * spins, double * grad )
         grad_ddi( s, spins, grad );
 }
 
+/* Deterministic parallel sum: fixed blocks of ORC_SUM_BLOCK terms summed in parallel, the
+ * block sums added in index order, so the result does not depend on the thread count or on
+ * the order in which threads finish (an OpenMP reduction clause guarantees neither). */
+#define ORC_SUM_BLOCK 4096
+struct sum_ctx
+{
+    const orc_sys * s;
+    const double * spins;
+    const double * grad;
+    const double * ext;
+};
+static double det_sum( long n, double ( *term )( const struct sum_ctx *, long ), const struct sum_ctx * ctx )
+{
+    const long nb  = ( n + ORC_SUM_BLOCK - 1 ) / ORC_SUM_BLOCK;
+    double * part  = (double *)malloc( sizeof( double ) * (size_t)( nb > 0 ? nb : 1 ) );
+#pragma omp parallel for schedule( static )
+    for( long b = 0; b < nb; ++b )
+    {
+        const long i1 = ( b + 1 ) * ORC_SUM_BLOCK < n ? ( b + 1 ) * ORC_SUM_BLOCK : n;
+        double t      = 0;
+        for( long i = b * ORC_SUM_BLOCK; i < i1; ++i )
+            t += term( ctx, i );
+        part[b] = t;
+    }
+    double total = 0;
+    for( long b = 0; b < nb; ++b )
+        total += part[b];
+    free( part );
+    return total;
+}
+/* 1/2 g_i . s_i (Hamiltonian_Heisenberg.cpp:732) */
+static double term_half_g_dot_s( const struct sum_ctx * c, long i )
+{
+    return 0.5 * v3_dot( c->grad + 3 * i, c->spins + 3 * i );
+}
+/* cubic anisotropy energy of one cell, -K/2 (sx^4 + sy^4 + sz^4) per cubic atom (:734-742) */
+static double term_cubic_cell( const struct sum_ctx * c, long icell )
+{
+    const orc_params * p = &c->s->p;
+    double e             = 0;
+    for( int ia = 0; ia < p->n_cubic; ++ia )
+    {
+        long i           = icell * p->n_cell_atoms + p->cubic_idx[ia];
+        const double * v = c->spins + 3 * i;
+        e -= p->cubic_K[ia] / 2 * ( pow( v[0], 4.0 ) + pow( v[1], 4.0 ) + pow( v[2], 4.0 ) );
+    }
+    return e;
+}
+/* Zeeman energy -mu_i B . s_i (:749-750) */
+static double term_zeeman( const struct sum_ctx * c, long i )
+{
+    return -c->s->mu_s[i] * v3_dot( c->ext, c->spins + 3 * i );
+}
+
 /* Hamiltonian_Heisenberg.cpp:704-766 */
 void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, double * energy )
 {
@@ -987,25 +1041,11 @@ void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, 
         grad_dmi( s, spins, grad );
     if( s->p.ddi_method != ORC_DDI_NONE )
         grad_ddi( s, spins, grad );
-    double e1 = 0;
-#pragma omp parallel for reduction( + : e1 )
-    for( int i = 0; i < s->nos; ++i )
-        e1 += 0.5 * v3_dot( grad + 3 * i, spins + 3 * i );
-    E += e1;
+    E += det_sum( s->nos, term_half_g_dot_s, &( struct sum_ctx ){ s, spins, grad, NULL } );
     if( s->p.n_cubic > 0 )
     {
         grad_cubic( s, spins, grad );
-        const orc_params * p = &s->p;
-        double ec            = 0;
-#pragma omp parallel for reduction( + : ec )
-        for( int icell = 0; icell < s->n_cells_total; ++icell )
-            for( int ia = 0; ia < p->n_cubic; ++ia )
-            {
-                int i         = icell * p->n_cell_atoms + p->cubic_idx[ia];
-                const double * v = spins + 3 * i;
-                ec -= p->cubic_K[ia] / 2 * ( pow( v[0], 4.0 ) + pow( v[1], 4.0 ) + pow( v[2], 4.0 ) );
-            }
-        E += ec;
+        E += det_sum( s->n_cells_total, term_cubic_cell, &( struct sum_ctx ){ s, spins, grad, NULL } );
     }
     if( has_zeeman( s ) )
     {
@@ -1014,11 +1054,7 @@ void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, 
         for( int q = 0; q < 3; ++q )
             ext[q] = s->p.field_normal[q] * B;
         grad_zeeman( s, grad );
-        double ez = 0;
-#pragma omp parallel for reduction( + : ez )
-        for( int i = 0; i < s->nos; ++i )
-            ez += -s->mu_s[i] * v3_dot( ext, spins + 3 * i );
-        E += ez;
+        E += det_sum( s->nos, term_zeeman, &( struct sum_ctx ){ s, spins, grad, ext } );
     }
     *energy = E;
 }

@@ -129,3 +129,25 @@ def scaled(cfg, n_cells, **kw):
     d["n_basis_cells"] = list(n_cells)
     d.update(kw)
     return d
+
+
+# Uneven ensemble: independent stochastic replicas on periodic L x L x 1 square lattices, L from
+# {32, 48, 64, 96, 128, 192, 256}; J=10, Neel DMI 3, K=0.1 z, B ~ U[0, 2] T z, T ~ U[1, 50] K,
+# alpha=0.2, Depondt, dt=1e-3 ps.
+ENSEMBLE_SIDES = (32, 48, 64, 96, 128, 192, 256)
+
+
+def ensemble(n_replicas=256, seed=20006):
+    """The replica configurations of the uneven ensemble, drawn from a seed (replica k's thermal
+    noise is seeded with seed + k)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = rng.choice(ENSEMBLE_SIDES, n_replicas)
+    fields = rng.uniform(0.0, 2.0, n_replicas)
+    temps = rng.uniform(1.0, 50.0, n_replicas)
+    return [dict(bravais_lattice="sc", n_basis_cells=[int(L), int(L), 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+                 external_field_magnitude=float(B), external_field_normal=[0, 0, 1], anisotropy_magnitude=0.1,
+                 anisotropy_normal=[0, 0, 1], n_shells_exchange=1, jij=[10.0], n_shells_dmi=1, dij=[3.0],
+                 dm_chirality=2, ddi_method="none", llg_dt=1e-3, llg_damping=0.2, llg_temperature=float(T),
+                 llg_seed=seed + k, llg_force_convergence=1e-12)
+            for k, (L, B, T) in enumerate(zip(sides, fields, temps))]
