@@ -97,6 +97,18 @@ def run_gpu(args, wl):
                 norm_err=norm_err, setup_s=setup_s, ddi_info=info)
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of `mib` MiB: under a counter profiler, the copy kernel's
+    FETCH_SIZE / WRITE_SIZE per dispatch against its known size give the counters' scale."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def run_batched(n_steps, n_replicas):
     """The uneven ensemble (n_replicas independent stochastic replicas, L x L x 1 with L from
     {32..256}, per-replica B and T, Depondt dt=1e-3 ps, n_steps steps): every replica run by its
@@ -261,6 +273,9 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--adjoint-checkpoint", type=int, default=50)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device three times (a known-size copy kernel "
+                         "for scaling hardware byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -269,6 +284,8 @@ def main():
         sys.exit(2)
     wl = WORKLOADS[args.workload]
     ddi = args.workload != "square64_depondt"
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     g = run_gpu(args, wl)
     value = g["nos"] * args.steps / g["seconds"]
     b_alg, conv_bytes = alg_bytes_per_spin_step(wl["dims"], wl["solver"], ddi)
