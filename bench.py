@@ -17,7 +17,9 @@ replicas on L x L x 1 square lattices (L from {32..256}, per-replica B and T,
 Depondt, 500 steps) through one Spirit_HIP_Simulation_LLG_Start_Batch against the
 same replicas run one Simulation_LLG_Start after another; "adjoint" -- 500 steps of
 the default workload forward with checkpoints every 50 steps plus the discrete
-adjoint of a mean-square loss (gradients for J, D, B and every K_i).
+adjoint of a mean-square loss (gradients for J, D, B and every K_i); "gneb" -- a
+32-image chain on the 128x128 hexagonal monolayer (skyrmion collapse, climbing
+image, VP), 2000 iterations.
 """
 import argparse
 import json
@@ -207,6 +209,42 @@ def run_adjoint(n_steps, ckpt, plain_ms_per_step):
             "loss": loss, "dL_dJ": float(g["J"][0]), "dL_dD": float(g["D"][0]), "dL_dB": float(g["B"][0])}
 
 
+def run_gneb(n_iter, noi):
+    """GNEB on the 128x128 hexagonal monolayer: noi images from a Neel skyrmion (radius 8) to
+    the ferromagnet by homogeneous rotation, climbing image set automatically after a first
+    relaxation, VP; timed over n_iter iterations of one Simulation_GNEB_Start."""
+    import numpy as np
+    from spirit_amd import chain, configs, configuration, simulation, state, transition
+    from spirit_amd.parameters import gneb
+    path = os.path.join(tempfile.mkdtemp(), "gneb.cfg")
+    configs.write_config(configs.HEX_128, path)
+    p = state.setup(path, True)
+    if not p:
+        raise RuntimeError("State_Setup failed")
+    configuration.plus_z(p)
+    configuration.skyrmion(p, 8.0, 1.0, 0.0, False, False, False)
+    chain.image_to_clipboard(p)
+    for _ in range(noi - 1):
+        chain.insert_image_after(p)
+    chain.jump_to_image(p, noi - 1)
+    configuration.plus_z(p)
+    chain.jump_to_image(p, 0)
+    transition.homogeneous(p, 0, noi - 1)
+    simulation.start(p, simulation.METHOD_GNEB, simulation.SOLVER_VP, n_iterations=200)
+    gneb.set_image_type_automatically(p)
+    t0 = time.perf_counter()
+    info = simulation.start(p, simulation.METHOD_GNEB, simulation.SOLVER_VP, n_iterations=n_iter)
+    t = time.perf_counter() - t0
+    E = np.array(chain.get_energy(p))
+    nos = 128 * 128
+    state.delete(p)
+    return {"workload": "hex128_gneb%d_vp_climbing" % noi, "images": noi, "spins_per_image": nos,
+            "iterations": info.total_iterations, "seconds": t, "iterations_per_s": info.total_iterations / t,
+            "value": nos * noi * info.total_iterations / t, "unit": "spin-iterations/s",
+            "energy_first_last_max_meV": [float(E[0]), float(E[-1]), float(E.max())],
+            "max_torque": float(info.max_torque)}
+
+
 PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1"}
 
 
@@ -267,8 +305,10 @@ def main():
     ap.add_argument("--workload", default="cube256_depondt_ddi", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,adjoint",
-                    help="comma-separated: plain (always run), batched (uneven ensemble), adjoint")
+    ap.add_argument("--legs", default="plain,batched,adjoint,gneb",
+                    help="comma-separated: plain (always run), batched (uneven ensemble), adjoint, gneb")
+    ap.add_argument("--gneb-iterations", type=int, default=2000)
+    ap.add_argument("--gneb-images", type=int, default=32)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--adjoint-steps", type=int, default=500)
@@ -326,6 +366,8 @@ def main():
     if "adjoint" in legs:
         out["adjoint"] = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint,
                                      ms_per_step if args.workload == "cube256_depondt_ddi" else None)
+    if "gneb" in legs:
+        out["gneb"] = run_gneb(args.gneb_iterations, args.gneb_images)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         try:

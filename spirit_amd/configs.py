@@ -151,3 +151,13 @@ def ensemble(n_replicas=256, seed=20006):
                  dm_chirality=2, ddi_method="none", llg_dt=1e-3, llg_damping=0.2, llg_temperature=float(T),
                  llg_seed=seed + k, llg_force_convergence=1e-12)
             for k, (L, B, T) in enumerate(zip(sides, fields, temps))]
+
+
+# GNEB chain: 128x128 hexagonal (60 degrees) monolayer, periodic: J=10, Neel DMI 6, K=0.4 z,
+# B=3 T z, mu_s=2; 32 images from an isolated Neel skyrmion (radius 8) to the ferromagnet,
+# climbing image, VP.
+HEX_128 = dict(bravais_lattice="hex2d", n_basis_cells=[128, 128, 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+               external_field_magnitude=3.0, external_field_normal=[0, 0, 1], anisotropy_magnitude=0.4,
+               anisotropy_normal=[0, 0, 1], n_shells_exchange=1, jij=[10.0], n_shells_dmi=1, dij=[6.0],
+               dm_chirality=2, ddi_method="none", llg_dt=1e-3, gneb_spring_constant=1.0,
+               gneb_force_convergence=1e-12)

@@ -678,9 +678,10 @@ __global__ void __launch_bounds__( 256, 2 ) k_p2_c1( DdiDev d, double2 * __restr
         tw[e] = d.Lc.tw2[e];
     __syncthreads();
     double2 * L = lds + l * LS;
-    // Components are loaded, transformed and stored one after the other (scheduling barriers
-    // keep the next component's loads from being hoisted), so at most two components plus one
-    // transform's working set are live in registers.
+    // All three components' loads are issued before the first transform: the kernel runs at
+    // two blocks per CU and is bound by memory latency, so a block keeps every input value of
+    // its columns in flight at once.  The transforms then run one after the other (scheduling
+    // barriers keep one transform's working set live at a time).
     double2 v0[8], v1[8], v2[8];
     auto load = [&]( double2( &v )[8], int q ) {
 #pragma unroll
@@ -703,12 +704,13 @@ __global__ void __launch_bounds__( 256, 2 ) k_p2_c1( DdiDev d, double2 * __restr
         }
     };
     load( v0, 0 );
+    load( v1, 1 );
+    load( v2, 2 );
+    __builtin_amdgcn_sched_barrier( 0 );
     p2_fft_reg<N, false>( v0, L, tw, t );
     __builtin_amdgcn_sched_barrier( 0 );
-    load( v1, 1 );
     p2_fft_reg<N, false>( v1, L, tw, t );
     __builtin_amdgcn_sched_barrier( 0 );
-    load( v2, 2 );
     p2_fft_reg<N, false>( v2, L, tw, t );
     __builtin_amdgcn_sched_barrier( 0 );
     if( col )

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
+#include <string>
 
 namespace spirit_hip
 {
@@ -255,10 +257,27 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
 }
 
 // Times every pass C kernel that can run the plan on a zero configuration (the run time
-// does not depend on the values) and keeps the fastest.
+// does not depend on the values) and keeps the fastest.  The environment variable
+// SPIRIT_HIP_DDI_PASS_C = generic | lds | reg restricts the choice to that kernel where it
+// can run the plan (for tests of each kernel and for tuning).
 void DdiPlan::choose_pass_c( hipStream_t st )
 {
     auto cand = ddi_pass_c_candidates( d );
+    if( const char * force = std::getenv( "SPIRIT_HIP_DDI_PASS_C" ) )
+    {
+        const std::string f( force );
+        const int want = f == "generic" ? DDI_C_GENERIC : f == "lds" ? DDI_C_P2_LDS : f == "reg" ? DDI_C_P2_REG : -1;
+        std::vector<std::pair<int, int>> keep;
+        for( auto & c : cand )
+            if( c.first == want )
+                keep.push_back( c );
+        if( keep.empty() && want == DDI_C_GENERIC )
+            keep.push_back( { DDI_C_GENERIC, 0 } );
+        if( keep.empty() )
+            log_message( 2, "SPIRIT_HIP_DDI_PASS_C=" + f + " cannot run this DDI plan; timing every candidate" );
+        else
+            cand = keep;
+    }
     d.c_kind  = cand[0].first;
     d.c_tk    = cand[0].second;
     c_ms      = 0;

@@ -308,8 +308,20 @@ __device__ __forceinline__ void thermal_field( const LLGDev & P, double mu, uint
         xi[q] = f * z[q];
 }
 
-// block-wide reduction of (sum e, max t) into partials[blockIdx.x]
-__device__ __forceinline__ void block_reduce_store( double e, double t, double * part_E, double * part_T )
+// The hardware deals consecutive blocks of a launch to the 8 XCDs round-robin.  Renumber
+// them so that each XCD runs one contiguous range of logical blocks: the lattice rows and
+// planes a block's neighbours live in are then read through the same L2.  A bijection on
+// [0, gridDim.x) (a tail of fewer than 8 blocks keeps its numbers).
+__device__ __forceinline__ int xcd_contiguous_block()
+{
+    const int nblk = gridDim.x;
+    const int full = ( nblk / 8 ) * 8;
+    const int b    = blockIdx.x;
+    return b < full ? ( b % 8 ) * ( full / 8 ) + b / 8 : b;
+}
+
+// block-wide reduction of (sum e, max t) into partials[blk]
+__device__ __forceinline__ void block_reduce_store( double e, double t, double * part_E, double * part_T, int blk )
 {
     __shared__ double sE[LLG_THREADS / 64], sT[LLG_THREADS / 64];
 #pragma unroll
@@ -333,8 +345,8 @@ __device__ __forceinline__ void block_reduce_store( double e, double t, double *
             E += sE[k];
             T = fmax( T, sT[k] );
         }
-        part_E[blockIdx.x] = E;
-        part_T[blockIdx.x] = T;
+        part_E[blk] = E;
+        part_T[blk] = T;
     }
 }
 

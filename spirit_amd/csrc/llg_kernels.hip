@@ -28,7 +28,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 {
     if( *A.stop )
         return;
-    const int blk           = blockIdx.x;
+    const int blk           = xcd_contiguous_block();
     const int img           = A.blk_img[blk];
     if( A.img_stop[img] )
         return;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
         }
     }
     if( LAST && A.obs )
-        block_reduce_store( e_i, t_i, A.part_E, A.part_T );
+        block_reduce_store( e_i, t_i, A.part_E, A.part_T, blk );
 }
 
 // The method constructor's force evaluation and Hook_Post_Iteration (Method_LLG.cpp:22-63,
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 // virtual force projected on s, effective field P_perp(s) F.
 __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solver )
 {
-    const int blk           = blockIdx.x;
+    const int blk           = xcd_contiguous_block();
     const ImageDev * __restrict__ I = A.imgs + A.blk_img[blk];
     const HamDev & H        = I->ham;
     const int nos           = H.nos;
@@ -287,7 +287,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solv
             pr[q] = F[q] - d1 * si[q];
         store3( A.eff + base, nos, i, pr );
     }
-    block_reduce_store( e_i, t_i, A.part_E, A.part_T );
+    block_reduce_store( e_i, t_i, A.part_E, A.part_T, blk );
 }
 
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st )
@@ -299,7 +299,7 @@ void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream
 __global__ void __launch_bounds__( LLG_THREADS ) k_gradient( StepArgs A, const double * __restrict__ x0,
                                                             double * __restrict__ grad )
 {
-    const int blk           = blockIdx.x;
+    const int blk           = xcd_contiguous_block();
     const ImageDev * __restrict__ I = A.imgs + A.blk_img[blk];
     const HamDev & H        = I->ham;
     const int nos           = H.nos;
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_gradient( StepArgs A, const d
         e_i = e;
         store3( grad + base, nos, i, g );
     }
-    block_reduce_store( e_i, 0.0, A.part_E, A.part_T );
+    block_reduce_store( e_i, 0.0, A.part_E, A.part_T, blk );
 }
 
 // Per-image reduction of the partials (one block per image, fixed order); convergence of

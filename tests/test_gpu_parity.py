@@ -107,6 +107,28 @@ def test_gradient_and_energy(engine, oracle, name):
     assert abs(E - E0) <= 1e-11 * max(1.0, abs(E0))
 
 
+@pytest.mark.parametrize("kind,code", [("generic", 0), ("lds", 1), ("reg", 2)])
+def test_ddi_pass_c_kernels(engine, oracle, monkeypatch, kind, code):
+    """Every pass C kernel (forced through SPIRIT_HIP_DDI_PASS_C) gives the oracle's dipolar
+    gradient on a 16x12x128 film (padded 32x24x256: power-of-two c columns, mixed-radix b)."""
+    from spirit_amd import hip, simulation
+    print_case("ddi pass C kernel", kind)
+    monkeypatch.setenv("SPIRIT_HIP_DDI_PASS_C", kind)
+    cfg = configs.scaled(configs.FILM_128, [16, 12, 128])
+    e = engine(cfg)
+    S = oracle.System(cfg)
+    s = oracle.random_spins(S.nos, 4321)
+    e.set_spins(s)
+    simulation.start(e.p, simulation.METHOD_LLG, 2, single_shot=True)
+    info = hip.ddi_info(e.p)
+    simulation.stop(e.p)
+    assert ("pass C kernel %d " % code) in info, info
+    g, E = e.gradient()
+    g0, E0 = S.gradient_and_energy(s)
+    assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max()
+    assert abs(E - E0) <= 1e-11 * abs(E0)
+
+
 def test_gradient_site_anisotropy(engine, oracle):
     cfg = configs.scaled(configs.CUBE_256, [16, 12, 6])
     print_case("gradient", "site anisotropy + ddi")
