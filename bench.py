@@ -111,7 +111,7 @@ def copy_calibration(mib):
     del a, b
 
 
-def run_batched(n_steps, n_replicas):
+def run_batched(n_steps, n_replicas, sequential=True):
     """The uneven ensemble (n_replicas independent stochastic replicas, L x L x 1 with L from
     {32..256}, per-replica B and T, Depondt dt=1e-3 ps, n_steps steps): every replica run by its
     own Simulation_LLG_Start in turn, then all replicas by one Spirit_HIP_Simulation_LLG_Start_Batch.
@@ -141,12 +141,14 @@ def run_batched(n_steps, n_replicas):
     reset()
     simulation.start(states[0], simulation.METHOD_LLG, 2, n_iterations=2)
     hip.llg_start_batch(states[:2], 2, n_iterations=2)
-    reset()
-    t0 = time.perf_counter()
-    for p in states:
-        simulation.start(p, simulation.METHOD_LLG, 2, n_iterations=n_steps)
-    t_seq = time.perf_counter() - t0
-    print("bench: sequential ensemble %.3f s" % t_seq, file=sys.stderr, flush=True)
+    t_seq = None
+    if sequential:
+        reset()
+        t0 = time.perf_counter()
+        for p in states:
+            simulation.start(p, simulation.METHOD_LLG, 2, n_iterations=n_steps)
+        t_seq = time.perf_counter() - t0
+        print("bench: sequential ensemble %.3f s" % t_seq, file=sys.stderr, flush=True)
     reset()
     t0 = time.perf_counter()
     infos = hip.llg_start_batch(states, 2, n_iterations=n_steps)
@@ -156,8 +158,9 @@ def run_batched(n_steps, n_replicas):
     for p in states:
         state.delete(p)
     return {"workload": "ensemble%d_depondt" % n_replicas, "replicas": n_replicas, "spins": nos, "steps": n_steps,
-            "value": nos * n_steps / t_bat, "sequential_value": nos * n_steps / t_seq, "unit": "spin-steps/s",
-            "speedup_vs_sequential": t_seq / t_bat, "batched_s": t_bat, "sequential_s": t_seq,
+            "value": nos * n_steps / t_bat, "sequential_value": nos * n_steps / t_seq if t_seq else None,
+            "unit": "spin-steps/s", "speedup_vs_sequential": t_seq / t_bat if t_seq else None, "batched_s": t_bat,
+            "sequential_s": t_seq,
             "all_replicas_completed": done}
 
 
@@ -263,6 +266,18 @@ def pass_c_bytes(dims, info):
     return 96 * Ha * Pb * Nc + 6 * entry * kb * ha * kc, PASS_C_KERNELS.get(kind, str(kind))
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary (rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE, scaled by a copy of known size); None if not recorded."""
+    import re
+    try:
+        text = open(os.path.join(ROOT, "profiles", "hbm_traffic_cube256.md")).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch: %s = (\d+)" % re.escape(kernel), text)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(wl, threads, steps):
     """The CPU oracle (C + OpenMP restatement of the reference algorithm) on the host cores,
     timed on `steps` steps of the same workload after its set-up."""
@@ -311,6 +326,8 @@ def main():
     ap.add_argument("--gneb-images", type=int, default=32)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--no-ensemble-sequential", action="store_true",
+                    help="skip the one-replica-at-a-time comparison run (profiling the batched run alone)")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--adjoint-checkpoint", type=int, default=50)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -352,7 +369,9 @@ def main():
         c_bytes, c_kernel = pass_c_bytes(wl["dims"], g["ddi_info"])
         ach = c_bytes / t / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": c_kernel,
+                           "frac": ach / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(c_kernel) if args.workload == "cube256_depondt_ddi" else None,
+                           "kernel": c_kernel,
                            "alg_bytes_per_launch": c_bytes, "dense_spectrum_bytes_per_launch": conv_bytes,
                            "avg_launch_ms": t * 1e3}
         out["config"]["ddi_plan"] = g["ddi_info"]
@@ -362,7 +381,7 @@ def main():
                         "llg_stage_avg": g["stage_ms"] / max(1, g["stage_n"])}
     out["max_norm_error"] = g["norm_err"]
     if "batched" in legs:
-        out["batched"] = run_batched(args.ensemble_steps, args.replicas)
+        out["batched"] = run_batched(args.ensemble_steps, args.replicas, not args.no_ensemble_sequential)
     if "adjoint" in legs:
         out["adjoint"] = run_adjoint(args.adjoint_steps, args.adjoint_checkpoint,
                                      ms_per_step if args.workload == "cube256_depondt_ddi" else None)

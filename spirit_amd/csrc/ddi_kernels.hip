@@ -644,7 +644,7 @@ __global__ void __launch_bounds__( 512 ) k_p2_b( DdiDev d, double2 * __restrict_
 // transforms the spectrum multiply (3x3, symmetric) runs in registers on the same index set
 // the inverse transforms take as input.
 template<int N, bool REAL>
-__global__ void __launch_bounds__( 256, 2 ) k_p2_c1( DdiDev d, double2 * __restrict__ X, int order_pairs )
+__global__ void __launch_bounds__( 256, 3 ) k_p2_c1( DdiDev d, double2 * __restrict__ X, int order_pairs )
 {
     extern __shared__ double2 lds[];
     constexpr int T = P2<N>::T, CW = P2Col<N>::CWC, LS = P2<N>::LS;

@@ -54,7 +54,7 @@ struct HamDev
     int has_zeeman;
     int has_ddi;
     int has_site_K;
-    int pad0;
+    int dmi_same;            // DMI pair k has exchange pair k's (i, j, translation) for every k < n_dmi
     double B;                // external_field_magnitude * mu_B (Hamiltonian_Heisenberg.cpp:35)
     double Bn[3];            // external_field_normal
     double mu[MAX_BASIS];    // mu_s per basis atom

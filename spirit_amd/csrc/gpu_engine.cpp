@@ -388,6 +388,13 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
             throw std::runtime_error( "pair references a basis atom that does not exist" );
     H.n_exch = (int)h.exch_pairs.size();
     H.n_dmi  = (int)h.dmi_pairs.size();
+    H.dmi_same = H.n_dmi <= H.n_exch ? 1 : 0;
+    for( int k = 0; k < H.n_dmi && H.dmi_same; ++k )
+    {
+        const PairDev &e = pv[k], &m = pv[H.n_exch + k];
+        if( e.i != m.i || e.j != m.j || e.da != m.da || e.db != m.db || e.dc != m.dc )
+            H.dmi_same = 0;
+    }
     pairs.resize( sizeof( PairDev ) * std::max<size_t>( 1, pv.size() ) );
     if( !pv.empty() )
         HIPCHECK( hipMemcpy( pairs.as<void>(), pv.data(), sizeof( PairDev ) * pv.size(), hipMemcpyHostToDevice ) );

@@ -116,6 +116,16 @@ __device__ __forceinline__ void normal3( uint64_t seed, uint64_t step, uint64_t 
     z[2]      = r2 * cos( 2.0 * Pi * u4 );
 }
 
+// SoA field access: component q of spin i at f[q * nos + i]
+__device__ __forceinline__ void load3( const double * f, int nos, int i, double * v )
+{
+    v[0] = f[i], v[1] = f[nos + i], v[2] = f[2 * nos + i];
+}
+__device__ __forceinline__ void store3( double * f, int nos, int i, const double * v )
+{
+    f[i] = v[0], f[nos + i] = v[1], f[2 * nos + i] = v[2];
+}
+
 // ---------------------------------------------------------------------------
 // neighbour index, Vectormath.hpp:437-528 (defects off)
 // ---------------------------------------------------------------------------
@@ -155,6 +165,9 @@ __device__ __forceinline__ int pair_neighbour( const HamDev & H, int a, int b, i
     return p.j + H.nb * ( ja + Na * ( jb + Nb * jc ) );
 }
 
+// neighbours a spin gathers into registers before using them (larger tables loop)
+constexpr int LG_MAX_NB = 6;
+
 // Gradient_and_Energy for one spin: g (all terms) and its energy share e
 // (1/2 g.s over aniso+exchange+DMI+DDI, plus the cubic and Zeeman energies).
 __device__ __forceinline__ void local_gradient( const HamDev & H, const double * __restrict__ x, int nos, int i,
@@ -167,6 +180,31 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
     const int b    = rest % H.Nb;
     const int c    = rest / H.Nb;
     const double s[3] = { x[i], x[nos + i], x[2 * nos + i] };
+    // Every load this spin needs is issued before the first use: the per-site K_i, the DDI
+    // field and the neighbour spins (up to LG_MAX_NB of them gathered into registers), so
+    // the memory latencies overlap instead of adding up pair after pair.
+    const double Ki = H.has_site_K ? H.site_K[i] : 0.0;
+    double hd[3]    = { 0, 0, 0 };
+    if( hddi )
+        load3( hddi, nos, i, hd );
+    // exchange neighbours gathered into registers (tables of up to LG_MAX_NB pairs); a pair
+    // that does not start at this basis atom or leaves an open boundary reads the spin
+    // itself and contributes exactly zero (its loaded values are replaced by 0)
+    const bool gather = H.n_exch <= LG_MAX_NB;
+    double se[LG_MAX_NB][3];
+    if( gather )
+    {
+#pragma unroll
+        for( int k = 0; k < LG_MAX_NB; ++k )
+            if( k < H.n_exch )
+            {
+                const PairDev & p = H.exch[k];
+                const int j       = p.i == ib ? pair_neighbour( H, a, b, c, p ) : -1;
+                load3( x, nos, j >= 0 ? j : i, se[k] );
+                if( j < 0 )
+                    se[k][0] = se[k][1] = se[k][2] = 0.0;
+            }
+    }
     g[0] = g[1] = g[2] = 0;
     // anisotropy, :785-800
     for( int k = 0; k < H.n_aniso; ++k )
@@ -184,45 +222,73 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
         double d = dot3( H.site_n, s );
 #pragma unroll
         for( int q = 0; q < 3; ++q )
-            g[q] -= 2.0 * H.site_K[i] * H.site_n[q] * d;
+            g[q] -= 2.0 * Ki * H.site_n[q] * d;
     }
     // exchange, :822-842 (gather over the redundant table)
-    for( int k = 0; k < H.n_exch; ++k )
+    if( gather )
     {
-        const PairDev & p = H.exch[k];
-        if( p.i != ib )
-            continue;
-        int j = pair_neighbour( H, a, b, c, p );
-        if( j >= 0 )
-        {
-            g[0] -= p.J * x[j];
-            g[1] -= p.J * x[nos + j];
-            g[2] -= p.J * x[2 * nos + j];
-        }
-    }
-    // DMI, :844-864
-    for( int k = 0; k < H.n_dmi; ++k )
-    {
-        const PairDev & p = H.dmi[k];
-        if( p.i != ib )
-            continue;
-        int j = pair_neighbour( H, a, b, c, p );
-        if( j >= 0 )
-        {
-            const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
-            double cr[3];
-            cross3( sj, p.n, cr );
 #pragma unroll
-            for( int q = 0; q < 3; ++q )
-                g[q] -= p.D * cr[q];
-        }
+        for( int k = 0; k < LG_MAX_NB; ++k )
+            if( k < H.n_exch )
+            {
+                const double J = H.exch[k].J;
+#pragma unroll
+                for( int q = 0; q < 3; ++q )
+                    g[q] -= J * se[k][q];
+            }
     }
+    else
+        for( int k = 0; k < H.n_exch; ++k )
+        {
+            const PairDev & p = H.exch[k];
+            if( p.i != ib )
+                continue;
+            int j = pair_neighbour( H, a, b, c, p );
+            if( j >= 0 )
+            {
+                g[0] -= p.J * x[j];
+                g[1] -= p.J * x[nos + j];
+                g[2] -= p.J * x[2 * nos + j];
+            }
+        }
+    // DMI, :844-864 (on the exchange pairs' neighbours when the tables coincide)
+    if( gather && H.dmi_same )
+    {
+#pragma unroll
+        for( int k = 0; k < LG_MAX_NB; ++k )
+            if( k < H.n_dmi )
+            {
+                const PairDev & p = H.dmi[k];
+                double cr[3];
+                cross3( se[k], p.n, cr );
+#pragma unroll
+                for( int q = 0; q < 3; ++q )
+                    g[q] -= p.D * cr[q];
+            }
+    }
+    else
+        for( int k = 0; k < H.n_dmi; ++k )
+        {
+            const PairDev & p = H.dmi[k];
+            if( p.i != ib )
+                continue;
+            int j = pair_neighbour( H, a, b, c, p );
+            if( j >= 0 )
+            {
+                const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
+                double cr[3];
+                cross3( sj, p.n, cr );
+#pragma unroll
+                for( int q = 0; q < 3; ++q )
+                    g[q] -= p.D * cr[q];
+            }
+        }
     // DDI field (precomputed by the FFT passes), :995-1013
     if( hddi )
     {
-        g[0] -= hddi[i];
-        g[1] -= hddi[nos + i];
-        g[2] -= hddi[2 * nos + i];
+        g[0] -= hd[0];
+        g[1] -= hd[1];
+        g[2] -= hd[2];
     }
     e = 0.5 * dot3( g, s );
     // cubic anisotropy, :802-820 and :342-358
@@ -350,14 +416,6 @@ __device__ __forceinline__ void block_reduce_store( double e, double t, double *
     }
 }
 
-__device__ __forceinline__ void load3( const double * f, int nos, int i, double * v )
-{
-    v[0] = f[i], v[1] = f[nos + i], v[2] = f[2 * nos + i];
-}
-__device__ __forceinline__ void store3( double * f, int nos, int i, const double * v )
-{
-    f[i] = v[0], f[nos + i] = v[1], f[2 * nos + i] = v[2];
-}
 
 
 } // namespace spirit_hip

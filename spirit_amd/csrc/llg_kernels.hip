@@ -52,6 +52,13 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
     double e_i = 0, t_i = 0;
     if( act )
     {
+        // the stage's own inputs are requested before the field evaluation needs its loads
+        double s0[3], fv1[3];
+        if( STAGE != 0 )
+            load3( A.s + base, nos, i, s0 );
+        constexpr bool FV1_IN = ( SOLVER == SOLVER_DEPONDT || SOLVER == SOLVER_HEUN ) && STAGE == 1;
+        if( FV1_IN )
+            load3( A.aux + base, nos, i, fv1 );
         double g[3], e;
         local_gradient( H, x, nos, i, ( A.h && H.has_ddi ) ? A.h + base : nullptr, g, e );
         e_i            = e;
@@ -72,15 +79,11 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
         if( STAGE == 0 && A.obs )
             store3( A.eff_tmp + base, nos, i, F );
 
-        double s0[3];
         if( STAGE == 0 )
         {
             s0[0] = si[0], s0[1] = si[1], s0[2] = si[2];
         }
-        else
-            load3( A.s + base, nos, i, s0 );
         double snew[3];
-        double fv1[3];
 
         if( SOLVER == SOLVER_DEPONDT )
         {
@@ -96,7 +99,6 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
             }
             else
             {
-                load3( A.aux + base, nos, i, fv1 );
                 double t[3];
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
@@ -128,7 +130,6 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
             }
             else
             {
-                load3( A.aux + base, nos, i, fv1 );
                 double t[3], t1[3];
                 neg_cross3( s0, fv1, t );
 #pragma unroll
