@@ -15,6 +15,7 @@ namespace spirit_hip
 constexpr int GNEB_THREADS = 256;
 constexpr int GNEB_NPART   = 8;  // partial-sum slots per block
 constexpr int GNEB_NCH     = 32; // scalar slots per image
+constexpr int GNEB_CH_LDS  = 128; // chains up to this many images keep their scalars in LDS in the chain kernels
 
 // partial-sum slots
 enum GnebPart

@@ -540,13 +540,24 @@ __device__ void hermite_point( const double * ch, int noi, int n_interp, int k, 
     op               = h00 * p0 + h10 * p1 + h01 * m0 * ( x0 - x1 ) + h11 * m1 * ( x0 - x1 );
 }
 
-// Per-image sums of partial slots (one wave per image, fixed order), then the chain logic.
+template<int PHASE>
+__device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *red )[256] );
+
+// Per-image sums of partial slots (one wave per image, fixed order), then the chain logic on
+// one thread.  The chain scalars are staged in LDS for that serial part (chains of up to
+// GNEB_CH_LDS images): its dependent reads then cost LDS latency instead of L2 latency.
 template<int PHASE>
 __global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
 {
+    extern __shared__ double ch_lds[];
     if( *A.stop )
         return;
-    const int noi = A.noi;
+    const int noi       = A.noi;
+    const bool staged   = noi <= GNEB_CH_LDS;
+    const int n_scalars = noi * GNEB_NCH;
+    if( staged )
+        for( int e = threadIdx.x; e < n_scalars; e += blockDim.x )
+            ch_lds[e] = A.ch[e];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
     constexpr bool MAXRED = PHASE == GPH_HOOK;
     constexpr int NS      = ( PHASE == GPH_DIST || PHASE == GPH_TRANSLATE || PHASE == GPH_VP ) ? 2
@@ -588,9 +599,18 @@ __global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
                 red[k][img] = s[k];
     }
     __syncthreads();
-    if( threadIdx.x != 0 )
-        return;
-    double * ch = A.ch;
+    if( threadIdx.x == 0 )
+        chain_logic<PHASE>( A, staged ? ch_lds : A.ch, red );
+    __syncthreads();
+    if( staged )
+        for( int e = threadIdx.x; e < n_scalars; e += blockDim.x )
+            A.ch[e] = ch_lds[e];
+}
+
+template<int PHASE>
+__device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *red )[256] )
+{
+    const int noi           = A.noi;
     const GnebParamsDev & P = A.P;
     auto C = [&]( int img, int slot ) -> double & { return ch[img * GNEB_NCH + slot]; };
 
@@ -927,17 +947,18 @@ void gneb_launch_pass( int pass, const GnebArgs & A, hipStream_t st )
 void gneb_launch_chain( int phase, const GnebArgs & A, hipStream_t st )
 {
     const dim3 g( 1 ), b( 1024 );
+    const size_t lds = A.noi <= GNEB_CH_LDS ? sizeof( double ) * A.noi * GNEB_NCH : 0;
     switch( phase )
     {
-        case GPH_DIST: hipLaunchKernelGGL( k_gneb_chain<GPH_DIST>, g, b, 0, st, A ); break;
-        case GPH_TANGENT: hipLaunchKernelGGL( k_gneb_chain<GPH_TANGENT>, g, b, 0, st, A ); break;
-        case GPH_TRANSLATE: hipLaunchKernelGGL( k_gneb_chain<GPH_TRANSLATE>, g, b, 0, st, A ); break;
-        case GPH_SHRINK1: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK1>, g, b, 0, st, A ); break;
-        case GPH_SHRINK2: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK2>, g, b, 0, st, A ); break;
-        case GPH_SHRINK3: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK3>, g, b, 0, st, A ); break;
-        case GPH_SHRINK4: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK4>, g, b, 0, st, A ); break;
-        case GPH_VP: hipLaunchKernelGGL( k_gneb_chain<GPH_VP>, g, b, 0, st, A ); break;
-        case GPH_HOOK: hipLaunchKernelGGL( k_gneb_chain<GPH_HOOK>, g, b, 0, st, A ); break;
+        case GPH_DIST: hipLaunchKernelGGL( k_gneb_chain<GPH_DIST>, g, b, lds, st, A ); break;
+        case GPH_TANGENT: hipLaunchKernelGGL( k_gneb_chain<GPH_TANGENT>, g, b, lds, st, A ); break;
+        case GPH_TRANSLATE: hipLaunchKernelGGL( k_gneb_chain<GPH_TRANSLATE>, g, b, lds, st, A ); break;
+        case GPH_SHRINK1: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK1>, g, b, lds, st, A ); break;
+        case GPH_SHRINK2: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK2>, g, b, lds, st, A ); break;
+        case GPH_SHRINK3: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK3>, g, b, lds, st, A ); break;
+        case GPH_SHRINK4: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK4>, g, b, lds, st, A ); break;
+        case GPH_VP: hipLaunchKernelGGL( k_gneb_chain<GPH_VP>, g, b, lds, st, A ); break;
+        case GPH_HOOK: hipLaunchKernelGGL( k_gneb_chain<GPH_HOOK>, g, b, lds, st, A ); break;
         default: break;
     }
 }

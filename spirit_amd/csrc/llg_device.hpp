@@ -168,6 +168,25 @@ __device__ __forceinline__ int pair_neighbour( const HamDev & H, int a, int b, i
 // neighbours a spin gathers into registers before using them (larger tables loop)
 constexpr int LG_MAX_NB = 6;
 
+// The exchange neighbours of spin i (tables of up to LG_MAX_NB pairs), all loads issued
+// before any is used.  A pair that does not start at this basis atom or leaves an open
+// boundary reads the spin itself and yields exactly zero (its loaded values are replaced).
+__device__ __forceinline__ void gather_exchange_neighbours( const HamDev & H, const double * __restrict__ x, int nos,
+                                                            int i, int ib, int a, int b, int c,
+                                                            double ( &se )[LG_MAX_NB][3] )
+{
+#pragma unroll
+    for( int k = 0; k < LG_MAX_NB; ++k )
+        if( k < H.n_exch )
+        {
+            const PairDev & p = H.exch[k];
+            const int j       = p.i == ib ? pair_neighbour( H, a, b, c, p ) : -1;
+            load3( x, nos, j >= 0 ? j : i, se[k] );
+            if( j < 0 )
+                se[k][0] = se[k][1] = se[k][2] = 0.0;
+        }
+}
+
 // Gradient_and_Energy for one spin: g (all terms) and its energy share e
 // (1/2 g.s over aniso+exchange+DMI+DDI, plus the cubic and Zeeman energies).
 __device__ __forceinline__ void local_gradient( const HamDev & H, const double * __restrict__ x, int nos, int i,
@@ -187,24 +206,10 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
     double hd[3]    = { 0, 0, 0 };
     if( hddi )
         load3( hddi, nos, i, hd );
-    // exchange neighbours gathered into registers (tables of up to LG_MAX_NB pairs); a pair
-    // that does not start at this basis atom or leaves an open boundary reads the spin
-    // itself and contributes exactly zero (its loaded values are replaced by 0)
     const bool gather = H.n_exch <= LG_MAX_NB;
     double se[LG_MAX_NB][3];
     if( gather )
-    {
-#pragma unroll
-        for( int k = 0; k < LG_MAX_NB; ++k )
-            if( k < H.n_exch )
-            {
-                const PairDev & p = H.exch[k];
-                const int j       = p.i == ib ? pair_neighbour( H, a, b, c, p ) : -1;
-                load3( x, nos, j >= 0 ? j : i, se[k] );
-                if( j < 0 )
-                    se[k][0] = se[k][1] = se[k][2] = 0.0;
-            }
-    }
+        gather_exchange_neighbours( H, x, nos, i, ib, a, b, c, se );
     g[0] = g[1] = g[2] = 0;
     // anisotropy, :785-800
     for( int k = 0; k < H.n_aniso; ++k )
