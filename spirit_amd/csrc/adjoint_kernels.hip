@@ -186,7 +186,7 @@ __global__ void __launch_bounds__( ADJ_THREADS ) k_adj_final( AdjArgs A )
     if( i >= A.nos )
         return;
     const int nos   = A.nos;
-    const double mu = A.H.mu[i % A.H.nb];
+    const double mu = A.H.mu[basis_of( A.H, i )];
     double lam[3], s[3], sp[3], fv1[3], fv2[3], F2[3];
     load3( A.lam, nos, i, lam );
     load3( A.s, nos, i, s );
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__( ADJ_THREADS ) k_adj_predictor_back( AdjArgs A
     if( i >= A.nos )
         return;
     const int nos   = A.nos;
-    const double mu = A.H.mu[i % A.H.nb];
+    const double mu = A.H.mu[basis_of( A.H, i )];
     double s[3], sp[3], fv1[3], F1[3], msp[3], ms[3], mfv1[3];
     load3( A.s, nos, i, s );
     load3( A.sp, nos, i, sp );
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__( ADJ_THREADS ) k_adj_hessian( AdjArgs A )
         double v[3], x[3];
         load3( A.mu_g, nos, i, v );
         load3( A.xg, nos, i, x );
-        const int ib   = i % H.nb;
+        const int ib   = basis_of( H, i );
         const int cell = i / H.nb;
         const int a    = cell % H.Na;
         const int rest = cell / H.Na;
@@ -422,7 +422,7 @@ __device__ __forceinline__ void tl_force( const TlArgs & T, int i, const double 
 {
     const int nos   = T.nos;
     const HamDev & H = T.H;
-    const double mu = H.mu[i % H.nb];
+    const double mu = H.mu[basis_of( H, i )];
     double g[3], e, dg[3], gd[3];
     local_gradient( H, x, nos, i, ( T.h && H.has_ddi ) ? T.h : nullptr, g, e );
     local_gradient( T.Hh, dx, nos, i, ( T.hd && H.has_ddi ) ? T.hd : nullptr, dg, e );
@@ -430,7 +430,7 @@ __device__ __forceinline__ void tl_force( const TlArgs & T, int i, const double 
     double xi_[3], dxi[3];
     load3( x, nos, i, xi_ );
     load3( dx, nos, i, dxi );
-    const int ib = i % H.nb;
+    const int ib = basis_of( H, i );
     for( int k = 0; k < H.n_cubic; ++k )
         if( H.cubic_idx[k] == ib )
 #pragma unroll

@@ -34,11 +34,22 @@ enum SolverId
     SOLVER_VP_OSO      = 7,
 };
 
+// Division by an invariant positive integer d for numerators 0 <= n < 2^31:
+// floor(n / d) = (n * m) >> shift with shift = 31 + ceil(log2 d) and m = floor(2^shift / d) + 1
+// (m < 2^33, held as m_lo + 2^32 m_hi).  The rounding error of m stays below 1/d for such n.
+struct FastDiv
+{
+    uint32_t m_lo, m_hi;
+    int shift, d;
+};
+
 // A neighbour pair (i, j, translation) with its exchange constant J, or its DM
 // magnitude D and unit DM vector n (Vectormath_Defines.hpp:98-107).
 struct PairDev
 {
     int i, j, da, db, dc, pad;
+    int delta;  // index offset j - i of the neighbour for a site away from every boundary
+    int pad2;
     double J;
     double D;
     double n[3];
@@ -55,6 +66,9 @@ struct HamDev
     int has_ddi;
     int has_site_K;
     int dmi_same;            // DMI pair k has exchange pair k's (i, j, translation) for every k < n_dmi
+    int halo[3];             // largest |translation| of an exchange pair along a, b, c
+    int pad1;
+    FastDiv div_nb, div_Na, div_Nb; // index -> (basis atom, a, b, c) without hardware division
     double B;                // external_field_magnitude * mu_B (Hamiltonian_Heisenberg.cpp:35)
     double Bn[3];            // external_field_normal
     double mu[MAX_BASIS];    // mu_s per basis atom

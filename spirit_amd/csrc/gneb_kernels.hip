@@ -541,11 +541,11 @@ __device__ void hermite_point( const double * ch, int noi, int n_interp, int k, 
 }
 
 template<int PHASE>
-__device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *red )[256] );
+__device__ __forceinline__ void chain_phase( const GnebArgs & A, double * ch, const double ( *red )[256] );
 
-// Per-image sums of partial slots (one wave per image, fixed order), then the chain logic on
-// one thread.  The chain scalars are staged in LDS for that serial part (chains of up to
-// GNEB_CH_LDS images): its dependent reads then cost LDS latency instead of L2 latency.
+// Per-image sums of partial slots (one wave per image, fixed order), then the chain logic.
+// The chain scalars are staged in LDS (chains of up to GNEB_CH_LDS images): the sequential
+// parts of the logic then pay LDS latency instead of L2 latency per dependent read.
 template<int PHASE>
 __global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
 {
@@ -599,138 +599,155 @@ __global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
                 red[k][img] = s[k];
     }
     __syncthreads();
-    if( threadIdx.x == 0 )
-        chain_logic<PHASE>( A, staged ? ch_lds : A.ch, red );
+    chain_phase<PHASE>( A, staged ? ch_lds : A.ch, red );
     __syncthreads();
     if( staged )
         for( int e = threadIdx.x; e < n_scalars; e += blockDim.x )
             A.ch[e] = ch_lds[e];
 }
 
+// The chain logic of one phase, run by the whole block: per-image work on one thread per
+// image, the sequential parts (the reaction-coordinate prefix sum, chain-wide sums and
+// maxima, the spline, the endpoints) on thread 0 in the reference's order.  Every thread
+// calls it (it synchronises the block).
 template<int PHASE>
-__device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *red )[256] )
+__device__ __forceinline__ void chain_phase( const GnebArgs & A, double * ch, const double ( *red )[256] )
 {
     const int noi           = A.noi;
+    const int tid           = threadIdx.x, nth = blockDim.x;
     const GnebParamsDev & P = A.P;
     auto C = [&]( int img, int slot ) -> double & { return ch[img * GNEB_NCH + slot]; };
 
     if constexpr( PHASE == GPH_DIST )
     {
         // Method_GNEB.cpp:96-126 and Manifoldmath::Tangents weights, :144-199
-        for( int img = 0; img < noi; ++img )
+        __shared__ double dist[256];
+        for( int img = tid; img < noi; img += nth )
         {
             C( img, GC_E )     = red[0][img];
             C( img, GC_DIST2 ) = red[1][img];
-        }
-        C( 0, GC_RX ) = 0;
-        for( int img = 1; img < noi; ++img )
-        {
-            C( img, GC_RX ) = C( img - 1, GC_RX ) + sqrt( C( img, GC_DIST2 ) );
-            if( C( img, GC_RX ) - C( img - 1, GC_RX ) < 1e-10 )
+            dist[img]          = sqrt( red[1][img] );
+            if( img >= 1 && img < noi - 1 )
             {
-                *A.stop = 2;
-                return;
+                const double E_mid = red[0][img], E_plus = red[0][img + 1], E_minus = red[0][img - 1];
+                double mode = 0, cp = 1, cm = 1;
+                if( ( E_plus < E_mid && E_mid > E_minus ) || ( E_plus > E_mid && E_mid < E_minus ) )
+                {
+                    const double E_max = fmax( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
+                    const double E_min = fmin( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
+                    cp                 = E_plus > E_minus ? E_max : E_min;
+                    cm                 = E_plus > E_minus ? E_min : E_max;
+                }
+                else if( E_plus > E_mid && E_mid > E_minus )
+                    mode = 1;
+                else if( E_plus < E_mid && E_mid < E_minus )
+                    mode = 2;
+                C( img, GC_MODE ) = mode;
+                C( img, GC_CP )   = cp;
+                C( img, GC_CM )   = cm;
             }
         }
-        for( int img = 1; img < noi - 1; ++img )
+        __syncthreads();
+        if( tid == 0 )
         {
-            const double E_mid = C( img, GC_E ), E_plus = C( img + 1, GC_E ), E_minus = C( img - 1, GC_E );
-            double mode = 0, cp = 1, cm = 1;
-            if( ( E_plus < E_mid && E_mid > E_minus ) || ( E_plus > E_mid && E_mid < E_minus ) )
+            C( 0, GC_RX ) = 0;
+            for( int img = 1; img < noi; ++img )
             {
-                const double E_max = fmax( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
-                const double E_min = fmin( fabs( E_plus - E_mid ), fabs( E_minus - E_mid ) );
-                cp                 = E_plus > E_minus ? E_max : E_min;
-                cm                 = E_plus > E_minus ? E_min : E_max;
+                C( img, GC_RX ) = C( img - 1, GC_RX ) + dist[img];
+                if( C( img, GC_RX ) - C( img - 1, GC_RX ) < 1e-10 )
+                {
+                    *A.stop = 2;
+                    break;
+                }
             }
-            else if( E_plus > E_mid && E_mid > E_minus )
-                mode = 1;
-            else if( E_plus < E_mid && E_mid < E_minus )
-                mode = 2;
-            C( img, GC_MODE ) = mode;
-            C( img, GC_CP )   = cp;
-            C( img, GC_CM )   = cm;
         }
     }
     else if constexpr( PHASE == GPH_TANGENT )
     {
-        for( int img = 0; img < noi; ++img )
+        for( int img = tid; img < noi; img += nth )
         {
             const double sc    = 1.0 / sqrt( red[0][img] );
             C( img, GC_SC )    = sc;
             C( img, GC_PROJ )  = sc * red[1][img];
             C( img, GC_DEDRX ) = sc * red[2][img];
         }
+        __syncthreads();
         // spring coefficients, Method_GNEB.cpp:131-167 and 232-236
         if( P.spring_force_ratio > 0 )
         {
-            const double ratio_E = fmin( 1.0, P.spring_force_ratio ), ratio_Rx = 1 - ratio_E;
-            const int n_interp = 20;
-            const int npts     = noi + ( noi - 1 ) * n_interp;
-            double xmin = 0, xmax = 0, pmin = 0, pmax = 0;
-            for( int k = 0; k < npts; ++k )
+            if( tid == 0 )
             {
-                double ox, op;
-                hermite_point( ch, noi, n_interp, k, ox, op );
-                if( k == 0 || ox > xmax )
-                    xmax = ox;
-                if( k == 0 || ox < xmin )
-                    xmin = ox;
-                if( k == 0 || op > pmax )
-                    pmax = op;
-                if( k == 0 || op < pmin )
-                    pmin = op;
-            }
-            const double range_Rx = xmax - xmin, range_E = pmax - pmin;
-            C( 0, GC_LEN ) = 0;
-            for( int img = 1; img < noi; ++img )
-            {
-                double len = 0;
-                for( int i = 1; i <= n_interp; ++i )
+                const double ratio_E = fmin( 1.0, P.spring_force_ratio ), ratio_Rx = 1 - ratio_E;
+                const int n_interp = 20;
+                const int npts     = noi + ( noi - 1 ) * n_interp;
+                double xmin = 0, xmax = 0, pmin = 0, pmax = 0;
+                for( int k = 0; k < npts; ++k )
                 {
-                    const int idx = ( img - 1 ) * ( n_interp + 1 ) + i;
-                    double x0, p0, x1, p1;
-                    hermite_point( ch, noi, n_interp, idx - 1, x0, p0 );
-                    hermite_point( ch, noi, n_interp, idx, x1, p1 );
-                    const double dRx = ratio_Rx * ( x1 - x0 ) / range_Rx;
-                    const double dE  = ratio_E * ( p1 - p0 ) / range_E;
-                    len += sqrt( dRx * dRx + dE * dE );
+                    double ox, op;
+                    hermite_point( ch, noi, n_interp, k, ox, op );
+                    if( k == 0 || ox > xmax )
+                        xmax = ox;
+                    if( k == 0 || ox < xmin )
+                        xmin = ox;
+                    if( k == 0 || op > pmax )
+                        pmax = op;
+                    if( k == 0 || op < pmin )
+                        pmin = op;
                 }
-                C( img, GC_LEN ) = len * range_Rx;
+                const double range_Rx = xmax - xmin, range_E = pmax - pmin;
+                C( 0, GC_LEN ) = 0;
+                for( int img = 1; img < noi; ++img )
+                {
+                    double len = 0;
+                    for( int i = 1; i <= n_interp; ++i )
+                    {
+                        const int idx = ( img - 1 ) * ( n_interp + 1 ) + i;
+                        double x0, p0, x1, p1;
+                        hermite_point( ch, noi, n_interp, idx - 1, x0, p0 );
+                        hermite_point( ch, noi, n_interp, idx, x1, p1 );
+                        const double dRx = ratio_Rx * ( x1 - x0 ) / range_Rx;
+                        const double dE  = ratio_E * ( p1 - p0 ) / range_E;
+                        len += sqrt( dRx * dRx + dE * dE );
+                    }
+                    C( img, GC_LEN ) = len * range_Rx;
+                }
+                for( int img = 1; img < noi - 1; ++img )
+                    C( img, GC_D ) = P.spring_constant * ( C( img + 1, GC_LEN ) - C( img, GC_LEN ) );
             }
-            for( int img = 1; img < noi - 1; ++img )
-                C( img, GC_D ) = P.spring_constant * ( C( img + 1, GC_LEN ) - C( img, GC_LEN ) );
         }
         else
-            for( int img = 1; img < noi - 1; ++img )
+            for( int img = tid + 1; img < noi - 1; img += nth )
                 C( img, GC_D )
                     = P.spring_constant * ( C( img + 1, GC_RX ) - 2 * C( img, GC_RX ) + C( img - 1, GC_RX ) );
         // moving endpoints, Method_GNEB.cpp:265-355
-        if( P.moving_endpoints )
+        if( P.moving_endpoints && tid == 0 )
         {
             double rot = 1.0;
             if( P.escape_first && C( 0, GC_PROJ ) > C( noi - 1, GC_PROJ ) )
                 rot = 0.0;
             for( int e = 0; e < 2; ++e )
             {
-                const int img      = e == 0 ? 0 : noi - 1;
-                const double d0    = e == 0 ? P.delta_Rx_left : P.delta_Rx_right;
-                const double dRx   = e == 0 ? C( 1, GC_RX ) - C( 0, GC_RX ) : C( noi - 1, GC_RX ) - C( noi - 2, GC_RX );
-                const double k     = ( e == 0 ? 1.0 : -1.0 ) * P.spring_constant;
-                C( img, GC_ROT )   = rot;
-                C( img, GC_TC )    = k * ( dRx - d0 );
-                C( img, GC_PT )    = 0;
+                const int img    = e == 0 ? 0 : noi - 1;
+                const double d0  = e == 0 ? P.delta_Rx_left : P.delta_Rx_right;
+                const double dRx = e == 0 ? C( 1, GC_RX ) - C( 0, GC_RX ) : C( noi - 1, GC_RX ) - C( noi - 2, GC_RX );
+                const double k   = ( e == 0 ? 1.0 : -1.0 ) * P.spring_constant;
+                C( img, GC_ROT ) = rot;
+                C( img, GC_TC )  = k * ( dRx - d0 );
+                C( img, GC_PT )  = 0;
             }
         }
     }
     else if constexpr( PHASE == GPH_TRANSLATE )
     {
-        C( 0, GC_PT )       = red[0][0];
-        C( noi - 1, GC_PT ) = red[1][0];
+        if( tid == 0 )
+        {
+            C( 0, GC_PT )       = red[0][0];
+            C( noi - 1, GC_PT ) = red[1][0];
+        }
     }
     else if constexpr( PHASE == GPH_SHRINK1 )
     {
-        for( int img = 1; img < noi - 1; ++img )
+        for( int img = tid + 1; img < noi - 1; img += nth )
         {
             C( img, GC_NTP )   = 1.0 / sqrt( red[0][img] );
             C( img, GC_NTM )   = 1.0 / sqrt( red[1][img] );
@@ -739,17 +756,17 @@ __device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *re
     }
     else if constexpr( PHASE == GPH_SHRINK2 )
     {
-        for( int img = 1; img < noi - 1; ++img )
+        for( int img = tid + 1; img < noi - 1; img += nth )
             C( img, GC_X1 ) = red[0][img];
     }
     else if constexpr( PHASE == GPH_SHRINK3 )
     {
-        for( int img = 1; img < noi - 1; ++img )
+        for( int img = tid + 1; img < noi - 1; img += nth )
             C( img, GC_X2 ) = red[0][img];
     }
     else if constexpr( PHASE == GPH_SHRINK4 )
     {
-        for( int img = 1; img < noi - 1; ++img )
+        for( int img = tid + 1; img < noi - 1; img += nth )
         {
             C( img, GC_SC2 )   = 1.0 / sqrt( red[0][img] );
             C( img, GC_SCALE ) = fmax( C( img, GC_GNORM ), A.nos * P.path_shortening_constant );
@@ -757,31 +774,36 @@ __device__ void chain_logic( const GnebArgs & A, double * ch, const double ( *re
     }
     else if constexpr( PHASE == GPH_VP )
     {
-        // Solver_VP.hpp:73-93
-        double proj = 0, fn2 = 0;
-        for( int img = 0; img < noi; ++img )
+        // Solver_VP.hpp:73-93 (chain-wide sums in image order)
+        if( tid == 0 )
         {
-            proj += red[0][img];
-            fn2 += red[1][img];
+            double proj = 0, fn2 = 0;
+            for( int img = 0; img < noi; ++img )
+            {
+                proj += red[0][img];
+                fn2 += red[1][img];
+            }
+            C( 0, GC_RATIO ) = proj / fn2;
+            C( 0, GC_VZERO ) = proj <= 0 ? 1.0 : 0.0;
         }
-        C( 0, GC_RATIO ) = proj / fn2;
-        C( 0, GC_VZERO ) = proj <= 0 ? 1.0 : 0.0;
     }
     else if constexpr( PHASE == GPH_HOOK )
     {
         // Hook_Post_Iteration, Method_GNEB.cpp:411-432; convergence, Method_GNEB.cpp:393-397
-        double mt = 0;
-        for( int img = 0; img < noi; ++img )
+        for( int img = tid; img < noi; img += nth )
+            C( img, GC_TORQUE ) = sqrt( red[0][img] );
+        __syncthreads();
+        if( tid == 0 )
         {
-            const double t     = sqrt( red[0][img] );
-            C( img, GC_TORQUE ) = t;
-            if( t > mt )
-                mt = t;
+            double mt = 0;
+            for( int img = 0; img < noi; ++img )
+                if( C( img, GC_TORQUE ) > mt )
+                    mt = C( img, GC_TORQUE );
+            C( 0, GC_MAXT ) = mt;
+            *A.iter += 1;
+            if( P.check_convergence && mt < P.force_convergence )
+                *A.stop = 1;
         }
-        C( 0, GC_MAXT ) = mt;
-        *A.iter += 1;
-        if( P.check_convergence && mt < P.force_convergence )
-            *A.stop = 1;
     }
 }
 

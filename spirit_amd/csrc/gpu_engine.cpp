@@ -328,6 +328,23 @@ void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStrea
 // ---------------------------------------------------------------------------
 // Hamiltonian upload
 // ---------------------------------------------------------------------------
+static FastDiv make_fast_div( int d )
+{
+    if( d < 1 )
+        throw std::runtime_error( "lattice dimension must be positive" );
+    int L = 0;
+    while( ( 1LL << L ) < d )
+        ++L;
+    const int S             = 31 + L;
+    const unsigned long long m = ( ( 1ULL << S ) / (unsigned long long)d ) + 1;
+    FastDiv f;
+    f.m_lo  = (uint32_t)( m & 0xffffffffULL );
+    f.m_hi  = (uint32_t)( m >> 32 );
+    f.shift = S;
+    f.d     = d;
+    return f;
+}
+
 void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
 {
     if( g.nb > MAX_BASIS )
@@ -340,6 +357,9 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
     H.Nc  = g.n_cells[2];
     H.nb  = g.nb;
     H.nos = g.nos;
+    H.div_nb = make_fast_div( g.nb );
+    H.div_Na = make_fast_div( g.n_cells[0] );
+    H.div_Nb = make_fast_div( g.n_cells[1] );
     for( int c = 0; c < 3; ++c )
         H.bc[c] = h.bc[c];
     H.has_zeeman = h.has_zeeman() ? 1 : 0;
@@ -384,10 +404,21 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
         pv.push_back( q );
     }
     for( auto & q : pv )
+    {
         if( q.i < 0 || q.i >= g.nb || q.j < 0 || q.j >= g.nb )
             throw std::runtime_error( "pair references a basis atom that does not exist" );
+        const long long d = ( q.j - q.i ) + (long long)g.nb * ( q.da + (long long)g.n_cells[0]
+                                                                        * ( q.db + (long long)g.n_cells[1] * q.dc ) );
+        q.delta = (int)d; // only used where the site is at least |translation| away from every boundary
+    }
     H.n_exch = (int)h.exch_pairs.size();
     H.n_dmi  = (int)h.dmi_pairs.size();
+    for( int k = 0; k < H.n_exch; ++k )
+    {
+        H.halo[0] = std::max( H.halo[0], std::abs( pv[k].da ) );
+        H.halo[1] = std::max( H.halo[1], std::abs( pv[k].db ) );
+        H.halo[2] = std::max( H.halo[2], std::abs( pv[k].dc ) );
+    }
     H.dmi_same = H.n_dmi <= H.n_exch ? 1 : 0;
     for( int k = 0; k < H.n_dmi && H.dmi_same; ++k )
     {

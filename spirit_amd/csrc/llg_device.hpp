@@ -116,6 +116,19 @@ __device__ __forceinline__ void normal3( uint64_t seed, uint64_t step, uint64_t 
     z[2]      = r2 * cos( 2.0 * Pi * u4 );
 }
 
+// floor(n / f.d) for 0 <= n < 2^31 (FastDiv)
+__device__ __forceinline__ int fast_div( int n, const FastDiv & f )
+{
+    const uint64_t p = (uint64_t)(uint32_t)n * f.m_lo + ( (uint64_t)( (uint32_t)n * f.m_hi ) << 32 );
+    return (int)( p >> f.shift );
+}
+
+// basis atom of spin i
+__device__ __forceinline__ int basis_of( const HamDev & H, int i )
+{
+    return i - fast_div( i, H.div_nb ) * H.nb;
+}
+
 // SoA field access: component q of spin i at f[q * nos + i]
 __device__ __forceinline__ void load3( const double * f, int nos, int i, double * v )
 {
@@ -175,12 +188,15 @@ __device__ __forceinline__ void gather_exchange_neighbours( const HamDev & H, co
                                                             int i, int ib, int a, int b, int c,
                                                             double ( &se )[LG_MAX_NB][3] )
 {
+    // away from every boundary a neighbour is the site's index plus a fixed offset
+    const bool interior = a >= H.halo[0] && a < H.Na - H.halo[0] && b >= H.halo[1] && b < H.Nb - H.halo[1]
+                          && c >= H.halo[2] && c < H.Nc - H.halo[2];
 #pragma unroll
     for( int k = 0; k < LG_MAX_NB; ++k )
         if( k < H.n_exch )
         {
             const PairDev & p = H.exch[k];
-            const int j       = p.i == ib ? pair_neighbour( H, a, b, c, p ) : -1;
+            const int j = p.i != ib ? -1 : interior ? i + p.delta : pair_neighbour( H, a, b, c, p );
             load3( x, nos, j >= 0 ? j : i, se[k] );
             if( j < 0 )
                 se[k][0] = se[k][1] = se[k][2] = 0.0;
@@ -192,12 +208,12 @@ __device__ __forceinline__ void gather_exchange_neighbours( const HamDev & H, co
 __device__ __forceinline__ void local_gradient( const HamDev & H, const double * __restrict__ x, int nos, int i,
                                                 const double * __restrict__ hddi, double * g, double & e )
 {
-    const int ib   = i % H.nb;
-    const int cell = i / H.nb;
-    const int a    = cell % H.Na;
-    const int rest = cell / H.Na;
-    const int b    = rest % H.Nb;
-    const int c    = rest / H.Nb;
+    const int cell = fast_div( i, H.div_nb );
+    const int ib   = i - cell * H.nb;
+    const int rest = fast_div( cell, H.div_Na );
+    const int a    = cell - rest * H.Na;
+    const int c    = fast_div( rest, H.div_Nb );
+    const int b    = rest - c * H.Nb;
     const double s[3] = { x[i], x[nos + i], x[2 * nos + i] };
     // Every load this spin needs is issued before the first use: the per-site K_i, the DDI
     // field and the neighbour spins (up to LG_MAX_NB of them gathered into registers), so
@@ -389,6 +405,28 @@ __device__ __forceinline__ int xcd_contiguous_block()
     const int full = ( nblk / 8 ) * 8;
     const int b    = blockIdx.x;
     return b < full ? ( b % 8 ) * ( full / 8 ) + b / 8 : b;
+}
+
+// Order in which the blocks of one image visit the lattice in the stage kernels: rows of
+// cells along a (all basis atoms, whole blocks per row) in tiles of STAGE_ROW_TILE
+// consecutive b, each tile swept through every c plane, so the rows b +- 1 and c +- 1 a
+// block reads were fetched into the same L2 by blocks that ran just before it instead of
+// a whole 2D plane earlier.  A bijection on the image's blocks; the identity where the
+// rows do not hold whole blocks, Nb is not a multiple of the tile or the lattice is 2D.
+constexpr int STAGE_ROW_TILE = 8;
+__device__ __forceinline__ int stage_block_order( const HamDev & H, int q )
+{
+    const int R = H.nb * H.Na;
+    if( R % LLG_THREADS != 0 || H.Nb % STAGE_ROW_TILE != 0 || H.Nc == 1 )
+        return q;
+    const int per_row = R / LLG_THREADS;
+    const int j       = q % per_row;
+    int t             = q / per_row;
+    const int bi      = t % STAGE_ROW_TILE;
+    t /= STAGE_ROW_TILE;
+    const int c  = t % H.Nc;
+    const int bt = t / H.Nc;
+    return ( bt * STAGE_ROW_TILE + bi + H.Nb * c ) * per_row + j;
 }
 
 // block-wide reduction of (sum e, max t) into partials[blk]

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
     const HamDev & H        = I->ham;
     const int nos           = H.nos;
     const long long base    = 3 * I->off;
-    const int i             = ( blk - I->block_begin ) * LLG_THREADS + threadIdx.x;
+    const int i             = stage_block_order( H, blk - I->block_begin ) * LLG_THREADS + threadIdx.x;
     const bool act          = i < nos;
     constexpr bool LAST     = ( SOLVER == SOLVER_RK4 ) ? ( STAGE == 3 ) : ( STAGE == 1 );
 
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
         double g[3], e;
         local_gradient( H, x, nos, i, ( A.h && H.has_ddi ) ? A.h + base : nullptr, g, e );
         e_i            = e;
-        const double mu = H.mu[i % H.nb];
+        const double mu = H.mu[basis_of( H, i )];
         double si[3];
         load3( x, nos, i, si );
         const double F[3] = { -1 * g[0], -1 * g[1], -1 * g[2] };
@@ -276,7 +276,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solv
         LLGDev P           = I->llg;
         P.thermal          = 0;
         double fv[3];
-        virtual_force( P, solver, H.mu[i % H.nb], si, F, xi, fv );
+        virtual_force( P, solver, H.mu[basis_of( H, i )], si, F, xi, fv );
         double d = dot3( fv, si ), pr[3];
 #pragma unroll
         for( int q = 0; q < 3; ++q )
