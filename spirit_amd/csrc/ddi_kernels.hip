@@ -514,21 +514,47 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_fwd( DdiDev d, const double * __
         }
         v[k] = make_double2( x1, x2 );
     }
-    double2 * L = lds + l * LS;
-    p2_fft<N, false>( v, L, d.La.tw2, t );
+    double * Ld = reinterpret_cast<double *>( lds ) + l * LS;
+    p2_fft_reg_split<N, false>( v, Ld, d.La.tw2, t );
+    // Two real rows from one complex transform Z: X1 = (Z + conj Z[-k]) / 2 and
+    // X2 = (Z - conj Z[-k]) / (2i), for k < Ha = N/2 + 1; real parts of Z cross LDS first,
+    // then imaginary parts.
+    constexpr int NK = ( N / 2 + 1 + T - 1 ) / T;
+    double r0x[NK], r1y[NK];
+#pragma unroll
+    for( int k = 0; k < 8; ++k )
+        Ld[pad8( t + T * k )] = v[k].x;
+    __syncthreads();
+#pragma unroll
+    for( int it = 0; it < NK; ++it )
+    {
+        const int k = t + T * it;
+        if( k <= N / 2 )
+        {
+            const double zx = Ld[pad8( k )], zmx = Ld[pad8( ( N - k ) & ( N - 1 ) )];
+            r0x[it]         = 0.5 * ( zx + zmx );
+            r1y[it]         = -0.5 * ( zx - zmx );
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for( int k = 0; k < 8; ++k )
+        Ld[pad8( t + T * k )] = v[k].y;
+    __syncthreads();
     if( !act )
         return;
     double2 * row0 = d.X + ( ( (long long)arr * d.Nc + c ) * d.Pb + b0 ) * d.Hs;
     double2 * row1 = row0 + d.Hs;
-    for( int k = t; k < d.Ha; k += T )
+#pragma unroll
+    for( int it = 0; it < NK; ++it )
     {
-        const double2 Z  = L[pad8( k )];
-        const double2 Zm = cconj( L[pad8( ( N - k ) & ( N - 1 ) )] );
-        row0[k]          = make_double2( 0.5 * ( Z.x + Zm.x ), 0.5 * ( Z.y + Zm.y ) );
-        if( has1 )
+        const int k = t + T * it;
+        if( k <= N / 2 )
         {
-            const double2 D = csub( Z, Zm ); // X2 = (Z - conj Z[-k]) / (2i)
-            row1[k]         = make_double2( 0.5 * D.y, -0.5 * D.x );
+            const double zy = Ld[pad8( k )], zmy = -Ld[pad8( ( N - k ) & ( N - 1 ) )];
+            row0[k]         = make_double2( r0x[it], 0.5 * ( zy + zmy ) );
+            if( has1 )
+                row1[k] = make_double2( 0.5 * ( zy - zmy ), r1y[it] );
         }
     }
 }
@@ -571,21 +597,26 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_inv( DdiDev d, double * __restri
         }
         v[k] = make_double2( X1.x - X2.y, X1.y + X2.x );
     }
-    double2 * L = lds + l * LS;
-    p2_fft<N, true>( v, L, d.La.tw2, t );
+    double * Ld = reinterpret_cast<double *>( lds ) + l * LS;
+    p2_fft_reg_split<N, true>( v, Ld, d.La.tw2, t );
     if( !act )
         return;
+    // the transform ends in registers, v[k] = z[a = t + T k]: no final exchange
     const int ib          = arr / 3;
     const int q           = arr - 3 * ib;
     const double mu       = d.mu[ib];
     const long long base0 = (long long)q * nos + ib + (long long)d.nb * d.Na * ( b0 + (long long)d.Nb * c );
     const long long base1 = base0 + (long long)d.nb * d.Na;
-    for( int a = t; a < d.Na; a += T )
+#pragma unroll
+    for( int k = 0; k < 8; ++k )
     {
-        const double2 z                = L[pad8( a )];
-        h[base0 + (long long)d.nb * a] = mu * z.x / d.M;
-        if( has1 )
-            h[base1 + (long long)d.nb * a] = mu * z.y / d.M;
+        const int a = t + T * k;
+        if( a < d.Na )
+        {
+            h[base0 + (long long)d.nb * a] = mu * v[k].x / d.M;
+            if( has1 )
+                h[base1 + (long long)d.nb * a] = mu * v[k].y / d.M;
+        }
     }
 }
 
@@ -889,7 +920,7 @@ static int p2_ab_lines( int n )
 }
 static size_t p2_ab_lds( int n )
 {
-    return sizeof( double2 ) * p2_ab_lines( n ) * p2_ls( n );
+    return sizeof( double ) * p2_ab_lines( n ) * p2_ls( n ); // half-width exchanges (p2_fft_reg_split)
 }
 static bool p2_ab_ok( const DdiDev & d, const FftLen & L )
 {

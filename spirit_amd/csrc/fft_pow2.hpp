@@ -131,6 +131,95 @@ __device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restric
     }
 }
 
+// As p2_stage with KEEP = true, but the values cross LDS real parts first, then imaginary
+// parts: a line needs pad8(N) doubles of LDS instead of complex values (twice the blocks
+// per CU where LDS limits the occupancy) for two more barriers per stage.  Same arithmetic,
+// same result.
+template<int N, bool INV, int S>
+__device__ __forceinline__ void p2_stage_split( double2 ( &v )[8], double * __restrict__ Ld,
+                                                const double2 * __restrict__ tw, int t )
+{
+    using St           = P2Stage<N, S>;
+    constexpr int R    = St::R;
+    constexpr int Ns   = St::Ns;
+    constexpr int M    = 8 / R;
+    constexpr int T    = P2<N>::T;
+    constexpr bool LST = ( S + 1 == P2<N>::NST );
+    double2 out[8];
+#pragma unroll
+    for( int m = 0; m < M; ++m )
+    {
+        const int j  = t + T * m;
+        const int jm = j & ( Ns - 1 );
+        double2 u[R];
+#pragma unroll
+        for( int r = 0; r < R; ++r )
+            u[r] = v[m + M * r];
+        if constexpr( S > 0 )
+        {
+#pragma unroll
+            for( int r = 1; r < R; ++r )
+            {
+                double2 w = tw[St::TWOFF + jm * ( R - 1 ) + ( r - 1 )];
+                if( INV )
+                    w.y = -w.y;
+                u[r] = cmul( u[r], w );
+            }
+        }
+        dft_small<R, INV>( u, nullptr, N );
+#pragma unroll
+        for( int r = 0; r < R; ++r )
+            out[m + M * r] = u[r];
+    }
+    if constexpr( LST )
+    {
+        // Ns = N/R: out[m + M*r] is X[t + T*(m + M*r)]
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+            v[k] = out[k];
+    }
+    else
+    {
+#pragma unroll
+        for( int m = 0; m < M; ++m )
+        {
+            const int j    = t + T * m;
+            const int idxD = ( j / Ns ) * Ns * R + ( j & ( Ns - 1 ) );
+#pragma unroll
+            for( int r = 0; r < R; ++r )
+                Ld[pad8( idxD + r * Ns )] = out[m + M * r].x;
+        }
+        __syncthreads();
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+            v[k].x = Ld[pad8( t + T * k )];
+        __syncthreads();
+#pragma unroll
+        for( int m = 0; m < M; ++m )
+        {
+            const int j    = t + T * m;
+            const int idxD = ( j / Ns ) * Ns * R + ( j & ( Ns - 1 ) );
+#pragma unroll
+            for( int r = 0; r < R; ++r )
+                Ld[pad8( idxD + r * Ns )] = out[m + M * r].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+            v[k].y = Ld[pad8( t + T * k )];
+        __syncthreads();
+        p2_stage_split<N, INV, S + 1>( v, Ld, tw, t );
+    }
+}
+
+// Transform ending in registers (v[k] = X[t + T*k]) through half-width LDS exchanges.
+template<int N, bool INV>
+__device__ __forceinline__ void p2_fft_reg_split( double2 ( &v )[8], double * __restrict__ Ld,
+                                                  const double2 * __restrict__ tw, int t )
+{
+    p2_stage_split<N, INV, 0>( v, Ld, tw, t );
+}
+
 // Transform ending in LDS (natural order, L[pad8(i)]).
 template<int N, bool INV>
 __device__ __forceinline__ void p2_fft( double2 ( &v )[8], double2 * __restrict__ L, const double2 * __restrict__ tw,
