@@ -544,6 +544,7 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
     b_stopstep.resize( sizeof( long long ) * specs.size() );
     b_img_stop.resize( sizeof( int ) * specs.size() );
     b_nstop.resize( sizeof( int ) );
+    b_step.resize( sizeof( uint64_t ) );
     HIPCHECK( hipMemset( b_img_stop.as<void>(), 0, sizeof( int ) * specs.size() ) );
     HIPCHECK( hipMemset( b_nstop.as<void>(), 0, sizeof( int ) ) );
     HIPCHECK( hipMemset( b_stop.as<void>(), 0, sizeof( int ) ) );
@@ -565,6 +566,7 @@ LLGRunner::~LLGRunner()
     if( stream )
     {
         (void)hipStreamSynchronize( stream );
+        release_step_graph();
         for( auto e : ev_pool )
             (void)hipEventDestroy( e );
         (void)hipStreamDestroy( stream );
@@ -642,6 +644,7 @@ void LLGRunner::set_llg( int img, const LLGParameters & p )
 
 void LLGRunner::refresh_hamiltonian()
 {
+    release_step_graph(); // DDI plans and buffers may be rebuilt
     for( size_t k = 0; k < specs.size(); ++k )
     {
         const uint64_t old_ddi = hams[k]->ddi_version;
@@ -752,20 +755,8 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
 #define STEP_ARGS()                                                                                                    \
     base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop )
 
-void LLGRunner::step( bool obs, bool count, bool check, double conv )
+void LLGRunner::launch_step( const StepArgs & A )
 {
-    (void)count;
-    StepArgs A = STEP_ARGS();
-    A.step  = step_counter;
-    A.obs   = obs ? 1 : 0;
-    A.noise = nullptr;
-    if( noise_steps > 0 )
-    {
-        if( noise_pos >= noise_steps )
-            throw std::runtime_error( "injected thermal noise exhausted" );
-        A.noise = b_noise.as<double>() + (size_t)noise_pos * 3 * (size_t)n_total;
-        ++noise_pos;
-    }
     const int n_stages = solver == SOLVER_RK4 ? 4 : 2;
     for( int st = 0; st < n_stages; ++st )
     {
@@ -783,11 +774,86 @@ void LLGRunner::step( bool obs, bool count, bool check, double conv )
         else
             llg_launch_stage( solver, st, A, n_blocks, stream );
     }
+}
+
+void LLGRunner::step( bool obs, bool count, bool check, double conv )
+{
+    (void)count;
+    StepArgs A = STEP_ARGS();
+    A.step     = dev_step ? 0 : step_counter; // with the device counter: offset 0 from it
+    A.step_dev = dev_step ? b_step.as<uint64_t>() : nullptr;
+    A.obs      = obs ? 1 : 0;
+    A.noise    = nullptr;
+    if( noise_steps > 0 )
+    {
+        if( noise_pos >= noise_steps )
+            throw std::runtime_error( "injected thermal noise exhausted" );
+        A.noise = b_noise.as<double>() + (size_t)noise_pos * 3 * (size_t)n_total;
+        ++noise_pos;
+    }
+    launch_step( A );
     if( obs )
         llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_img_stop.as<int>(), b_nstop.as<int>(),
-                             b_stopstep.as<long long>(),
-                             (long long)step_counter, conv, check ? 1 : 0, stream );
+                             b_stopstep.as<long long>(), (long long)A.step, conv, check ? 1 : 0, stream );
+    if( dev_step )
+        llg_launch_step_advance( b_step.as<uint64_t>(), 1, stream );
     ++step_counter;
+}
+
+// Long runs replay a captured graph of a few steps: the launches of the DDI passes, the
+// solver stages and, on observed steps, the per-image reductions and convergence test, each
+// step ending with the device step counter advanced.  The host then submits one graph instead
+// of every launch; kernels and arguments are those of the launches one by one (the step index
+// is a device base plus the step's offset in the graph, the base advanced once per replay), and
+// stopped images and the run-wide stop flag are read on the device as before.
+// SPIRIT_HIP_GRAPHS=0 switches graphs off.
+bool LLGRunner::capture_step_graph( int len, int period, bool check, double conv )
+{
+    if( step_graph && graph_len == len && graph_period == period && graph_check == ( check ? 1 : 0 )
+        && graph_conv == conv )
+        return true;
+    release_step_graph();
+    StepArgs A = STEP_ARGS();
+    A.step_dev = b_step.as<uint64_t>();
+    A.noise    = nullptr;
+    hipGraph_t graph = nullptr;
+    if( hipStreamBeginCapture( stream, hipStreamCaptureModeThreadLocal ) != hipSuccess )
+        return false;
+    for( int k = 0; k < len; ++k )
+    {
+        // step k of the graph: index *b_step + k
+        const bool obs = period > 0 && ( k + 1 ) % period == 0;
+        A.obs          = obs ? 1 : 0;
+        A.step         = (uint64_t)k;
+        launch_step( A );
+        if( obs )
+            llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_img_stop.as<int>(), b_nstop.as<int>(),
+                                 b_stopstep.as<long long>(), (long long)k, conv, check ? 1 : 0, stream );
+    }
+    llg_launch_step_advance( b_step.as<uint64_t>(), len, stream );
+    if( hipStreamEndCapture( stream, &graph ) != hipSuccess || !graph )
+        return false;
+    const hipError_t e = hipGraphInstantiate( &step_graph, graph, nullptr, nullptr, 0 );
+    (void)hipGraphDestroy( graph );
+    if( e != hipSuccess )
+    {
+        step_graph = nullptr;
+        return false;
+    }
+    graph_len    = len;
+    graph_period = period;
+    graph_check  = check ? 1 : 0;
+    graph_conv   = conv;
+    return true;
+}
+
+void LLGRunner::release_step_graph()
+{
+    if( step_graph )
+        (void)hipGraphExecDestroy( step_graph );
+    step_graph   = nullptr;
+    graph_len    = 0;
+    graph_period = -1;
 }
 
 long LLGRunner::run( long n, int observe_every, bool check_convergence, double force_convergence )
@@ -807,12 +873,59 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
     HIPCHECK( hipMemcpyAsync( b_stop.as<void>(), &stop_host, sizeof( int ), hipMemcpyHostToDevice, stream ) );
     const std::vector<int> was_stopped = img_stopped;
     const uint64_t first = step_counter;
-    auto t0              = std::chrono::steady_clock::now();
-    for( long it = 0; it < n; ++it )
+    // Graph replays (not with injected noise, whose pointer changes every step, nor while
+    // kernels are timed one by one).  With an observation period P <= GRAPH_MAX_STEPS a graph
+    // holds whole periods and covers the steps of complete periods; otherwise (P = 0 or long
+    // periods) a graph of unobserved steps covers the stretches between observed steps.
+    const char * g_env  = std::getenv( "SPIRIT_HIP_GRAPHS" );
+    const bool periodic = observe_every > 0 && observe_every <= GRAPH_MAX_STEPS;
+    const int g_len     = periodic ? observe_every * ( ( GRAPH_STEPS + observe_every - 1 ) / observe_every )
+                                   : GRAPH_STEPS;
+    const bool graphs   = !( g_env && std::string( g_env ) == "0" ) && noise_steps == 0 && !profiling && n > g_len;
+    dev_step = false;
+    if( graphs )
+    {
+        HIPCHECK( hipMemcpyAsync( b_step.as<void>(), &step_counter, sizeof( uint64_t ), hipMemcpyHostToDevice,
+                                  stream ) );
+        HIPCHECK( hipStreamSynchronize( stream ) );
+        if( capture_step_graph( g_len, periodic ? observe_every : 0, check_convergence, force_convergence ) )
+            dev_step = true;
+        else
+        {
+            (void)hipGetLastError();
+            release_step_graph();
+        }
+    }
+    const long whole = periodic ? ( n / observe_every ) * observe_every : 0; // steps in complete periods
+    auto t0 = std::chrono::steady_clock::now();
+    for( long it = 0; it < n; )
     {
         bool obs = ( it == n - 1 ) || ( observe_every > 0 && ( it + 1 ) % observe_every == 0 );
+        if( dev_step )
+        {
+            bool replay = false;
+            if( periodic )
+                replay = it % observe_every == 0 && it + g_len <= whole;
+            else if( !obs )
+            {
+                // steps it .. next_obs - 1 are unobserved
+                long next_obs = n - 1;
+                if( observe_every > 0 )
+                    next_obs = std::min( next_obs, ( it / observe_every + 1 ) * (long)observe_every - 1 );
+                replay = next_obs - it >= g_len;
+            }
+            if( replay )
+            {
+                HIPCHECK( hipGraphLaunch( step_graph, stream ) );
+                it += g_len;
+                step_counter += g_len;
+                continue;
+            }
+        }
         step( obs, true, check_convergence && obs, force_convergence );
+        ++it;
     }
+    dev_step = false;
     HIPCHECK( hipGetLastError() );
     std::vector<long long> stop_step( n_img, 0 );
     HIPCHECK( hipMemcpyAsync( img_stopped.data(), b_img_stop.as<void>(), sizeof( int ) * n_img, hipMemcpyDeviceToHost,

@@ -234,6 +234,18 @@ private:
     std::vector<std::pair<std::string, ProfileEntry>> prof;
 
     void step( bool obs, bool count, bool check, double conv );
+    // the DDI fields and solver stages of one step (no observation)
+    void launch_step( const StepArgs & A );
+    // Graph of `len` steps reading the step index from b_step; with period > 0 every
+    // period-th step is observed (finalize with check / conv), with period 0 none is.
+    bool capture_step_graph( int len, int period, bool check, double conv );
+    void release_step_graph();
+    static constexpr int GRAPH_STEPS     = 16; // steps per graph (at least)
+    static constexpr int GRAPH_MAX_STEPS = 64; // longest observation period captured whole
+    hipGraphExec_t step_graph = nullptr;
+    int graph_len = 0, graph_period = -1, graph_check = -1;
+    double graph_conv = 0;
+    bool dev_step = false; // steps take their index from b_step (and advance it)
     void ddi_field( const double * x, bool prof_events = false );
     void upload_images();
 
@@ -246,7 +258,7 @@ private:
     std::vector<std::unique_ptr<HamUpload>> hams;
     std::vector<std::unique_ptr<DdiPlan>> ddi;
     DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
-        b_noise, b_stopstep, b_img_stop, b_nstop;
+        b_noise, b_stopstep, b_img_stop, b_nstop, b_step;
     std::vector<double> obs_host;
     std::vector<int> img_stopped; // host view of the per-image stop flags
     std::vector<long> steps_last; // steps each image advanced in the last run

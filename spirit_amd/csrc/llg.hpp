@@ -30,12 +30,15 @@ struct StepArgs
     double * s_out;        // where the last stage writes the new spins (nullptr: in place into s)
     double * tape_F;       // this stage's force F = -g (nullable; adjoint tape)
     double * tape_fv;      // this stage's virtual force (nullable; adjoint tape)
-    uint64_t step;         // global step index (thermal-noise counter)
+    uint64_t step;         // global step index (thermal-noise counter); with step_dev, the offset from *step_dev
+    const uint64_t * step_dev; // base step index in device memory (graph replays); nullptr: step is absolute
     int obs;               // compute observables (energy, torque, effective field)
     int pad;
 };
 
 void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, hipStream_t st );
+// *step += by on the device (the step counter of graph-replayed step sequences)
+void llg_launch_step_advance( uint64_t * step, int by, hipStream_t st );
 void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, int n_blocks, hipStream_t st );
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
 // Per-image energy / max torque into obs[2*img], [2*img+1]; with check_convergence an image whose

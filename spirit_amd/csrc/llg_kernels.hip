@@ -68,7 +68,8 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
         const double F[3] = { -1 * g[0], -1 * g[1], -1 * g[2] };
         double xi[3]      = { 0, 0, 0 };
         if( I->llg.thermal && !I->llg.direct_minimization )
-            thermal_field( I->llg, mu, A.step, i, A.noise ? A.noise + base : nullptr, xi );
+            thermal_field( I->llg, mu, A.step_dev ? *A.step_dev + A.step : A.step, i, A.noise ? A.noise + base : nullptr,
+                           xi );
         double fv[3];
         virtual_force( I->llg, SOLVER, mu, si, F, xi, fv );
         if( A.tape_F )
@@ -324,7 +325,8 @@ __global__ void __launch_bounds__( 256 )
     k_finalize( const ImageDev * __restrict__ imgs, int n_images, const double * __restrict__ part_E,
                 const double * __restrict__ part_T, double * __restrict__ obs, int * __restrict__ stop,
                 int * __restrict__ img_stop, int * __restrict__ n_stopped, long long * __restrict__ stop_step,
-                long long step_index, double force_convergence, int check_convergence )
+                long long step_index, const uint64_t * __restrict__ step_dev, double force_convergence,
+                int check_convergence )
 {
     __shared__ double sE[256], sT[256];
     const int img = blockIdx.x;
@@ -357,7 +359,7 @@ __global__ void __launch_bounds__( 256 )
         if( check_convergence && torque < I.llg.force_convergence )
         {
             img_stop[img]  = 1;
-            stop_step[img] = step_index;
+            stop_step[img] = step_dev ? (long long)*step_dev + step_index : step_index;
             if( atomicAdd( n_stopped, 1 ) + 1 == n_images )
                 *stop = 1;
         }
@@ -371,6 +373,17 @@ template<int SOLVER, int STAGE>
 static void launch_stage( const StepArgs & A, int n_blocks, hipStream_t st )
 {
     hipLaunchKernelGGL( ( k_llg_stage<SOLVER, STAGE> ), dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A );
+}
+
+__global__ void k_step_advance( uint64_t * step, int by )
+{
+    if( threadIdx.x == 0 )
+        *step += by;
+}
+
+void llg_launch_step_advance( uint64_t * step, int by, hipStream_t st )
+{
+    hipLaunchKernelGGL( k_step_advance, dim3( 1 ), dim3( 64 ), 0, st, step, by );
 }
 
 void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, hipStream_t st )
@@ -419,7 +432,8 @@ void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, int * 
                           hipStream_t st )
 {
     hipLaunchKernelGGL( k_finalize, dim3( n_images ), dim3( 256 ), 0, st, A.imgs, n_images, A.part_E, A.part_T, obs,
-                        A.stop, img_stop, n_stopped, stop_step, step_index, force_convergence, check_convergence );
+                        A.stop, img_stop, n_stopped, stop_step, step_index, A.step_dev, force_convergence,
+                        check_convergence );
 }
 
 } // namespace spirit_hip

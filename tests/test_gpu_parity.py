@@ -202,6 +202,23 @@ def test_llg_ddi_trajectory(engine, oracle, solver):
     assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
 
 
+def test_llg_ddi_trajectory_row_tiles(engine, oracle):
+    """A 256x16x4 film with per-site K_i: rows of 256 cells fill whole blocks, so the stage
+    kernels visit the lattice in 8-row tiles swept through c, and most sites take their
+    neighbours as fixed index offsets; Depondt + DDI, 30 steps."""
+    cfg = configs.scaled(configs.CUBE_256, [256, 16, 4])
+    print_case("llg ddi", "256x16x4 row tiles")
+    K = configs.site_anisotropy(256 * 16 * 4, 3)
+    e = engine(cfg, site_K=K)
+    S = oracle.System(cfg, site_K=K)
+    s0 = oracle.random_spins(S.nos, 6)
+    e.set_spins(s0)
+    n = 30
+    s, _, _ = _run_llg(e, 2, n)
+    s_ref, _, _, _ = S.llg(s0, 2, n, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
+    assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
+
+
 def test_llg_thermal_counter_rng(engine, oracle):
     cfg = configs.scaled(configs.FILM_128, [12, 10, 4])
     print_case("llg thermal philox")
