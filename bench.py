@@ -248,7 +248,7 @@ def run_gneb(n_iter, noi):
             "max_torque": float(info.max_torque)}
 
 
-PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1"}
+PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1", 3: "k_p4_c1"}
 
 
 def pass_c_bytes(dims, info):

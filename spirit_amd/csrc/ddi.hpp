@@ -38,6 +38,7 @@ enum DdiPassC
     DDI_C_GENERIC = 0, // mixed-radix LDS kernel (any length)
     DDI_C_P2_LDS  = 1, // power of two, all 3*nb lines of c_tk columns held in LDS
     DDI_C_P2_REG  = 2, // power of two, one basis atom, real spectrum: components held in registers
+    DDI_C_P4_REG  = 3, // as DDI_C_P2_REG with radix-4 transforms: 4 values per lane, more lanes per CU
 };
 
 struct DipoleArgs

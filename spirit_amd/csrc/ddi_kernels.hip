@@ -800,6 +800,122 @@ __global__ void __launch_bounds__( 256, 3 ) k_p2_c1( DdiDev d, double2 * __restr
     store( v2, 2 );
 }
 
+// Pass C for one basis atom with radix-4 transforms: CW columns (ka) of one kb plane, N/4
+// threads per column, each holding c = t + (N/4) k, k = 0..3, of all three components.  The
+// same dataflow as k_p2_c1 with half the registers per column, so more waves fit on a CU.
+template<int N>
+struct P4Col
+{
+    static constexpr int T  = N / 4;
+    static constexpr int CW = ( 512 / T ) > 16 ? 16 : ( 512 / T < 1 ? 1 : 512 / T );
+};
+
+template<int N>
+__global__ void __launch_bounds__( 512 ) k_p4_c1( DdiDev d, double2 * __restrict__ X, int order_pairs )
+{
+    extern __shared__ double2 lds[];
+    constexpr int T = P4<N>::T, CW = P4Col<N>::CW, LS = P4<N>::LS;
+    const int n_tiles = ( d.Ha + CW - 1 ) / CW;
+    // XCD-contiguous block order; planes kb and Pb-kb (one stored spectrum plane) back to back
+    const int nblk = gridDim.x;
+    const int full = ( nblk / 8 ) * 8;
+    int lb         = blockIdx.x;
+    if( lb < full )
+        lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
+    const int tile = lb % n_tiles;
+    const int o    = lb / n_tiles;
+    int kb         = o;
+    if( order_pairs && o > 0 )
+    {
+        const int p = ( o + 1 ) / 2;
+        kb          = ( o & 1 ) ? p : d.Pb - p;
+    }
+    const int l           = threadIdx.x % CW;
+    const int t           = threadIdx.x / CW;
+    const int ka          = tile * CW + l;
+    const bool col        = ka < d.Ha;
+    const int Nc          = d.Nc;
+    const long long plane = (long long)d.Pb * d.Hs;
+    double2 * base        = X + (long long)kb * d.Hs + ka;
+    // the N-th roots live in LDS behind the transpose lines
+    double2 * W = lds + CW * LS;
+    for( int e = threadIdx.x; e < N; e += blockDim.x )
+        W[e] = d.Lc.W[e];
+    __syncthreads();
+    double2 * L = lds + l * LS;
+    double2 v0[4], v1[4], v2[4];
+    auto load = [&]( double2( &v )[4], int q ) {
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
+        {
+            const int c = t + T * k;
+            v[k]        = ( col && c < Nc ) ? base[(long long)( q * Nc + c ) * plane] : make_double2( 0, 0 );
+        }
+    };
+    auto store = [&]( const double2( &v )[4], int q ) {
+        if( col )
+        {
+#pragma unroll
+            for( int k = 0; k < 4; ++k )
+            {
+                const int c = t + T * k;
+                if( c < Nc )
+                    base[(long long)( q * Nc + c ) * plane] = v[k];
+            }
+        }
+    };
+    load( v0, 0 );
+    load( v1, 1 );
+    load( v2, 2 );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, false>( v0, L, W, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, false>( v1, L, W, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, false>( v2, L, W, t );
+    __builtin_amdgcn_sched_barrier( 0 );
+    if( col )
+    {
+        // kernel spectrum at (kb, ka, kc = t + T k); mirror images in b and c carry the
+        // per-component signs
+        const bool mb       = d.sym_b && kb > d.Pb / 2;
+        const int kbs       = mb ? d.Pb - kb : kb;
+        const long long row = ( (long long)kbs * d.Ha + ka ) * d.KC;
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
+        {
+            const int kc     = t + T * k;
+            const bool mc    = d.sym_c && kc > d.Pc / 2;
+            const int kcs    = mc ? d.Pc - kc : kc;
+            const double * p = d.Dh_r + ( row + kcs ) * 6;
+            double D[6];
+#pragma unroll
+            for( int q = 0; q < 6; ++q )
+            {
+                const int sg = ( mb ? d.sig_b[q] : 1 ) * ( mc ? d.sig_c[q] : 1 );
+                D[q]         = sg < 0 ? -p[q] : p[q];
+            }
+            // real symmetric D: r_q = sum_p D_qp s_p, component-wise on re and im
+            const double2 s0 = v0[k], s1 = v1[k], s2 = v2[k];
+            v0[k] = make_double2( D[0] * s0.x + D[1] * s1.x + D[2] * s2.x, D[0] * s0.y + D[1] * s1.y + D[2] * s2.y );
+            v1[k] = make_double2( D[1] * s0.x + D[3] * s1.x + D[4] * s2.x, D[1] * s0.y + D[3] * s1.y + D[4] * s2.y );
+            v2[k] = make_double2( D[2] * s0.x + D[4] * s1.x + D[5] * s2.x, D[2] * s0.y + D[4] * s1.y + D[5] * s2.y );
+            // bound the kernel-spectrum loads in flight (register pressure)
+            if( k & 1 )
+                __builtin_amdgcn_sched_barrier( 0 );
+        }
+    }
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, true>( v0, L, W, t );
+    store( v0, 0 );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, true>( v1, L, W, t );
+    store( v1, 1 );
+    __builtin_amdgcn_sched_barrier( 0 );
+    p4_fft_reg<N, true>( v2, L, W, t );
+    store( v2, 2 );
+}
+
 // Pass C: tk columns (ka) of one kb plane, all 3*nb arrays (3 nb tk lines of T threads).
 template<int N>
 __global__ void __launch_bounds__( 1024 ) k_p2_c( DdiDev d, double2 * __restrict__ X, int tk, int order_pairs )
@@ -966,6 +1082,22 @@ static bool p2_c1_ok( const DdiDev & d )
            && sizeof( double2 ) * ( p2_c1_cw( d.Lc.n ) * p2_ls( d.Lc.n ) + p2_ntw( d.Lc.n ) ) <= (size_t)d.lds_max;
 }
 
+// radix-4 register pass C: columns per block (N/4 threads each, <= 512 threads) and LDS
+static int p4_c1_cw( int n )
+{
+    const int T = n / 4;
+    const int c = 512 / T;
+    return c > 16 ? 16 : ( c < 1 ? 1 : c );
+}
+static size_t p4_c1_lds( int n )
+{
+    return sizeof( double2 ) * ( p4_c1_cw( n ) * p2_ls( n ) + (size_t)n );
+}
+static bool p4_c1_ok( const DdiDev & d )
+{
+    return d.nb == 1 && d.dh_real && is_p2( d.Lc ) && d.Lc.n <= 2048 && p4_c1_lds( d.Lc.n ) <= (size_t)d.lds_max;
+}
+
 // pass C column tile: 3*nb*tk*(N/8) threads <= 1024, LDS <= min(lds_max, 120 KiB), tk <= 16; 0 if none fits
 static int p2_c_tk( const DdiDev & d )
 {
@@ -990,6 +1122,8 @@ std::vector<std::pair<int, int>> ddi_pass_c_candidates( const DdiDev & d )
     std::vector<std::pair<int, int>> c;
     if( p2_c1_ok( d ) )
         c.push_back( { DDI_C_P2_REG, 0 } );
+    if( p4_c1_ok( d ) )
+        c.push_back( { DDI_C_P4_REG, 0 } );
     const int tk = p2_c_tk( d );
     if( tk > 0 )
     {
@@ -1051,6 +1185,18 @@ static void p2_launch_c1( const DdiDev & d, hipStream_t st )
         const unsigned blocks = (unsigned)( (long long)tiles * d.Pb );
         const size_t lds = sizeof( double2 ) * ( CW * p2_ls( N ) + P2<N>::NTW );
         hipLaunchKernelGGL( ( k_p2_c1<N, true> ), dim3( blocks ), dim3( CW * P2<N>::T ), lds, st, d, d.X,
+                            d.sym_b ? 1 : 0 );
+    }
+}
+template<int N>
+static void p4_launch_c1( const DdiDev & d, hipStream_t st )
+{
+    if constexpr( N <= 2048 )
+    {
+        const int CW          = p4_c1_cw( N );
+        const int tiles       = ( d.Ha + CW - 1 ) / CW;
+        const unsigned blocks = (unsigned)( (long long)tiles * d.Pb );
+        hipLaunchKernelGGL( k_p4_c1<N>, dim3( blocks ), dim3( CW * P4<N>::T ), p4_c1_lds( N ), st, d, d.X,
                             d.sym_b ? 1 : 0 );
     }
 }
@@ -1122,6 +1268,12 @@ void ddi_launch_field( const DdiDev & d, const double * spins, double * h, int n
 #define CALL_C1( n ) p2_launch_c1<n>( d, st )
         P2_DISPATCH( d.Pc, CALL_C1 );
 #undef CALL_C1
+    }
+    else if( d.c_kind == DDI_C_P4_REG && p4_c1_ok( d ) )
+    {
+#define CALL_C4( n ) p4_launch_c1<n>( d, st )
+        P2_DISPATCH( d.Pc, CALL_C4 );
+#undef CALL_C4
     }
     else if( d.c_kind == DDI_C_P2_LDS && d.c_tk > 0 && d.c_tk <= p2_c_tk( d ) )
     {

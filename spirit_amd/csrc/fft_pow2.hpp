@@ -220,6 +220,100 @@ __device__ __forceinline__ void p2_fft_reg_split( double2 ( &v )[8], double * __
     p2_stage_split<N, INV, 0>( v, Ld, tw, t );
 }
 
+// ---------------------------------------------------------------------------
+// Radix-4 variant: a line of N points owned by N/4 threads, thread t holding x[t + (N/4) k],
+// k = 0..3; radix-4 stages and a final radix-2 stage when log2 N is odd.  Half the registers
+// per line of the radix-8 form (for kernels that hold several lines per thread), one more
+// LDS exchange per transform.  Twiddles w_N^e come from the table W of N-th roots.
+// ---------------------------------------------------------------------------
+constexpr int ipow4c( int s )
+{
+    return s == 0 ? 1 : 4 * ipow4c( s - 1 );
+}
+
+template<int N>
+struct P4
+{
+    static_assert( N >= 4 && ( N & ( N - 1 ) ) == 0, "power of two >= 4" );
+    static constexpr int T   = N / 4;
+    static constexpr int LOG = ilog2c( N );
+    static constexpr int N4  = LOG / 2;
+    static constexpr int NST = N4 + ( LOG % 2 );
+    static constexpr int LS  = N + N / 8 + 1; // padded LDS line stride (complex values)
+};
+
+// Stage S.. of the transform, outputs of the last stage in registers: v[k] = X[t + T*k].
+// Every thread of the block must call it; L may be reused once it returns.
+template<int N, bool INV, int S>
+__device__ __forceinline__ void p4_stage( double2 ( &v )[4], double2 * __restrict__ L,
+                                          const double2 * __restrict__ W, int t )
+{
+    constexpr int R    = S < P4<N>::N4 ? 4 : 2;
+    constexpr int Ns   = ipow4c( S );
+    constexpr int M    = 4 / R;
+    constexpr int T    = P4<N>::T;
+    constexpr bool LST = ( S + 1 == P4<N>::NST );
+    double2 out[4];
+#pragma unroll
+    for( int m = 0; m < M; ++m )
+    {
+        const int j  = t + T * m;
+        const int jm = j & ( Ns - 1 );
+        double2 u[R];
+#pragma unroll
+        for( int r = 0; r < R; ++r )
+            u[r] = v[m + M * r];
+        if constexpr( S > 0 )
+        {
+#pragma unroll
+            for( int r = 1; r < R; ++r )
+            {
+                double2 w = W[( r * jm * ( N / ( Ns * R ) ) ) & ( N - 1 )];
+                if( INV )
+                    w.y = -w.y;
+                u[r] = cmul( u[r], w );
+            }
+        }
+        dft_small<R, INV>( u, nullptr, N );
+        if constexpr( LST )
+        {
+            // Ns = N/R: output j + r*N/R = t + T*(m + M*r)
+#pragma unroll
+            for( int r = 0; r < R; ++r )
+                out[m + M * r] = u[r];
+        }
+        else
+        {
+            const int idxD = ( j / Ns ) * Ns * R + jm;
+#pragma unroll
+            for( int r = 0; r < R; ++r )
+                L[pad8( idxD + r * Ns )] = u[r];
+        }
+    }
+    if constexpr( LST )
+    {
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
+            v[k] = out[k];
+    }
+    else
+    {
+        __syncthreads();
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
+            v[k] = L[pad8( t + T * k )];
+        __syncthreads();
+        p4_stage<N, INV, S + 1>( v, L, W, t );
+    }
+}
+
+template<int N, bool INV>
+__device__ __forceinline__ void p4_fft_reg( double2 ( &v )[4], double2 * __restrict__ L,
+                                            const double2 * __restrict__ W, int t )
+{
+    p4_stage<N, INV, 0>( v, L, W, t );
+}
+
 // Transform ending in LDS (natural order, L[pad8(i)]).
 template<int N, bool INV>
 __device__ __forceinline__ void p2_fft( double2 ( &v )[8], double2 * __restrict__ L, const double2 * __restrict__ tw,

@@ -258,7 +258,7 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
 
 // Times every pass C kernel that can run the plan on a zero configuration (the run time
 // does not depend on the values) and keeps the fastest.  The environment variable
-// SPIRIT_HIP_DDI_PASS_C = generic | lds | reg restricts the choice to that kernel where it
+// SPIRIT_HIP_DDI_PASS_C = generic | lds | reg | reg4 restricts the choice to that kernel where it
 // can run the plan (for tests of each kernel and for tuning).
 void DdiPlan::choose_pass_c( hipStream_t st )
 {
@@ -266,7 +266,11 @@ void DdiPlan::choose_pass_c( hipStream_t st )
     if( const char * force = std::getenv( "SPIRIT_HIP_DDI_PASS_C" ) )
     {
         const std::string f( force );
-        const int want = f == "generic" ? DDI_C_GENERIC : f == "lds" ? DDI_C_P2_LDS : f == "reg" ? DDI_C_P2_REG : -1;
+        const int want = f == "generic" ? DDI_C_GENERIC
+                         : f == "lds"   ? DDI_C_P2_LDS
+                         : f == "reg"   ? DDI_C_P2_REG
+                         : f == "reg4"  ? DDI_C_P4_REG
+                                        : -1;
         std::vector<std::pair<int, int>> keep;
         for( auto & c : cand )
             if( c.first == want )

@@ -107,7 +107,7 @@ def test_gradient_and_energy(engine, oracle, name):
     assert abs(E - E0) <= 1e-11 * max(1.0, abs(E0))
 
 
-@pytest.mark.parametrize("kind,code", [("generic", 0), ("lds", 1), ("reg", 2)])
+@pytest.mark.parametrize("kind,code", [("generic", 0), ("lds", 1), ("reg", 2), ("reg4", 3)])
 def test_ddi_pass_c_kernels(engine, oracle, monkeypatch, kind, code):
     """Every pass C kernel (forced through SPIRIT_HIP_DDI_PASS_C) gives the oracle's dipolar
     gradient on a 16x12x128 film (padded 32x24x256: power-of-two c columns, mixed-radix b)."""
