@@ -515,7 +515,9 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_fwd( DdiDev d, const double * __
         v[k] = make_double2( x1, x2 );
     }
     double * Ld = reinterpret_cast<double *>( lds ) + l * LS;
-    p2_fft_reg_split<N, false>( v, Ld, d.La.tw2, t );
+    // each line inside one wavefront: the exchanges synchronise the wavefront only
+    constexpr bool WAVE = T <= 64 && 64 % T == 0;
+    p2_fft_reg_split<N, false, WAVE>( v, Ld, d.La.tw2, t );
     // Two real rows from one complex transform Z: X1 = (Z + conj Z[-k]) / 2 and
     // X2 = (Z - conj Z[-k]) / (2i), for k < Ha = N/2 + 1; real parts of Z cross LDS first,
     // then imaginary parts.
@@ -524,7 +526,7 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_fwd( DdiDev d, const double * __
 #pragma unroll
     for( int k = 0; k < 8; ++k )
         Ld[pad8( t + T * k )] = v[k].x;
-    __syncthreads();
+    p2_sync<WAVE>();
 #pragma unroll
     for( int it = 0; it < NK; ++it )
     {
@@ -536,11 +538,11 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_fwd( DdiDev d, const double * __
             r1y[it]         = -0.5 * ( zx - zmx );
         }
     }
-    __syncthreads();
+    p2_sync<WAVE>();
 #pragma unroll
     for( int k = 0; k < 8; ++k )
         Ld[pad8( t + T * k )] = v[k].y;
-    __syncthreads();
+    p2_sync<WAVE>();
     if( !act )
         return;
     double2 * row0 = d.X + ( ( (long long)arr * d.Nc + c ) * d.Pb + b0 ) * d.Hs;
@@ -598,7 +600,8 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_inv( DdiDev d, double * __restri
         v[k] = make_double2( X1.x - X2.y, X1.y + X2.x );
     }
     double * Ld = reinterpret_cast<double *>( lds ) + l * LS;
-    p2_fft_reg_split<N, true>( v, Ld, d.La.tw2, t );
+    constexpr bool WAVE = T <= 64 && 64 % T == 0;
+    p2_fft_reg_split<N, true, WAVE>( v, Ld, d.La.tw2, t );
     if( !act )
         return;
     // the transform ends in registers, v[k] = z[a = t + T k]: no final exchange

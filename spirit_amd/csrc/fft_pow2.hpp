@@ -135,7 +135,23 @@ __device__ __forceinline__ void p2_stage( double2 ( &v )[8], double2 * __restric
 // parts: a line needs pad8(N) doubles of LDS instead of complex values (twice the blocks
 // per CU where LDS limits the occupancy) for two more barriers per stage.  Same arithmetic,
 // same result.
-template<int N, bool INV, int S>
+// Synchronisation between the LDS writes and reads of an exchange: the whole block, or only
+// the wavefront when every line lives inside one wavefront (N/8 <= 64 threads per line and
+// lines aligned to wavefronts), which spares the other wavefronts of the block the wait.
+template<bool WAVE>
+__device__ __forceinline__ void p2_sync()
+{
+    if constexpr( WAVE )
+    {
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+    }
+    else
+        __syncthreads();
+}
+
+template<int N, bool INV, int S, bool WAVE = false>
 __device__ __forceinline__ void p2_stage_split( double2 ( &v )[8], double * __restrict__ Ld,
                                                 const double2 * __restrict__ tw, int t )
 {
@@ -189,11 +205,11 @@ __device__ __forceinline__ void p2_stage_split( double2 ( &v )[8], double * __re
             for( int r = 0; r < R; ++r )
                 Ld[pad8( idxD + r * Ns )] = out[m + M * r].x;
         }
-        __syncthreads();
+        p2_sync<WAVE>();
 #pragma unroll
         for( int k = 0; k < 8; ++k )
             v[k].x = Ld[pad8( t + T * k )];
-        __syncthreads();
+        p2_sync<WAVE>();
 #pragma unroll
         for( int m = 0; m < M; ++m )
         {
@@ -203,21 +219,21 @@ __device__ __forceinline__ void p2_stage_split( double2 ( &v )[8], double * __re
             for( int r = 0; r < R; ++r )
                 Ld[pad8( idxD + r * Ns )] = out[m + M * r].y;
         }
-        __syncthreads();
+        p2_sync<WAVE>();
 #pragma unroll
         for( int k = 0; k < 8; ++k )
             v[k].y = Ld[pad8( t + T * k )];
-        __syncthreads();
-        p2_stage_split<N, INV, S + 1>( v, Ld, tw, t );
+        p2_sync<WAVE>();
+        p2_stage_split<N, INV, S + 1, WAVE>( v, Ld, tw, t );
     }
 }
 
 // Transform ending in registers (v[k] = X[t + T*k]) through half-width LDS exchanges.
-template<int N, bool INV>
+template<int N, bool INV, bool WAVE = false>
 __device__ __forceinline__ void p2_fft_reg_split( double2 ( &v )[8], double * __restrict__ Ld,
                                                   const double2 * __restrict__ tw, int t )
 {
-    p2_stage_split<N, INV, 0>( v, Ld, tw, t );
+    p2_stage_split<N, INV, 0, WAVE>( v, Ld, tw, t );
 }
 
 // ---------------------------------------------------------------------------
