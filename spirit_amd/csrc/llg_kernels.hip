@@ -321,14 +321,15 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_gradient( StepArgs A, const d
 // Per-image reduction of the partials (one block per image, fixed order); convergence of
 // each image (Method_Solver.hpp:232-239) with the index of the step at which it converged.
 // Once every image has stopped, the run-wide flag stops all further stages.
-__global__ void __launch_bounds__( 256 )
+constexpr int FIN_THREADS = 1024;
+__global__ void __launch_bounds__( FIN_THREADS )
     k_finalize( const ImageDev * __restrict__ imgs, int n_images, const double * __restrict__ part_E,
                 const double * __restrict__ part_T, double * __restrict__ obs, int * __restrict__ stop,
                 int * __restrict__ img_stop, int * __restrict__ n_stopped, long long * __restrict__ stop_step,
                 long long step_index, const uint64_t * __restrict__ step_dev, double force_convergence,
                 int check_convergence )
 {
-    __shared__ double sE[256], sT[256];
+    __shared__ double sE[FIN_THREADS], sT[FIN_THREADS];
     const int img = blockIdx.x;
     if( *stop || img_stop[img] )
         return;
@@ -431,7 +432,7 @@ void llg_launch_finalize( const StepArgs & A, int n_images, double * obs, int * 
                           long long * stop_step, long long step_index, double force_convergence, int check_convergence,
                           hipStream_t st )
 {
-    hipLaunchKernelGGL( k_finalize, dim3( n_images ), dim3( 256 ), 0, st, A.imgs, n_images, A.part_E, A.part_T, obs,
+    hipLaunchKernelGGL( k_finalize, dim3( n_images ), dim3( FIN_THREADS ), 0, st, A.imgs, n_images, A.part_E, A.part_T, obs,
                         A.stop, img_stop, n_stopped, stop_step, step_index, A.step_dev, force_convergence,
                         check_convergence );
 }
