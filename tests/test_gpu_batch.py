@@ -94,3 +94,20 @@ def test_batch_per_replica_convergence(states):
     assert len(set(n_seq)) == 3 and max(n_seq) < 20000
     for k in range(3):
         np.testing.assert_array_equal(spins(bat[k]), spins(seq[k]))
+
+
+def test_batch_with_dipolar_replicas(states):
+    """Replicas with their own dipolar FFT plans (open films of different sizes, Heun at T > 0)
+    next to a replica without DDI: the batched run equals the sequential runs."""
+    from spirit_amd import hip, simulation
+    print_case("batch", "ddi replicas")
+    film = configs.scaled(configs.FILM_128, [12, 10, 4])
+    cfgs = [film, configs.scaled(configs.FILM_128, [16, 8, 3], llg_seed=77),
+            replica_cfg(20, 1.0, 30.0, 5)]
+    seq = states(cfgs)
+    for p in seq:
+        simulation.start(p, simulation.METHOD_LLG, 3, n_iterations=60)
+    bat = states(cfgs)
+    hip.llg_start_batch(bat, 3, n_iterations=60)
+    for k in range(len(cfgs)):
+        np.testing.assert_array_equal(spins(bat[k]), spins(seq[k]))

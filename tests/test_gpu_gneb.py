@@ -232,3 +232,16 @@ def test_gneb_saddle_point_golden_api(state, name):
     assert abs(m[2] - G["mz_saddle"]) < G["tolerance"], (name, m)
     assert gneb.get_climbing_falling(p, idx_image=i_max) == gneb.IMAGE_CLIMBING
     assert max(simulation.get_chain_max_torque_norms(p, noi)) < G["cfg"]["gneb_force_convergence"]
+
+
+def test_gneb_hexagonal_climbing_vp(state, oracle):
+    """The GNEB workload's lattice and Hamiltonian at a reduced size: a 24x24 hexagonal
+    monolayer (J=10, Neel DMI 6, K=0.4, B=3 T), 9 images from a Neel skyrmion to the
+    ferromagnet, climbing image in the middle, VP, 40 iterations."""
+    print_case("gneb", "hex 24x24 climbing vp")
+    cfg = dict(configs.HEX_128, n_basis_cells=[24, 24, 1], gneb_force_convergence=1e-14)
+    noi = 9
+    types = [0] * noi
+    types[4] = 1
+    _, cg, cr, Eg, Rg, data = run_both(state, oracle, cfg, SOLVERS["vp"], 40, noi=noi, types=types)
+    check(cg, cr, Eg, Rg, data)
