@@ -128,6 +128,8 @@ def lib():
         L.orc_gradient.argtypes = [vp, P(d), P(d)]
         L.orc_gradient_term.argtypes = [vp, i, P(d), P(d)]
         L.orc_llg_iterate.argtypes = [vp, i, P(OrcLLG), P(d), i, ctypes.c_uint64, P(d), P(d), P(d)]
+        L.orc_llg_run.argtypes = [vp, i, P(OrcLLG), P(d), ctypes.c_long, d, P(d), P(d), P(d)]
+        L.orc_llg_run.restype = ctypes.c_long
         L.orc_normal3.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(d)]
         L.orc_fft.argtypes = [P(d), i, i]
         L.orc_rotate.argtypes = [P(d), P(d), d, P(d)]
@@ -337,6 +339,21 @@ class System:
                               ctypes.byref(e), ctypes.byref(t), _dp(eff))
         del keep
         return s, e.value, t.value, eff
+
+    def llg_run(self, spins, solver, n_iterations, force_convergence, dt=1e-3, damping=0.3,
+                direct_minimization=False):
+        """Method::Iterate with the convergence test every iteration; returns (spins, energy,
+        max torque, effective field, iterations done)."""
+        s = np.array(spins, dtype=np.float64).reshape(self.nos, 3).copy()
+        par = OrcLLG()
+        par.dt, par.damping, par.temperature = dt, damping, 0.0
+        par.direct_minimization = 1 if direct_minimization else 0
+        e = ctypes.c_double(0)
+        t = ctypes.c_double(0)
+        eff = np.zeros((self.nos, 3))
+        n = lib().orc_llg_run(self.h, int(solver), ctypes.byref(par), _dp(s), int(n_iterations),
+                              float(force_convergence), ctypes.byref(e), ctypes.byref(t), _dp(eff))
+        return s, e.value, t.value, eff, n
 
     def gneb(self, chain, solver, n_iterations, image_type=None, dt=1e-3, spring_constant=1.0,
              spring_force_ratio=0.0, path_shortening_constant=0.0, moving_endpoints=False,

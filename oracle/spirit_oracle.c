@@ -1177,9 +1177,12 @@ static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, co
                                 double * Fv )
 {
     double a = par->damping;
-    if( par->direct_minimization || solver == ORC_SOLVER_VP )
+    if( solver == ORC_SOLVER_LBFGS_OSO || solver == ORC_SOLVER_LBFGS_ATLAS || par->direct_minimization
+        || solver == ORC_SOLVER_VP || solver == ORC_SOLVER_VP_OSO )
     {
-        double dtg = par->dt * ORC_GAMMA / ORC_MU_B;
+        /* Method_LLG.cpp:160-171: s x F, scaled by dt gamma / mu_B except for the L-BFGS solvers */
+        const int lbfgs = solver == ORC_SOLVER_LBFGS_OSO || solver == ORC_SOLVER_LBFGS_ATLAS;
+        double dtg      = lbfgs ? 1.0 : par->dt * ORC_GAMMA / ORC_MU_B;
 #pragma omp parallel for
         for( int i = 0; i < s->nos; ++i )
         {
@@ -1227,183 +1230,587 @@ static inline void neg_cross( const double * a, const double * b, double * o )
     o[2] = -1 * c[2];
 }
 
-void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spins, int n_steps, uint64_t step0,
-                      double * energy, double * max_torque, double * eff_field )
+/* ------------------------------------------------------------------ */
+/* Direct-minimisation solvers: VP (Solver_VP.hpp:29-114), VP_OSO      */
+/* (Solver_VP_OSO.hpp:33-110), LBFGS_OSO (Solver_LBFGS_OSO.hpp:37-72),  */
+/* LBFGS_Atlas (Solver_LBFGS_Atlas.hpp:44-101) with the kernels of     */
+/* Solver_Kernels.cpp:50-244 and Solver_Kernels.hpp:44-191.            */
+/* ------------------------------------------------------------------ */
+#define ORC_LBFGS_MEMORY 3 /* n_lbfgs_memory, Solver_LBFGS_OSO.hpp:15 */
+
+typedef struct min_state
 {
-    int nos = s->nos;
-    double E = 0;
-    for( int it = 0; it < n_steps; ++it )
+    int dim;             /* 3: cartesian / OSO tangent vectors; 2: atlas coordinates */
+    double * v;          /* VP, VP_OSO: velocities */
+    double * prev;       /* VP: previous force; VP_OSO: previous gradient */
+    double * grad;       /* VP_OSO, L-BFGS: gradient (OSO) or atlas residual */
+    double * grad_pr;    /* L-BFGS: previous gradient */
+    double * sd;         /* search direction */
+    double * q;          /* L-BFGS two-loop vector */
+    double * da[ORC_LBFGS_MEMORY], * dg[ORC_LBFGS_MEMORY];
+    double rho[ORC_LBFGS_MEMORY], alpha[ORC_LBFGS_MEMORY];
+    int local_iter;
+    double maxmove;
+    double * a3;         /* LBFGS_Atlas: chart (+1 / -1) of each spin */
+} min_state;
+
+static int is_minimiser( int solver )
+{
+    return solver == ORC_SOLVER_VP || solver == ORC_SOLVER_VP_OSO || solver == ORC_SOLVER_LBFGS_OSO
+           || solver == ORC_SOLVER_LBFGS_ATLAS;
+}
+
+/* Method_Solver<...>::Initialize of each solver (fresh state, as a new method object) */
+static min_state * min_state_create( int solver, int nos, const double * spins )
+{
+    min_state * st = (min_state *)calloc( 1, sizeof( min_state ) );
+    st->dim        = solver == ORC_SOLVER_LBFGS_ATLAS ? 2 : 3;
+    size_t n       = (size_t)st->dim * (size_t)nos;
+    st->v          = (double *)calloc( n, sizeof( double ) );
+    st->prev       = (double *)calloc( n, sizeof( double ) );
+    st->grad       = (double *)calloc( n, sizeof( double ) );
+    st->grad_pr    = (double *)calloc( n, sizeof( double ) );
+    st->sd         = (double *)calloc( n, sizeof( double ) );
+    st->q          = (double *)calloc( n, sizeof( double ) );
+    for( int k = 0; k < ORC_LBFGS_MEMORY; ++k )
     {
-        prepare_thermal( s, par, step0 + (uint64_t)it, it );
-        if( solver == ORC_SOLVER_DEPONDT )
+        st->da[k] = (double *)calloc( n, sizeof( double ) );
+        st->dg[k] = (double *)calloc( n, sizeof( double ) );
+    }
+    st->maxmove = solver == ORC_SOLVER_LBFGS_ATLAS ? 0.05 : ORC_PI / 200.0; /* Solver_LBFGS_*.hpp */
+    if( solver == ORC_SOLVER_LBFGS_ATLAS )
+    {
+        /* Solver_LBFGS_Atlas.hpp:34-41 */
+        st->a3 = (double *)calloc( (size_t)nos, sizeof( double ) );
+        for( int i = 0; i < nos; ++i )
+            st->a3[i] = spins[3 * i + 2] > 0 ? 1.0 : -1.0;
+    }
+    return st;
+}
+
+static void min_state_destroy( min_state * st )
+{
+    if( !st )
+        return;
+    free( st->v );
+    free( st->prev );
+    free( st->grad );
+    free( st->grad_pr );
+    free( st->sd );
+    free( st->q );
+    for( int k = 0; k < ORC_LBFGS_MEMORY; ++k )
+    {
+        free( st->da[k] );
+        free( st->dg[k] );
+    }
+    free( st->a3 );
+    free( st );
+}
+
+/* Vectormath::dot over a vector field (per-spin dot products, summed deterministically) */
+static double term_dot3( const struct sum_ctx * c, long i )
+{
+    return v3_dot( c->spins + 3 * i, c->grad + 3 * i );
+}
+static double term_dot2( const struct sum_ctx * c, long i )
+{
+    const double *a = c->spins + 2 * i, *b = c->grad + 2 * i;
+    return a[0] * b[0] + a[1] * b[1];
+}
+static double field_dot( const double * a, const double * b, int nos, int dim )
+{
+    struct sum_ctx c = { NULL, a, b, NULL };
+    return det_sum( nos, dim == 3 ? term_dot3 : term_dot2, &c );
+}
+
+/* Solver_Kernels::lbfgs_get_searchdir, Solver_Kernels.hpp:51-191 (one image) */
+static void lbfgs_searchdir( min_state * st, int nos )
+{
+    const int M      = ORC_LBFGS_MEMORY;
+    const long n     = (long)st->dim * nos;
+    const double eps = 1e-300;
+    const int m      = st->local_iter % M;
+    if( st->local_iter == 0 )
+    {
+        for( long j = 0; j < n; ++j )
         {
-            /* Solver_Depondt.hpp:29-77 */
-            E = calc_force( s, spins, s->f1 );
-            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
-#pragma omp parallel for
+            st->grad_pr[j] = st->grad[j];
+            st->sd[j]      = -st->grad[j];
+        }
+        for( int k = 0; k < M; ++k )
+        {
+            st->rho[k] = 0.0;
+            memset( st->da[k], 0, sizeof( double ) * (size_t)n );
+            memset( st->dg[k], 0, sizeof( double ) * (size_t)n );
+        }
+    }
+    else
+    {
+        for( long j = 0; j < n; ++j )
+        {
+            st->da[m][j] = st->sd[j];
+            st->dg[m][j] = st->grad[j] - st->grad_pr[j];
+        }
+        double rinv = field_dot( st->dg[m], st->da[m], nos, st->dim );
+        if( rinv > eps )
+            st->rho[m] = 1.0 / rinv;
+        else
+        {
+            st->local_iter = 0;
+            lbfgs_searchdir( st, nos );
+            return;
+        }
+        memcpy( st->q, st->grad, sizeof( double ) * (size_t)n );
+        for( int k = M - 1; k > -1; --k )
+        {
+            const int c  = ( k + m + 1 ) % M;
+            double temp  = field_dot( st->da[c], st->q, nos, st->dim );
+            st->alpha[c] = st->rho[c] * temp;
+            for( long j = 0; j < n; ++j )
+                st->q[j] += -st->alpha[c] * st->dg[c][j];
+        }
+        double dy2    = field_dot( st->dg[m], st->dg[m], nos, st->dim );
+        double rhody2 = dy2 * st->rho[m];
+        double inv    = rhody2 > eps ? 1.0 / rhody2 : 1.0 / ( eps );
+        for( long j = 0; j < n; ++j )
+            st->sd[j] = inv * st->q[j];
+        for( int k = 0; k < M; ++k )
+        {
+            const int c   = st->local_iter < M ? k : ( k + m + 1 ) % M;
+            double rhopdg = field_dot( st->dg[c], st->sd, nos, st->dim );
+            rhopdg *= st->rho[c];
+            for( long j = 0; j < n; ++j )
+                st->sd[j] += ( st->alpha[c] - rhopdg ) * st->da[c][j];
+        }
+        for( long j = 0; j < n; ++j )
+        {
+            st->grad_pr[j] = st->grad[j];
+            st->sd[j]      = -st->sd[j];
+        }
+    }
+    st->local_iter++;
+}
+
+/* sqrt(sum |d_i|^2 / nos) -> maxmove / rms if above maxmove (Solver_Kernels.cpp:95-103,
+   Solver_LBFGS_Atlas.hpp:74-86) */
+static double rms_scaling( const min_state * st, int nos )
+{
+    double ss  = field_dot( st->sd, st->sd, nos, st->dim );
+    double rms = sqrt( ss / nos );
+    return rms > st->maxmove ? st->maxmove / rms : 1.0;
+}
+
+/* Solver_Kernels::oso_rotate, Solver_Kernels.cpp:63-93 (one spin) */
+static void oso_rotate_spin( double * sp, const double * sd )
+{
+    double theta = sqrt( v3_sqnorm( sd ) );
+    double q = cos( theta ), w = 1 - q, x = -sd[0] / theta, y = -sd[1] / theta, z = -sd[2] / theta,
+           s1 = -y * z * w, s2 = x * z * w, s3 = -x * y * w, p1 = x * sin( theta ), p2 = y * sin( theta ),
+           p3 = z * sin( theta );
+    if( theta > 1.0e-20 )
+    {
+        double t1 = ( q + z * z * w ) * sp[0] + ( s1 + p1 ) * sp[1] + ( s2 + p2 ) * sp[2];
+        double t2 = ( s1 - p1 ) * sp[0] + ( q + y * y * w ) * sp[1] + ( s3 + p3 ) * sp[2];
+        double t3 = ( s2 - p2 ) * sp[0] + ( s3 - p3 ) * sp[1] + ( q + x * x * w ) * sp[2];
+        sp[0]     = t1;
+        sp[1]     = t2;
+        sp[2]     = t3;
+    }
+}
+
+/* the OSO gradient t (-s x F), t = ((0,0,1),(0,-1,0),(1,0,0)) (Solver_Kernels.cpp:50-61) */
+static void oso_gradient( const double * sp, const double * F, double * g )
+{
+    double c[3];
+    v3_cross( sp, F, c );
+    const double w[3] = { -c[0], -c[1], -c[2] };
+    g[0] = w[2];
+    g[1] = -w[1];
+    g[2] = w[0];
+}
+
+/* per-spin sums of the atlas chart change (lbfgs_atlas_transform_direction's rho update) */
+struct atlas_ctx
+{
+    const double * spins;
+    const double * a3;
+    const double * da;
+    const double * dg;
+};
+static double term_atlas_rho( const struct sum_ctx * c, long i )
+{
+    const struct atlas_ctx * a = (const struct atlas_ctx *)c->ext;
+    const double sz            = a->spins[3 * i + 2];
+    if( !( sz * a->a3[i] < 0 ) )
+        return 0.0;
+    const double a3n    = sz > 0 ? 1 : -1;
+    const double factor = ( 1 - a3n * sz ) / ( 1 + a3n * sz );
+    return ( factor * factor - 1 ) * ( a->da[2 * i] * a->dg[2 * i] + a->da[2 * i + 1] * a->dg[2 * i + 1] );
+}
+
+/* One iteration of a direct-minimisation solver; fills s->f1 (force) and s->fv1 (the virtual
+   force Hook_Post_Iteration projects).  Returns the energy of the force evaluation. */
+static double min_iteration( orc_sys * s, int solver, const orc_llg * par, double * spins, min_state * st )
+{
+    const int nos     = s->nos;
+    const double mass = 1.0; /* Method_Solver.hpp:174 */
+    const double dt   = par->dt;
+    double E          = calc_force( s, spins, s->f1 );
+    const double * F  = s->f1;
+    calc_force_virtual( s, par, solver, spins, F, s->fv1 );
+    if( solver == ORC_SOLVER_VP || solver == ORC_SOLVER_VP_OSO )
+    {
+        /* VP on the force (VP) or on the OSO gradient's negative (VP_OSO) */
+        double * g = solver == ORC_SOLVER_VP ? s->tmp : st->grad;
+        if( solver == ORC_SOLVER_VP )
+            memcpy( g, F, sizeof( double ) * 3 * (size_t)nos );
+        else
+        {
             for( int i = 0; i < nos; ++i )
             {
-                double axis[3] = { 1 * s->fv1[3 * i], 1 * s->fv1[3 * i + 1], 1 * s->fv1[3 * i + 2] };
-                double angle   = sqrt( v3_sqnorm( s->fv1 + 3 * i ) );
-                v3_normalize( axis );
-                orc_rotate( spins + 3 * i, axis, angle, s->sp + 3 * i );
+                oso_gradient( spins + 3 * i, F + 3 * i, g + 3 * i );
+                for( int q = 0; q < 3; ++q )
+                    g[3 * i + q] *= -1.0;
             }
-            E = calc_force( s, s->sp, s->f2 );
-            calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
-#pragma omp parallel for
+        }
+        for( long j = 0; j < 3L * nos; ++j )
+            st->v[j] += 0.5 / mass * ( st->prev[j] + g[j] );
+        double proj  = field_dot( st->v, g, nos, 3 );
+        double fn2   = field_dot( g, g, nos, 3 );
+        double ratio = proj / fn2;
+        for( long j = 0; j < 3L * nos; ++j )
+            st->v[j] = proj <= 0 ? 0.0 : g[j] * ratio;
+        if( solver == ORC_SOLVER_VP )
+        {
             for( int i = 0; i < nos; ++i )
             {
                 double t[3];
                 for( int q = 0; q < 3; ++q )
-                    t[q] = 0.5 * s->fv1[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 0.5 * s->fv2[3 * i + q];
-                double angle = sqrt( v3_sqnorm( t ) );
+                    t[q] = spins[3 * i + q] + dt * st->v[3 * i + q] + 0.5 / mass * dt * g[3 * i + q];
                 v3_normalize( t );
-                orc_rotate( spins + 3 * i, t, angle, spins + 3 * i );
+                for( int q = 0; q < 3; ++q )
+                    spins[3 * i + q] = t[q];
             }
         }
-        else if( solver == ORC_SOLVER_HEUN )
+        else
         {
-            /* Solver_Heun.hpp:30-83 */
-            E = calc_force( s, spins, s->f1 );
-            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+            for( int i = 0; i < nos; ++i )
+            {
+                for( int q = 0; q < 3; ++q )
+                    st->sd[3 * i + q] = dt * st->v[3 * i + q] + 0.5 / mass * dt * g[3 * i + q];
+                oso_rotate_spin( spins + 3 * i, st->sd + 3 * i );
+            }
+        }
+        memcpy( st->prev, g, sizeof( double ) * 3 * (size_t)nos );
+        return E;
+    }
+    if( solver == ORC_SOLVER_LBFGS_OSO )
+    {
+        for( int i = 0; i < nos; ++i )
+            oso_gradient( spins + 3 * i, F + 3 * i, st->grad + 3 * i );
+        lbfgs_searchdir( st, nos );
+        double scaling = rms_scaling( st, nos );
+        if( scaling > 1 )
+            scaling = 1;
+        for( int i = 0; i < nos; ++i )
+        {
+            for( int q = 0; q < 3; ++q )
+                st->sd[3 * i + q] *= scaling;
+            oso_rotate_spin( spins + 3 * i, st->sd + 3 * i );
+        }
+        return E;
+    }
+    /* LBFGS_Atlas */
+    for( int i = 0; i < nos; ++i )
+    {
+        /* Solver_Kernels::atlas_calc_gradients, Solver_Kernels.cpp:129-152 */
+        const double * sp = spins + 3 * i;
+        const double * f  = F + 3 * i;
+        const double a3   = st->a3[i];
+        double J00 = sp[1] * sp[1] + sp[2] * ( sp[2] + a3 );
+        double J10 = -sp[0] * sp[1];
+        double J01 = -sp[0] * sp[1];
+        double J11 = sp[0] * sp[0] + sp[2] * ( sp[2] + a3 );
+        double J02 = -sp[0] * ( sp[2] + a3 );
+        double J12 = -sp[1] * ( sp[2] + a3 );
+        st->grad[2 * i]     = -( J00 * f[0] + J01 * f[1] + J02 * f[2] );
+        st->grad[2 * i + 1] = -( J10 * f[0] + J11 * f[1] + J12 * f[2] );
+    }
+    lbfgs_searchdir( st, nos );
+    double scaling = rms_scaling( st, nos );
+    int any_flip   = 0;
+    for( int i = 0; i < nos; ++i )
+    {
+        double * d = st->sd + 2 * i;
+        d[0] *= scaling;
+        d[1] *= scaling;
+        /* Solver_Kernels::atlas_rotate, Solver_Kernels.cpp:105-127 */
+        double * sp        = spins + 3 * i;
+        const double a3    = st->a3[i];
+        const double gamma = ( 1 + sp[2] * a3 );
+        const double denom = ( sp[0] * sp[0] + sp[1] * sp[1] ) / gamma + 2 * ( d[0] * sp[0] + d[1] * sp[1] )
+                             + gamma * ( d[0] * d[0] + d[1] * d[1] );
+        sp[0]          = 2 * ( sp[0] + d[0] * gamma );
+        sp[1]          = 2 * ( sp[1] + d[1] * gamma );
+        sp[2]          = a3 * ( gamma - denom );
+        const double r = 1 / ( gamma + denom );
+        sp[0] *= r;
+        sp[1] *= r;
+        sp[2] *= r;
+        /* ncg_atlas_check_coordinates, Solver_Kernels.cpp:154-180 (tol -0.6) */
+        if( sp[2] * a3 < -0.6 )
+            any_flip = 1;
+    }
+    if( any_flip )
+    {
+        /* lbfgs_atlas_transform_direction, Solver_Kernels.cpp:182-242.  The reference adds the
+           per-spin terms into rho[n] from parallel threads without synchronisation; here the
+           terms are summed deterministically and added once. */
+        for( int n = 0; n < ORC_LBFGS_MEMORY; ++n )
+        {
+            struct atlas_ctx a = { spins, st->a3, st->da[n], st->dg[n] };
+            double add         = det_sum( nos, term_atlas_rho, &( struct sum_ctx ){ s, NULL, NULL, (const double *)&a } );
+            st->rho[n]         = 1 / st->rho[n];
+            st->rho[n]         = st->rho[n] + add;
+            st->rho[n]         = 1 / st->rho[n];
+        }
+        for( int i = 0; i < nos; ++i )
+        {
+            const double sz = spins[3 * i + 2];
+            if( sz * st->a3[i] < 0 )
+            {
+                st->a3[i]           = sz > 0 ? 1 : -1;
+                const double factor = ( 1 - st->a3[i] * sz ) / ( 1 + st->a3[i] * sz );
+                st->sd[2 * i] *= factor;
+                st->sd[2 * i + 1] *= factor;
+                st->grad_pr[2 * i] *= factor;
+                st->grad_pr[2 * i + 1] *= factor;
+                for( int n = 0; n < ORC_LBFGS_MEMORY; ++n )
+                {
+                    st->da[n][2 * i] *= factor;
+                    st->da[n][2 * i + 1] *= factor;
+                    st->dg[n][2 * i] *= factor;
+                    st->dg[n][2 * i + 1] *= factor;
+                }
+            }
+        }
+    }
+    return E;
+}
+
+/* One LLG iteration of a dynamics solver (Method_LLG.cpp:66-301 + Solver_*.hpp) */
+static double llg_step( orc_sys * s, int solver, const orc_llg * par, double * spins, uint64_t step, int it )
+{
+    const int nos = s->nos;
+    double E      = 0;
+    prepare_thermal( s, par, step, it );
+    if( solver == ORC_SOLVER_DEPONDT )
+    {
+        /* Solver_Depondt.hpp:29-77 */
+        E = calc_force( s, spins, s->f1 );
+        calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+        {
+            double axis[3] = { 1 * s->fv1[3 * i], 1 * s->fv1[3 * i + 1], 1 * s->fv1[3 * i + 2] };
+            double angle   = sqrt( v3_sqnorm( s->fv1 + 3 * i ) );
+            v3_normalize( axis );
+            orc_rotate( spins + 3 * i, axis, angle, s->sp + 3 * i );
+        }
+        E = calc_force( s, s->sp, s->f2 );
+        calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+        {
+            double t[3];
+            for( int q = 0; q < 3; ++q )
+                t[q] = 0.5 * s->fv1[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                t[q] += 0.5 * s->fv2[3 * i + q];
+            double angle = sqrt( v3_sqnorm( t ) );
+            v3_normalize( t );
+            orc_rotate( spins + 3 * i, t, angle, spins + 3 * i );
+        }
+    }
+    else if( solver == ORC_SOLVER_HEUN )
+    {
+        /* Solver_Heun.hpp:30-83 */
+        E = calc_force( s, spins, s->f1 );
+        calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+        {
+            double * t  = s->tmp + 3 * i;
+            double * pr = s->sp + 3 * i;
+            neg_cross( spins + 3 * i, s->fv1 + 3 * i, t );
+            for( int q = 0; q < 3; ++q )
+                pr[q] = 1 * spins[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                pr[q] += 1 * t[q];
+            v3_normalize( pr );
+        }
+        E = calc_force( s, s->sp, s->f2 );
+        calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+        {
+            double * t = s->tmp + 3 * i;
+            double t1[3];
+            for( int q = 0; q < 3; ++q )
+                t[q] *= 0.5;
+            for( int q = 0; q < 3; ++q )
+                t[q] += 1 * spins[3 * i + q];
+            neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, t1 );
+            for( int q = 0; q < 3; ++q )
+                t[q] += 0.5 * t1[q];
+            v3_normalize( t );
+            for( int q = 0; q < 3; ++q )
+                spins[3 * i + q] = t[q];
+        }
+    }
+    else if( solver == ORC_SOLVER_SIB )
+    {
+        /* Solver_SIB.hpp:22-52 */
+        E = calc_force( s, spins, s->f1 );
+        calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+        {
+            double * pr = s->sp + 3 * i;
+            orc_sib_transform( spins + 3 * i, s->fv1 + 3 * i, pr );
+            for( int q = 0; q < 3; ++q )
+                pr[q] += 1 * spins[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                pr[q] *= 0.5;
+        }
+        E = calc_force( s, s->sp, s->f2 );
+        calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
+#pragma omp parallel for
+        for( int i = 0; i < nos; ++i )
+            orc_sib_transform( spins + 3 * i, s->fv2 + 3 * i, spins + 3 * i );
+    }
+    else if( solver == ORC_SOLVER_RK4 )
+    {
+        /* Solver_RK4.hpp:41-147 */
+        double * ks[4] = { s->k1, s->k2, s->k3, s->k4 };
+        const double cpred[3] = { 0.5, 0.5, 1.0 };
+        E = calc_force( s, spins, s->f1 );
+        calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+        const double * conf = spins;
+        const double * fv   = s->fv1;
+        for( int stage = 0; stage < 3; ++stage )
+        {
+            double * k = ks[stage];
 #pragma omp parallel for
             for( int i = 0; i < nos; ++i )
             {
-                double * t  = s->tmp + 3 * i;
-                double * pr = s->sp + 3 * i;
-                neg_cross( spins + 3 * i, s->fv1 + 3 * i, t );
+                neg_cross( conf + 3 * i, fv + 3 * i, k + 3 * i );
+                double pr[3];
                 for( int q = 0; q < 3; ++q )
                     pr[q] = 1 * spins[3 * i + q];
                 for( int q = 0; q < 3; ++q )
-                    pr[q] += 1 * t[q];
+                    pr[q] += cpred[stage] * k[3 * i + q];
                 v3_normalize( pr );
+                for( int q = 0; q < 3; ++q )
+                    s->tmp[3 * i + q] = pr[q];
             }
+            memcpy( s->sp, s->tmp, sizeof( double ) * 3 * (size_t)nos );
             E = calc_force( s, s->sp, s->f2 );
             calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
-#pragma omp parallel for
-            for( int i = 0; i < nos; ++i )
-            {
-                double * t = s->tmp + 3 * i;
-                double t1[3];
-                for( int q = 0; q < 3; ++q )
-                    t[q] *= 0.5;
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 1 * spins[3 * i + q];
-                neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, t1 );
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 0.5 * t1[q];
-                v3_normalize( t );
-                for( int q = 0; q < 3; ++q )
-                    spins[3 * i + q] = t[q];
-            }
+            conf = s->sp;
+            fv   = s->fv2;
         }
-        else if( solver == ORC_SOLVER_SIB )
-        {
-            /* Solver_SIB.hpp:22-52 */
-            E = calc_force( s, spins, s->f1 );
-            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
 #pragma omp parallel for
-            for( int i = 0; i < nos; ++i )
-            {
-                double * pr = s->sp + 3 * i;
-                orc_sib_transform( spins + 3 * i, s->fv1 + 3 * i, pr );
-                for( int q = 0; q < 3; ++q )
-                    pr[q] += 1 * spins[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    pr[q] *= 0.5;
-            }
-            E = calc_force( s, s->sp, s->f2 );
-            calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
-#pragma omp parallel for
-            for( int i = 0; i < nos; ++i )
-                orc_sib_transform( spins + 3 * i, s->fv2 + 3 * i, spins + 3 * i );
-        }
-        else if( solver == ORC_SOLVER_RK4 )
-        {
-            /* Solver_RK4.hpp:41-147 */
-            double * ks[4] = { s->k1, s->k2, s->k3, s->k4 };
-            const double cpred[3] = { 0.5, 0.5, 1.0 };
-            E = calc_force( s, spins, s->f1 );
-            calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
-            const double * conf = spins;
-            const double * fv   = s->fv1;
-            for( int stage = 0; stage < 3; ++stage )
-            {
-                double * k = ks[stage];
-#pragma omp parallel for
-                for( int i = 0; i < nos; ++i )
-                {
-                    neg_cross( conf + 3 * i, fv + 3 * i, k + 3 * i );
-                    double pr[3];
-                    for( int q = 0; q < 3; ++q )
-                        pr[q] = 1 * spins[3 * i + q];
-                    for( int q = 0; q < 3; ++q )
-                        pr[q] += cpred[stage] * k[3 * i + q];
-                    v3_normalize( pr );
-                    for( int q = 0; q < 3; ++q )
-                        s->tmp[3 * i + q] = pr[q];
-                }
-                memcpy( s->sp, s->tmp, sizeof( double ) * 3 * (size_t)nos );
-                E = calc_force( s, s->sp, s->f2 );
-                calc_force_virtual( s, par, solver, s->sp, s->f2, s->fv2 );
-                conf = s->sp;
-                fv   = s->fv2;
-            }
-#pragma omp parallel for
-            for( int i = 0; i < nos; ++i )
-            {
-                neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, s->k4 + 3 * i );
-                double t[3];
-                for( int q = 0; q < 3; ++q )
-                    t[q] = 1 * spins[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 1.0 / 6.0 * s->k1[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 1.0 / 3.0 * s->k2[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 1.0 / 3.0 * s->k3[3 * i + q];
-                for( int q = 0; q < 3; ++q )
-                    t[q] += 1.0 / 6.0 * s->k4[3 * i + q];
-                v3_normalize( t );
-                for( int q = 0; q < 3; ++q )
-                    spins[3 * i + q] = t[q];
-            }
-        }
-    }
-    if( energy )
-        *energy = E;
-    /* Hook_Post_Iteration, Method_LLG.cpp:247-301: torque of the first virtual force, projected on the new spins */
-    if( max_torque && n_steps > 0 )
-    {
-        double m = 0;
-#pragma omp parallel for reduction( max : m )
         for( int i = 0; i < nos; ++i )
         {
-            double * f = s->fv1 + 3 * i;
-            const double * v = spins + 3 * i;
-            double d       = v3_dot( f, v );
-            double pr[3];
+            neg_cross( s->sp + 3 * i, s->fv2 + 3 * i, s->k4 + 3 * i );
+            double t[3];
             for( int q = 0; q < 3; ++q )
-                pr[q] = f[q] - d * v[q];
-            double n2 = v3_sqnorm( pr );
-            if( n2 > m )
-                m = n2;
+                t[q] = 1 * spins[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                t[q] += 1.0 / 6.0 * s->k1[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                t[q] += 1.0 / 3.0 * s->k2[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                t[q] += 1.0 / 3.0 * s->k3[3 * i + q];
+            for( int q = 0; q < 3; ++q )
+                t[q] += 1.0 / 6.0 * s->k4[3 * i + q];
+            v3_normalize( t );
+            for( int q = 0; q < 3; ++q )
+                spins[3 * i + q] = t[q];
         }
-        *max_torque = sqrt( m );
     }
-    if( eff_field && n_steps > 0 )
+    return E;
+}
+
+/* Hook_Post_Iteration, Method_LLG.cpp:247-301 and Method_Solver.hpp:224-230: max torque of
+   the virtual force s->fv1 projected on the new spins */
+static double llg_max_torque( const orc_sys * s, const double * spins )
+{
+    const int nos = s->nos;
+    double m      = 0;
+#pragma omp parallel for reduction( max : m )
+    for( int i = 0; i < nos; ++i )
+    {
+        const double * f = s->fv1 + 3 * i;
+        const double * v = spins + 3 * i;
+        double d         = v3_dot( f, v );
+        double pr[3];
+        for( int q = 0; q < 3; ++q )
+            pr[q] = f[q] - d * v[q];
+        double n2 = v3_sqnorm( pr );
+        if( n2 > m )
+            m = n2;
+    }
+    return sqrt( m );
+}
+
+static long llg_loop( orc_sys * s, int solver, const orc_llg * par, double * spins, long n_steps, uint64_t step0,
+                      double force_convergence, int check, double * energy, double * max_torque, double * eff_field )
+{
+    const int nos   = s->nos;
+    double E        = 0;
+    min_state * st  = is_minimiser( solver ) ? min_state_create( solver, nos, spins ) : NULL;
+    long done       = 0;
+    while( done < n_steps )
+    {
+        if( st )
+            E = min_iteration( s, solver, par, spins, st );
+        else
+            E = llg_step( s, solver, par, spins, step0 + (uint64_t)done, (int)done );
+        ++done;
+        if( check && llg_max_torque( s, spins ) < force_convergence )
+            break;
+    }
+    min_state_destroy( st );
+    if( energy )
+        *energy = E;
+    if( max_torque && done > 0 )
+        *max_torque = llg_max_torque( s, spins );
+    if( eff_field && done > 0 )
     {
         for( int i = 0; i < nos; ++i )
         {
             const double * f = s->f1 + 3 * i;
             const double * v = spins + 3 * i;
-            double d       = v3_dot( f, v );
+            double d         = v3_dot( f, v );
             for( int q = 0; q < 3; ++q )
                 eff_field[3 * i + q] = f[q] - d * v[q];
         }
     }
+    return done;
+}
+
+void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spins, int n_steps, uint64_t step0,
+                      double * energy, double * max_torque, double * eff_field )
+{
+    llg_loop( s, solver, par, spins, n_steps, step0, 0.0, 0, energy, max_torque, eff_field );
+}
+
+long orc_llg_run( orc_sys * s, int solver, const orc_llg * par, double * spins, long n_iterations,
+                  double force_convergence, double * energy, double * max_torque, double * eff_field )
+{
+    return llg_loop( s, solver, par, spins, n_iterations, 0, force_convergence, 1, energy, max_torque, eff_field );
 }
 
 /* ======================================================================

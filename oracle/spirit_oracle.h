@@ -37,6 +37,9 @@ extern "C" {
 #define ORC_SOLVER_DEPONDT 2
 #define ORC_SOLVER_HEUN 3
 #define ORC_SOLVER_RK4 4
+#define ORC_SOLVER_LBFGS_OSO 5
+#define ORC_SOLVER_LBFGS_ATLAS 6
+#define ORC_SOLVER_VP_OSO 7
 
 /* DDI methods, core/include/Spirit/Hamiltonian.h:47-57 */
 #define ORC_DDI_NONE 0
@@ -146,6 +149,15 @@ typedef struct orc_llg
    (Method_Solver.hpp:224-230), effective field (Method_LLG.cpp:287-288). */
 void orc_llg_iterate( orc_sys * s, int solver, const orc_llg * par, double * spins, int n_steps, uint64_t step0,
                       double * energy, double * max_torque, double * eff_field );
+
+/* As orc_llg_iterate, with Method::Iterate's convergence test (Method.cpp:57-113,
+   n_iterations_amortize = 1; Method_LLG.cpp:247-270): stops after the first iteration whose
+   max torque is below force_convergence.  Returns the number of iterations done.  The
+   direct-minimisation solvers VP, VP_OSO, LBFGS_OSO and LBFGS_Atlas keep their state
+   (velocities, L-BFGS memory, atlas charts) across the iterations of one call, as one
+   method object does; every call starts them afresh, as Simulation_LLG_Start does. */
+long orc_llg_run( orc_sys * s, int solver, const orc_llg * par, double * spins, long n_iterations,
+                  double force_convergence, double * energy, double * max_torque, double * eff_field );
 
 /* Thermal N(0,1) triple for site i at step `step` (Philox4x32-10 + Box-Muller) -- the build's own
    counter-based generator that replaces the reference's sequential mt19937 stream. */

@@ -489,10 +489,13 @@ static LLGDev make_llg_dev( const LLGParameters & p )
 
 LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : specs( images ), solver( solver_ )
 {
-    if( solver != SOLVER_DEPONDT && solver != SOLVER_HEUN && solver != SOLVER_SIB && solver != SOLVER_RK4 )
+    if( solver != SOLVER_DEPONDT && solver != SOLVER_HEUN && solver != SOLVER_SIB && solver != SOLVER_RK4
+        && !min_is_solver( solver ) )
         throw std::runtime_error( "LLG solver " + std::to_string( solver ) + " is not available in the HIP engine" );
     if( specs.empty() )
         throw std::runtime_error( "LLGRunner: no images" );
+    if( min_is_solver( solver ) && specs.size() != 1 )
+        throw std::runtime_error( "the direct-minimisation solvers run one image at a time" );
     HIPCHECK( hipStreamCreateWithFlags( &stream, hipStreamNonBlocking ) );
     long long off = 0;
     n_blocks      = 0;
@@ -563,6 +566,60 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
                          hipMemcpyHostToDevice ) );
     upload_images();
     upload_spins();
+    if( min_is_solver( solver ) )
+        setup_minimiser();
+}
+
+// Buffers and scalar state of a direct-minimisation run, zero as after the solver's
+// Initialize (Solver_VP.hpp:2-18, Solver_VP_OSO.hpp:2-19, Solver_LBFGS_OSO.hpp:12-31,
+// Solver_LBFGS_Atlas.hpp:13-42); the atlas charts from the signs of the initial s_z.
+void LLGRunner::setup_minimiser()
+{
+    const size_t nos = (size_t)n_total;
+    const int dim    = solver == SOLVER_LBFGS_ATLAS ? 2 : 3;
+    auto zeroed      = [&]( DeviceBuffer & b, size_t bytes ) {
+        b.resize( bytes );
+        HIPCHECK( hipMemset( b.as<void>(), 0, bytes ) );
+    };
+    const size_t f3 = sizeof( double ) * 3 * nos, fd = sizeof( double ) * dim * nos;
+    zeroed( b_mv, f3 );
+    zeroed( b_mprev, f3 );
+    zeroed( b_mf, f3 );
+    zeroed( b_mgrad, fd );
+    zeroed( b_mgpr, fd );
+    zeroed( b_msd, fd );
+    zeroed( b_mq, fd );
+    for( int k = 0; k < MIN_MEMORY; ++k )
+    {
+        zeroed( b_mda[k], fd );
+        zeroed( b_mdg[k], fd );
+    }
+    zeroed( b_mpart, sizeof( double ) * MIN_SLOTS * (size_t)n_blocks );
+    zeroed( b_msc, sizeof( MinScalars ) );
+    b_ma3.resize( sizeof( double ) * nos );
+    std::vector<double> a3( nos, 1.0 );
+    for( size_t i = 0; i < nos; ++i )
+        a3[i] = specs[0].spins_aos[3 * i + 2] > 0 ? 1.0 : -1.0;
+    HIPCHECK( hipMemcpy( b_ma3.as<void>(), a3.data(), sizeof( double ) * nos, hipMemcpyHostToDevice ) );
+    MinArgs & M = min_args;
+    M.v         = b_mv.as<double>();
+    M.prev      = b_mprev.as<double>();
+    M.f         = b_mf.as<double>();
+    M.grad      = b_mgrad.as<double>();
+    M.grad_pr   = b_mgpr.as<double>();
+    M.sd        = b_msd.as<double>();
+    M.q         = b_mq.as<double>();
+    for( int k = 0; k < MIN_MEMORY; ++k )
+    {
+        M.da[k] = b_mda[k].as<double>();
+        M.dg[k] = b_mdg[k].as<double>();
+    }
+    M.a3       = b_ma3.as<double>();
+    M.part     = b_mpart.as<double>();
+    M.sc       = b_msc.as<MinScalars>();
+    M.n_blocks = n_blocks;
+    M.dim      = dim;
+    M.maxmove  = solver == SOLVER_LBFGS_ATLAS ? 0.05 : Pi / 200.0; // Solver_LBFGS_*.hpp Initialize
 }
 
 LLGRunner::~LLGRunner()
@@ -761,6 +818,13 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
 
 void LLGRunner::launch_step( const StepArgs & A )
 {
+    if( min_is_solver( solver ) )
+    {
+        if( any_ddi )
+            ddi_field( A.s, profiling );
+        min_launch_iteration( solver, A, min_args, (int)n_total, stream );
+        return;
+    }
     const int n_stages = solver == SOLVER_RK4 ? 4 : 2;
     for( int st = 0; st < n_stages; ++st )
     {

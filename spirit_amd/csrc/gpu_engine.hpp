@@ -5,6 +5,7 @@
 #include "gneb.hpp"
 #include "host_core.hpp"
 #include "llg.hpp"
+#include "minimize.hpp"
 
 #include <memory>
 #include <stdexcept>
@@ -265,6 +266,11 @@ private:
     int stop_host = 0, nstop_host = 0;
     bool any_ddi = false;
     long noise_steps = 0, noise_pos = 0;
+    // direct-minimisation solvers (VP, VP_OSO, LBFGS_OSO, LBFGS_Atlas): state of the run
+    void setup_minimiser();
+    DeviceBuffer b_mv, b_mprev, b_mf, b_mgrad, b_mgpr, b_msd, b_mq, b_ma3, b_mpart, b_msc;
+    DeviceBuffer b_mda[MIN_MEMORY], b_mdg[MIN_MEMORY];
+    MinArgs min_args{};
 };
 
 // Differentiable LLG run of one image (Depondt or Heun, fixed number of steps): forward with

@@ -339,9 +339,11 @@ __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, dou
                                                const double * F, const double * xi, double * fv )
 {
     const double a = P.damping;
-    if( P.direct_minimization || solver == SOLVER_VP )
+    if( solver == SOLVER_LBFGS_OSO || solver == SOLVER_LBFGS_ATLAS || P.direct_minimization || solver == SOLVER_VP
+        || solver == SOLVER_VP_OSO )
     {
-        const double dtg = P.dt * gamma_e / mu_B;
+        // Method_LLG.cpp:160-171: s x F, scaled by dt gamma / mu_B except for the L-BFGS solvers
+        const double dtg = ( solver == SOLVER_LBFGS_OSO || solver == SOLVER_LBFGS_ATLAS ) ? 1.0 : P.dt * gamma_e / mu_B;
         double c[3];
         cross3( s, F, c );
 #pragma unroll

@@ -252,6 +252,63 @@ def test_solvers_minimum_energy(oracle):
     assert abs(float(np.float32(mz)) - G["mz_minimum"]) < G["tolerance"]
 
 
+MINIMISERS = {"vp": 0, "lbfgs_oso": 5, "lbfgs_atlas": 6, "vp_oso": 7}
+
+
+@pytest.mark.parametrize("name", list(MINIMISERS))
+def test_minimisers_reach_reference_minimum(oracle, name):
+    """core/test/test_solvers.cpp:40-72 for the direct-minimisation solvers (VP, VP_OSO,
+    LBFGS_OSO, LBFGS_Atlas): one run from the skyrmion to llg_force_convergence reaches the
+    reference's minimum energy and magnetisation (float, 1e-5)."""
+    G = GOLD["solvers"]
+    cfg = G["cfg"]
+    S = oracle.System(cfg)
+    pos = S.positions()
+    center = 0.5 * (pos.min(axis=0) + pos.max(axis=0))
+    s0 = _skyrmion(pos, center, G["skyrmion"])
+    s, e, torque, _, n = S.llg_run(s0, MINIMISERS[name], 200000, cfg["llg_force_convergence"], dt=cfg["llg_dt"],
+                                   damping=cfg["llg_damping"], direct_minimization=True)
+    assert 0 < n < 200000 and torque < cfg["llg_force_convergence"]
+    _, e = S.gradient_and_energy(s)
+    assert abs(float(np.float32(e)) - G["energy_minimum"]) < G["tolerance"]
+    assert abs(float(np.float32(2.0 * s[:, 2].mean())) - G["mz_minimum"]) < G["tolerance"]
+    np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-12)
+
+
+def _skew(w):
+    return np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+
+
+def test_minimisers_first_iteration_closed_forms(oracle):
+    """First iteration of each solver from rest, against closed forms: VP (v = F/2, projected
+    to F/2) moves s to normalize(s + dt F); LBFGS_OSO's steepest-descent step is the rotation
+    exp([omega]_x) s with omega = c (s x F), c = min(1, (pi/200) / rms|s x F|) -- the matrix
+    exponential from scipy, not the solver's own rotation formula; VP_OSO's step is the same
+    rotation with c = dt (the OSO gradient's negative is t (s x F), v = that gradient / 2
+    projected to it / 2, search direction dt v + dt/2 g)."""
+    from scipy.linalg import expm
+    G = GOLD["solvers"]
+    cfg = dict(G["cfg"])
+    S = oracle.System(cfg)
+    s0 = oracle.random_spins(S.nos, 31)
+    dt = cfg["llg_dt"]
+    g, _ = S.gradient_and_energy(s0)
+    F = -g
+    s_vp, _, _, _ = S.llg(s0, 0, 1, dt=dt, direct_minimization=True)
+    ref = s0 + dt * F
+    ref /= np.linalg.norm(ref, axis=1)[:, None]
+    np.testing.assert_allclose(s_vp, ref, rtol=0, atol=1e-13)
+    c = np.cross(s0, F)
+    rms = math.sqrt((c * c).sum() / S.nos)
+    scale = min(1.0, (math.pi / 200.0) / rms)
+    s_lb, _, _, _ = S.llg(s0, 5, 1, dt=dt, direct_minimization=True)
+    ref = np.array([expm(_skew(scale * c[i])) @ s0[i] for i in range(S.nos)])
+    np.testing.assert_allclose(s_lb, ref, rtol=0, atol=1e-13)
+    s_vo, _, _, _ = S.llg(s0, 7, 1, dt=dt, direct_minimization=True)
+    ref = np.array([expm(_skew(dt * c[i])) @ s0[i] for i in range(S.nos)])
+    np.testing.assert_allclose(s_vo, ref, rtol=0, atol=1e-13)
+
+
 def _skyrmion_minimum(oracle):
     G = GOLD["solvers"]
     cfg = G["cfg"]
