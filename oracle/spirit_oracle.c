@@ -1391,13 +1391,20 @@ static void lbfgs_searchdir( min_state * st, int nos )
     st->local_iter++;
 }
 
-/* sqrt(sum |d_i|^2 / nos) -> maxmove / rms if above maxmove (Solver_Kernels.cpp:95-103,
-   Solver_LBFGS_Atlas.hpp:74-86) */
-static double rms_scaling( const min_state * st, int nos )
+/* Step scaling to the maximum move from the largest per-image rms of the search direction:
+   the min over images of maximum_rotation (Solver_LBFGS_OSO.hpp:59-62, Solver_Kernels.cpp:95-103)
+   and the max-rms rule of Solver_LBFGS_Atlas.hpp:74-86 give the same factor. */
+static double rms_scaling( const min_state * st, int noi, int nos )
 {
-    double ss  = field_dot( st->sd, st->sd, nos, st->dim );
-    double rms = sqrt( ss / nos );
-    return rms > st->maxmove ? st->maxmove / rms : 1.0;
+    double rms_max = 0;
+    for( int img = 0; img < noi; ++img )
+    {
+        const double * d = st->sd + (size_t)img * st->dim * nos;
+        double rms       = sqrt( field_dot( d, d, nos, st->dim ) / nos );
+        if( rms > rms_max )
+            rms_max = rms;
+    }
+    return rms_max > st->maxmove ? st->maxmove / rms_max : 1.0;
 }
 
 /* Solver_Kernels::oso_rotate, Solver_Kernels.cpp:63-93 (one spin) */
@@ -1448,41 +1455,39 @@ static double term_atlas_rho( const struct sum_ctx * c, long i )
     return ( factor * factor - 1 ) * ( a->da[2 * i] * a->dg[2 * i] + a->da[2 * i + 1] * a->dg[2 * i + 1] );
 }
 
-/* One iteration of a direct-minimisation solver; fills s->f1 (force) and s->fv1 (the virtual
-   force Hook_Post_Iteration projects).  Returns the energy of the force evaluation. */
-static double min_iteration( orc_sys * s, int solver, const orc_llg * par, double * spins, min_state * st )
+/* One update of a direct-minimisation solver on noi images of nos spins (contiguous AoS
+   spins), given their forces F (the LLG force -g, or GNEB's F_total).  dt: the images' LLG
+   time step.  gtmp: 3 noi nos doubles of scratch. */
+static void min_update( int solver, double dt, int noi, int nos, double * spins, const double * F, min_state * st,
+                        double * gtmp )
 {
-    const int nos     = s->nos;
-    const double mass = 1.0; /* Method_Solver.hpp:174 */
-    const double dt   = par->dt;
-    double E          = calc_force( s, spins, s->f1 );
-    const double * F  = s->f1;
-    calc_force_virtual( s, par, solver, spins, F, s->fv1 );
+    const int ns      = noi * nos; /* spins of all images */
+    const double mass = 1.0;       /* Method_Solver.hpp:174 */
     if( solver == ORC_SOLVER_VP || solver == ORC_SOLVER_VP_OSO )
     {
         /* VP on the force (VP) or on the OSO gradient's negative (VP_OSO) */
-        double * g = solver == ORC_SOLVER_VP ? s->tmp : st->grad;
+        double * g = solver == ORC_SOLVER_VP ? gtmp : st->grad;
         if( solver == ORC_SOLVER_VP )
-            memcpy( g, F, sizeof( double ) * 3 * (size_t)nos );
+            memcpy( g, F, sizeof( double ) * 3 * (size_t)ns );
         else
         {
-            for( int i = 0; i < nos; ++i )
+            for( int i = 0; i < ns; ++i )
             {
                 oso_gradient( spins + 3 * i, F + 3 * i, g + 3 * i );
                 for( int q = 0; q < 3; ++q )
                     g[3 * i + q] *= -1.0;
             }
         }
-        for( long j = 0; j < 3L * nos; ++j )
+        for( long j = 0; j < 3L * ns; ++j )
             st->v[j] += 0.5 / mass * ( st->prev[j] + g[j] );
-        double proj  = field_dot( st->v, g, nos, 3 );
-        double fn2   = field_dot( g, g, nos, 3 );
+        double proj  = field_dot( st->v, g, ns, 3 );
+        double fn2   = field_dot( g, g, ns, 3 );
         double ratio = proj / fn2;
-        for( long j = 0; j < 3L * nos; ++j )
+        for( long j = 0; j < 3L * ns; ++j )
             st->v[j] = proj <= 0 ? 0.0 : g[j] * ratio;
         if( solver == ORC_SOLVER_VP )
         {
-            for( int i = 0; i < nos; ++i )
+            for( int i = 0; i < ns; ++i )
             {
                 double t[3];
                 for( int q = 0; q < 3; ++q )
@@ -1494,34 +1499,32 @@ static double min_iteration( orc_sys * s, int solver, const orc_llg * par, doubl
         }
         else
         {
-            for( int i = 0; i < nos; ++i )
+            for( int i = 0; i < ns; ++i )
             {
                 for( int q = 0; q < 3; ++q )
                     st->sd[3 * i + q] = dt * st->v[3 * i + q] + 0.5 / mass * dt * g[3 * i + q];
                 oso_rotate_spin( spins + 3 * i, st->sd + 3 * i );
             }
         }
-        memcpy( st->prev, g, sizeof( double ) * 3 * (size_t)nos );
-        return E;
+        memcpy( st->prev, g, sizeof( double ) * 3 * (size_t)ns );
+        return;
     }
     if( solver == ORC_SOLVER_LBFGS_OSO )
     {
-        for( int i = 0; i < nos; ++i )
+        for( int i = 0; i < ns; ++i )
             oso_gradient( spins + 3 * i, F + 3 * i, st->grad + 3 * i );
-        lbfgs_searchdir( st, nos );
-        double scaling = rms_scaling( st, nos );
-        if( scaling > 1 )
-            scaling = 1;
-        for( int i = 0; i < nos; ++i )
+        lbfgs_searchdir( st, ns );
+        double scaling = rms_scaling( st, noi, nos );
+        for( int i = 0; i < ns; ++i )
         {
             for( int q = 0; q < 3; ++q )
                 st->sd[3 * i + q] *= scaling;
             oso_rotate_spin( spins + 3 * i, st->sd + 3 * i );
         }
-        return E;
+        return;
     }
     /* LBFGS_Atlas */
-    for( int i = 0; i < nos; ++i )
+    for( int i = 0; i < ns; ++i )
     {
         /* Solver_Kernels::atlas_calc_gradients, Solver_Kernels.cpp:129-152 */
         const double * sp = spins + 3 * i;
@@ -1536,10 +1539,9 @@ static double min_iteration( orc_sys * s, int solver, const orc_llg * par, doubl
         st->grad[2 * i]     = -( J00 * f[0] + J01 * f[1] + J02 * f[2] );
         st->grad[2 * i + 1] = -( J10 * f[0] + J11 * f[1] + J12 * f[2] );
     }
-    lbfgs_searchdir( st, nos );
-    double scaling = rms_scaling( st, nos );
-    int any_flip   = 0;
-    for( int i = 0; i < nos; ++i )
+    lbfgs_searchdir( st, ns );
+    double scaling = rms_scaling( st, noi, nos );
+    for( int i = 0; i < ns; ++i )
     {
         double * d = st->sd + 2 * i;
         d[0] *= scaling;
@@ -1557,10 +1559,17 @@ static double min_iteration( orc_sys * s, int solver, const orc_llg * par, doubl
         sp[0] *= r;
         sp[1] *= r;
         sp[2] *= r;
-        /* ncg_atlas_check_coordinates, Solver_Kernels.cpp:154-180 (tol -0.6) */
-        if( sp[2] * a3 < -0.6 )
-            any_flip = 1;
     }
+    /* ncg_atlas_check_coordinates, Solver_Kernels.cpp:154-180 (tol -0.6); the reference
+       compares the spins of image 0 with every image's charts */
+    int any_flip = 0;
+    for( int img = 0; img < noi && !any_flip; ++img )
+        for( int i = 0; i < nos; ++i )
+            if( spins[3 * i + 2] * st->a3[(size_t)img * nos + i] < -0.6 )
+            {
+                any_flip = 1;
+                break;
+            }
     if( any_flip )
     {
         /* lbfgs_atlas_transform_direction, Solver_Kernels.cpp:182-242.  The reference adds the
@@ -1569,12 +1578,12 @@ static double min_iteration( orc_sys * s, int solver, const orc_llg * par, doubl
         for( int n = 0; n < ORC_LBFGS_MEMORY; ++n )
         {
             struct atlas_ctx a = { spins, st->a3, st->da[n], st->dg[n] };
-            double add         = det_sum( nos, term_atlas_rho, &( struct sum_ctx ){ s, NULL, NULL, (const double *)&a } );
+            double add         = det_sum( ns, term_atlas_rho, &( struct sum_ctx ){ NULL, NULL, NULL, (const double *)&a } );
             st->rho[n]         = 1 / st->rho[n];
             st->rho[n]         = st->rho[n] + add;
             st->rho[n]         = 1 / st->rho[n];
         }
-        for( int i = 0; i < nos; ++i )
+        for( int i = 0; i < ns; ++i )
         {
             const double sz = spins[3 * i + 2];
             if( sz * st->a3[i] < 0 )
@@ -1595,6 +1604,15 @@ static double min_iteration( orc_sys * s, int solver, const orc_llg * par, doubl
             }
         }
     }
+}
+
+/* One LLG iteration of a direct-minimisation solver; fills s->f1 (force) and s->fv1 (the
+   virtual force Hook_Post_Iteration projects).  Returns the energy of the force evaluation. */
+static double min_iteration( orc_sys * s, int solver, const orc_llg * par, double * spins, min_state * st )
+{
+    double E = calc_force( s, spins, s->f1 );
+    calc_force_virtual( s, par, solver, spins, s->f1, s->fv1 );
+    min_update( solver, par->dt, 1, s->nos, spins, s->f1, st, s->tmp );
     return E;
 }
 
@@ -2283,6 +2301,12 @@ long orc_gneb_run( orc_sys * s, int solver, const orc_gneb * par, int noi, const
     double * proj = (double *)malloc( sizeof( double ) * (size_t)noi );
     double * fn2  = (double *)malloc( sizeof( double ) * (size_t)noi );
     const double m = 1.0;
+    /* VP_OSO, LBFGS_OSO, LBFGS_Atlas: solver state over the whole chain */
+    const int oso_atlas = solver == ORC_SOLVER_VP_OSO || solver == ORC_SOLVER_LBFGS_OSO
+                          || solver == ORC_SOLVER_LBFGS_ATLAS;
+    min_state * mst = oso_atlas ? min_state_create( solver, noi * s->nos, spins ) : NULL;
+    double * Fc     = oso_atlas ? (double *)calloc( (size_t)noi * n3, sizeof( double ) ) : NULL;
+    double * gtmp   = oso_atlas ? (double *)calloc( (size_t)noi * n3, sizeof( double ) ) : NULL;
 
     double max_torque = par->force_convergence + 1.0;
     long iteration    = 0;
@@ -2331,6 +2355,17 @@ long orc_gneb_run( orc_sys * s, int solver, const orc_gneb * par, int noi, const
                         conf[i][t + q] = c[q];
                 }
             }
+        }
+        else if( oso_atlas )
+        {
+            /* Solver_VP_OSO.hpp:33-110, Solver_LBFGS_OSO.hpp:37-72, Solver_LBFGS_Atlas.hpp:44-101
+               on the chain's forces F_total (Method_GNEB.cpp:88-357) */
+            gneb_calc_force( s, par, image_type, &w, conf, conf, forces );
+            if( w.stop )
+                break;
+            for( int i = 0; i < noi; ++i )
+                memcpy( Fc + (size_t)i * n3, forces[i], sizeof( double ) * n3 );
+            min_update( solver, par->dt, noi, s->nos, spins, Fc, mst, gtmp );
         }
         else if( solver == ORC_SOLVER_HEUN || solver == ORC_SOLVER_DEPONDT || solver == ORC_SOLVER_SIB )
         {
@@ -2509,6 +2544,9 @@ long orc_gneb_run( orc_sys * s, int solver, const orc_gneb * par, int noi, const
     free( w.Rx );
     free( w.E );
     free( conf );
+    min_state_destroy( mst );
+    free( Fc );
+    free( gtmp );
     free_fields( forces, noi );
     free_fields( fv, noi );
     free_fields( pred, noi );

@@ -178,7 +178,7 @@ typedef struct orc_gneb
     double force_convergence;
 } orc_gneb;
 
-/* Method_GNEB with solver VP / SIB / Depondt / Heun / RK4 (Method_GNEB.cpp:88-456,
+/* Method_GNEB with solver VP / SIB / Depondt / Heun / RK4 / VP_OSO / LBFGS_OSO / LBFGS_Atlas (Method_GNEB.cpp:88-456,
    Solver_*.hpp, Method_Solver.hpp:224-239, Method.cpp:57-113 with n_iterations_amortize = 1).
    spins: noi images of nos spins (AoS), advanced in place.  image_type: 0 normal,
    1 climbing, 2 falling, 3 stationary (Spin_System_Chain.hpp:13-19).  A fresh solver state

@@ -193,8 +193,8 @@ try
         m->solver           = solver_type;
         m->n_iterations     = std::max( 1L, gp.n_iterations );
         m->n_iterations_log = gp.n_iterations_log;
-        const char * names[] = { "VP", "SIB", "Depondt", "Heun", "RK4" };
-        m->solver_name       = ( solver_type >= 0 && solver_type <= 4 ) ? names[solver_type] : "unknown";
+        const char * names[] = { "VP", "SIB", "Depondt", "Heun", "RK4", "LBFGS_OSO", "LBFGS_Atlas", "VP_OSO" };
+        m->solver_name       = ( solver_type >= 0 && solver_type <= 7 ) ? names[solver_type] : "unknown";
         try
         {
             chain->image_type.resize( chain->noi(), GNEB_NORMAL );

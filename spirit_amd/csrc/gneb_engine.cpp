@@ -1,5 +1,5 @@
-// GNEB chain runner: the iteration sequence of Method_GNEB with the VP, SIB, Depondt, Heun
-// and RK4 solvers, enqueued on the device (core/src/engine/Method_GNEB.cpp,
+// GNEB chain runner: the iteration sequence of Method_GNEB with the VP, SIB, Depondt, Heun,
+// RK4, VP_OSO, LBFGS_OSO and LBFGS_Atlas solvers, enqueued on the device (core/src/engine/Method_GNEB.cpp,
 // core/include/engine/Solver_*.hpp, core/src/engine/Method.cpp:57-113).
 #include "gpu_engine.hpp"
 
@@ -13,7 +13,8 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
         : specs( images ), solver( solver_ )
 {
     if( solver != SOLVER_VP && solver != SOLVER_SIB && solver != SOLVER_DEPONDT && solver != SOLVER_HEUN
-        && solver != SOLVER_RK4 )
+        && solver != SOLVER_RK4 && solver != SOLVER_VP_OSO && solver != SOLVER_LBFGS_OSO
+        && solver != SOLVER_LBFGS_ATLAS )
         throw std::runtime_error( "GNEB solver " + std::to_string( solver ) + " is not available in the HIP engine" );
     noi = (int)images.size();
     if( noi < 2 )
@@ -73,7 +74,66 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
     A.iter     = b_iter.as<long long>();
     set_types( types );
     set_parameters( par );
+    if( min_is_solver( solver ) && solver != SOLVER_VP )
+        setup_minimiser();
     HIPCHECK( hipStreamSynchronize( st ) );
+}
+
+// State of VP_OSO / LBFGS_OSO / LBFGS_Atlas over the chain, zero as after the solver's
+// Initialize; the atlas charts from the signs of the images' initial s_z
+// (Solver_LBFGS_Atlas.hpp:34-41).
+void GNEBRunner::setup_minimiser()
+{
+    hipStream_t st   = field->stream;
+    const size_t ns  = (size_t)noi * nos;
+    const int dim    = solver == SOLVER_LBFGS_ATLAS ? 2 : 3;
+    auto zeroed      = [&]( DeviceBuffer & b, size_t bytes ) {
+        b.resize( bytes );
+        HIPCHECK( hipMemsetAsync( b.as<void>(), 0, bytes, st ) );
+    };
+    const size_t f3 = sizeof( double ) * 3 * ns, fd = sizeof( double ) * dim * ns;
+    zeroed( b_mv, f3 );
+    zeroed( b_mprev, f3 );
+    zeroed( b_mgrad, fd );
+    zeroed( b_mgpr, fd );
+    zeroed( b_msd, fd );
+    zeroed( b_mq, fd );
+    for( int k = 0; k < MIN_MEMORY; ++k )
+    {
+        zeroed( b_mda[k], fd );
+        zeroed( b_mdg[k], fd );
+    }
+    zeroed( b_mpart, sizeof( double ) * MIN_SLOTS * (size_t)noi * bpi );
+    zeroed( b_msc, sizeof( MinScalars ) );
+    std::vector<double> a3( ns, 1.0 );
+    for( int k = 0; k < noi; ++k )
+        for( int i = 0; i < nos; ++i )
+            a3[(size_t)k * nos + i] = specs[k].spins_aos[3 * (size_t)i + 2] > 0 ? 1.0 : -1.0;
+    b_ma3.resize( sizeof( double ) * ns );
+    HIPCHECK( hipMemcpyAsync( b_ma3.as<void>(), a3.data(), sizeof( double ) * ns, hipMemcpyHostToDevice, st ) );
+    HIPCHECK( hipStreamSynchronize( st ) );
+    MinArgs & M = min_args;
+    M.v         = b_mv.as<double>();
+    M.prev      = b_mprev.as<double>();
+    M.f         = nullptr;
+    M.grad      = b_mgrad.as<double>();
+    M.grad_pr   = b_mgpr.as<double>();
+    M.sd        = b_msd.as<double>();
+    M.q         = b_mq.as<double>();
+    for( int k = 0; k < MIN_MEMORY; ++k )
+    {
+        M.da[k] = b_mda[k].as<double>();
+        M.dg[k] = b_mdg[k].as<double>();
+    }
+    M.a3       = b_ma3.as<double>();
+    M.part     = b_mpart.as<double>();
+    M.F_ext    = b_Ft.as<double>();
+    M.sc       = b_msc.as<MinScalars>();
+    M.n_blocks = noi * bpi;
+    M.noi      = noi;
+    M.bpi      = bpi;
+    M.dim      = dim;
+    M.maxmove  = solver == SOLVER_LBFGS_ATLAS ? 0.05 : Pi / 200.0; // Solver_LBFGS_*.hpp Initialize
 }
 
 GNEBRunner::~GNEBRunner()
@@ -146,6 +206,23 @@ void GNEBRunner::iteration()
         gneb_launch_chain( GPH_VP, A, st );
         A.x = xs;
         gneb_launch_pass( GPASS_VP_UPDATE, A, st );
+        gneb_launch_chain( GPH_HOOK, A, st );
+        return;
+    }
+    if( min_is_solver( solver ) )
+    {
+        // VP_OSO, LBFGS_OSO, LBFGS_Atlas on F_total (Solver_VP_OSO.hpp:33-110,
+        // Solver_LBFGS_OSO.hpp:37-72, Solver_LBFGS_Atlas.hpp:44-101)
+        A.fv = b_fv.as<double>();
+        calc_force( xs, false );
+        StepArgs SA{};
+        SA.imgs     = A.imgs;
+        SA.s        = xs;
+        SA.stop     = A.stop;
+        SA.img_stop = A.stop;
+        min_launch_chain_update( solver, SA, min_args, nos, st );
+        A.x = xs;
+        gneb_launch_pass( GPASS_HOOK, A, st );
         gneb_launch_chain( GPH_HOOK, A, st );
         return;
     }

@@ -616,8 +616,11 @@ void LLGRunner::setup_minimiser()
     }
     M.a3       = b_ma3.as<double>();
     M.part     = b_mpart.as<double>();
+    M.F_ext    = nullptr;
     M.sc       = b_msc.as<MinScalars>();
     M.n_blocks = n_blocks;
+    M.noi      = 1;
+    M.bpi      = n_blocks;
     M.dim      = dim;
     M.maxmove  = solver == SOLVER_LBFGS_ATLAS ? 0.05 : Pi / 200.0; // Solver_LBFGS_*.hpp Initialize
 }

@@ -37,9 +37,28 @@ constexpr double LBFGS_EPS = 1e-300; // Solver_Kernels.hpp:59 (double)
 constexpr double MIN_MASS  = 1.0;    // Method_Solver.hpp:174
 constexpr int SC_THREADS   = 1024;
 
-__device__ __forceinline__ bool min_stopped( const StepArgs & A )
+// LLG runs stop on the run-wide flag or image 0's; chains (forces given) on the chain's flag
+__device__ __forceinline__ bool min_stopped( const StepArgs & A, const MinArgs & M )
 {
-    return *A.stop || A.img_stop[0];
+    return *A.stop || ( !M.F_ext && A.img_stop[0] );
+}
+
+// the spin of this thread: image img (blocks img*bpi ..), spin i of the image, and the
+// offsets of the image's 3-vector, dim-vector and scalar fields
+struct MinSite
+{
+    int img, i;
+    long long b3, bd, b1;
+};
+__device__ __forceinline__ MinSite min_site( int bpi, int dim, int nos )
+{
+    MinSite m;
+    m.img = blockIdx.x / bpi;
+    m.i   = ( blockIdx.x - m.img * bpi ) * LLG_THREADS + threadIdx.x;
+    m.b3  = 3LL * m.img * nos;
+    m.bd  = (long long)dim * m.img * nos;
+    m.b1  = (long long)m.img * nos;
+    return m;
 }
 
 // first loop of the two-loop recursion, k = M-1 .. 0
@@ -163,27 +182,34 @@ __device__ __forceinline__ void atlas_rotate( const double * sp, double a3, cons
 
 // ---------------------------------------------------------------------------
 // force evaluation: energy partials, force, solver gradient, velocity (VP) or the newest
-// L-BFGS pair and its dot products
+// L-BFGS pair and its dot products.  Chains take their forces from M.F_ext.
 // ---------------------------------------------------------------------------
 template<int SOLVER>
 __global__ void __launch_bounds__( LLG_THREADS ) k_min_force( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) )
+    if( min_stopped( A, M ) )
         return;
-    const ImageDev * __restrict__ I = A.imgs;
-    const HamDev & H = I->ham;
-    const int i      = blockIdx.x * LLG_THREADS + threadIdx.x;
-    const int li     = M.sc->local_iter;
-    double x[4]      = { 0, 0, 0, 0 }; // solver sums, then the energy
+    constexpr int D = SOLVER == SOLVER_LBFGS_ATLAS ? 2 : 3;
+    const MinSite t = min_site( M.bpi, D, nos );
+    const int i     = t.i;
+    const int li    = M.sc->local_iter;
+    double x[4]     = { 0, 0, 0, 0 }; // solver sums, then the energy
     if( i < nos )
     {
-        double g[3], e;
-        local_gradient( H, A.s, nos, i, ( A.h && H.has_ddi ) ? A.h : nullptr, g, e );
-        x[3]              = e;
-        const double F[3] = { -1 * g[0], -1 * g[1], -1 * g[2] };
-        store3( M.f, nos, i, F );
+        double F[3];
+        if( M.F_ext )
+            load3( M.F_ext + t.b3, nos, i, F );
+        else
+        {
+            const HamDev & H = A.imgs->ham;
+            double g[3], e;
+            local_gradient( H, A.s, nos, i, ( A.h && H.has_ddi ) ? A.h : nullptr, g, e );
+            x[3] = e;
+            F[0] = -1 * g[0], F[1] = -1 * g[1], F[2] = -1 * g[2];
+            store3( M.f, nos, i, F );
+        }
         double s[3];
-        load3( A.s, nos, i, s );
+        load3( A.s + t.b3, nos, i, s );
         if constexpr( SOLVER == SOLVER_VP || SOLVER == SOLVER_VP_OSO )
         {
             double gv[3];
@@ -198,37 +224,37 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_min_force( StepArgs A, MinArg
                     gv[q] = og[q] * -1.0;
             }
             double pv[3], v[3];
-            load3( M.prev, nos, i, pv );
-            load3( M.v, nos, i, v );
+            load3( M.prev + t.b3, nos, i, pv );
+            load3( M.v + t.b3, nos, i, v );
 #pragma unroll
             for( int q = 0; q < 3; ++q )
                 v[q] += 0.5 / MIN_MASS * ( pv[q] + gv[q] );
-            store3( M.v, nos, i, v );
-            store3( M.prev, nos, i, gv );
+            store3( M.v + t.b3, nos, i, v );
+            store3( M.prev + t.b3, nos, i, gv );
             x[0] = dot3( v, gv );
             x[1] = dot3( gv, gv );
         }
         else
         {
-            constexpr int D = SOLVER == SOLVER_LBFGS_ATLAS ? 2 : 3;
             double gr[3];
             if constexpr( SOLVER == SOLVER_LBFGS_OSO )
                 oso_gradient( s, F, gr );
             else
-                atlas_gradient( s, F, M.a3[i], gr );
+                atlas_gradient( s, F, M.a3[t.b1 + i], gr );
 #pragma unroll
             for( int d = 0; d < D; ++d )
-                M.grad[d * nos + i] = gr[d];
+                M.grad[t.bd + d * nos + i] = gr[d];
             if( li > 0 )
             {
                 const int m = li % MIN_MEMORY;
 #pragma unroll
                 for( int d = 0; d < D; ++d )
                 {
-                    const double a   = M.sd[d * nos + i];
-                    const double dgv = gr[d] - M.grad_pr[d * nos + i];
-                    M.da[m][d * nos + i] = a;
-                    M.dg[m][d * nos + i] = dgv;
+                    const long long k = t.bd + d * nos + i;
+                    const double a    = M.sd[k];
+                    const double dgv  = gr[d] - M.grad_pr[k];
+                    M.da[m][k]        = a;
+                    M.dg[m][k]        = dgv;
                     x[0] += dgv * a;
                     x[1] += dgv * dgv;
                     x[2] += a * gr[d];
@@ -241,7 +267,8 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_min_force( StepArgs A, MinArg
     {
         for( int s = 0; s < 3; ++s )
             M.part[s * M.n_blocks + blockIdx.x] = x[s];
-        A.part_E[blockIdx.x] = x[3];
+        if( !M.F_ext )
+            A.part_E[blockIdx.x] = x[3];
     }
 }
 
@@ -250,24 +277,47 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_min_force( StepArgs A, MinArg
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__( SC_THREADS ) k_min_scalar( StepArgs A, MinArgs M, int phase, int round, int nos )
 {
-    if( min_stopped( A ) )
+    if( min_stopped( A, M ) )
         return;
     __shared__ double sh[3][SC_THREADS];
     const int ns = ( phase == PH_RHO || phase == PH_FLIPRHO ) ? 3 : ( phase == PH_VP ? 2 : 1 );
-    for( int s = 0; s < ns; ++s )
+    if( phase == PH_SCALE )
     {
-        double t = 0;
-        for( int k = threadIdx.x; k < M.n_blocks; k += blockDim.x )
-            t += M.part[s * M.n_blocks + k];
-        sh[s][threadIdx.x] = t;
-    }
-    __syncthreads();
-    for( int w = blockDim.x / 2; w > 0; w >>= 1 )
-    {
-        if( (int)threadIdx.x < w )
-            for( int s = 0; s < ns; ++s )
-                sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+        // rms of the search direction of each image (its blocks in order), then the largest
+        double r = 0;
+        if( (int)threadIdx.x < M.noi )
+        {
+            double ss = 0;
+            for( int k = 0; k < M.bpi; ++k )
+                ss += M.part[threadIdx.x * M.bpi + k];
+            r = sqrt( ss / nos );
+        }
+        sh[0][threadIdx.x] = r;
         __syncthreads();
+        for( int w = blockDim.x / 2; w > 0; w >>= 1 )
+        {
+            if( (int)threadIdx.x < w )
+                sh[0][threadIdx.x] = fmax( sh[0][threadIdx.x], sh[0][threadIdx.x + w] );
+            __syncthreads();
+        }
+    }
+    else
+    {
+        for( int s = 0; s < ns; ++s )
+        {
+            double t = 0;
+            for( int k = threadIdx.x; k < M.n_blocks; k += blockDim.x )
+                t += M.part[s * M.n_blocks + k];
+            sh[s][threadIdx.x] = t;
+        }
+        __syncthreads();
+        for( int w = blockDim.x / 2; w > 0; w >>= 1 )
+        {
+            if( (int)threadIdx.x < w )
+                for( int s = 0; s < ns; ++s )
+                    sh[s][threadIdx.x] += sh[s][threadIdx.x + w];
+            __syncthreads();
+        }
     }
     if( threadIdx.x != 0 )
         return;
@@ -322,12 +372,10 @@ __global__ void __launch_bounds__( SC_THREADS ) k_min_scalar( StepArgs A, MinArg
             }
             break;
         case PH_SCALE:
-        {
-            const double rms = sqrt( t0 / nos );
-            S.scaling        = rms > M.maxmove ? M.maxmove / rms : 1.0;
-            S.local_iter     = li + 1;
+            // t0: the largest rms over the images
+            S.scaling    = t0 > M.maxmove ? M.maxmove / t0 : 1.0;
+            S.local_iter = li + 1;
             break;
-        }
         case PH_FLIP: S.flip = t0 > 0 ? 1 : 0; break;
         case PH_FLIPRHO:
             if( S.flip )
@@ -349,23 +397,24 @@ __global__ void __launch_bounds__( SC_THREADS ) k_min_scalar( StepArgs A, MinArg
 template<int D>
 __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_loop1( StepArgs A, MinArgs M, int nos, int k )
 {
-    if( min_stopped( A ) || M.sc->gd )
+    if( min_stopped( A, M ) || M.sc->gd )
         return;
     const int li       = M.sc->local_iter;
     const int cp       = lbfgs_c1( li, k + 1 ), c = lbfgs_c1( li, k );
     const double ap    = M.sc->alpha[cp];
     const double * src = k == MIN_MEMORY - 2 ? M.grad : M.q;
-    const int i        = blockIdx.x * LLG_THREADS + threadIdx.x;
+    const MinSite t    = min_site( M.bpi, D, nos );
     double x[1]        = { 0 };
-    if( i < nos )
+    if( t.i < nos )
     {
 #pragma unroll
         for( int d = 0; d < D; ++d )
         {
-            double qv = src[d * nos + i];
-            qv += -ap * M.dg[cp][d * nos + i];
-            M.q[d * nos + i] = qv;
-            x[0] += M.da[c][d * nos + i] * qv;
+            const long long j = t.bd + d * nos + t.i;
+            double qv         = src[j];
+            qv += -ap * M.dg[cp][j];
+            M.q[j] = qv;
+            x[0] += M.da[c][j] * qv;
         }
     }
     min_block_sums<1>( x );
@@ -377,24 +426,25 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_loop1( StepArgs A, MinA
 template<int D>
 __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_mid( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) || M.sc->gd )
+    if( min_stopped( A, M ) || M.sc->gd )
         return;
     const int li     = M.sc->local_iter;
     const int cp     = lbfgs_c1( li, 0 ), c = lbfgs_c2( li, 0 );
     const double ap  = M.sc->alpha[cp];
     const double inv = M.sc->inv_rhody2;
-    const int i      = blockIdx.x * LLG_THREADS + threadIdx.x;
+    const MinSite t  = min_site( M.bpi, D, nos );
     double x[1]      = { 0 };
-    if( i < nos )
+    if( t.i < nos )
     {
 #pragma unroll
         for( int d = 0; d < D; ++d )
         {
-            double qv = M.q[d * nos + i];
-            qv += -ap * M.dg[cp][d * nos + i];
-            const double sdv  = inv * qv;
-            M.sd[d * nos + i] = sdv;
-            x[0] += M.dg[c][d * nos + i] * sdv;
+            const long long j = t.bd + d * nos + t.i;
+            double qv         = M.q[j];
+            qv += -ap * M.dg[cp][j];
+            const double sdv = inv * qv;
+            M.sd[j]          = sdv;
+            x[0] += M.dg[c][j] * sdv;
         }
     }
     min_block_sums<1>( x );
@@ -406,22 +456,23 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_mid( StepArgs A, MinArg
 template<int D>
 __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_loop2( StepArgs A, MinArgs M, int nos, int k )
 {
-    if( min_stopped( A ) || M.sc->gd )
+    if( min_stopped( A, M ) || M.sc->gd )
         return;
     const int li      = M.sc->local_iter;
     const int cp      = lbfgs_c2( li, k - 1 ), c = lbfgs_c2( li, k );
     const double coef = M.sc->coef;
-    const int i       = blockIdx.x * LLG_THREADS + threadIdx.x;
+    const MinSite t   = min_site( M.bpi, D, nos );
     double x[1]       = { 0 };
-    if( i < nos )
+    if( t.i < nos )
     {
 #pragma unroll
         for( int d = 0; d < D; ++d )
         {
-            double sdv = M.sd[d * nos + i];
-            sdv += coef * M.da[cp][d * nos + i];
-            M.sd[d * nos + i] = sdv;
-            x[0] += M.dg[c][d * nos + i] * sdv;
+            const long long j = t.bd + d * nos + t.i;
+            double sdv        = M.sd[j];
+            sdv += coef * M.da[cp][j];
+            M.sd[j] = sdv;
+            x[0] += M.dg[c][j] * sdv;
         }
     }
     min_block_sums<1>( x );
@@ -434,21 +485,22 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_loop2( StepArgs A, MinA
 template<int D>
 __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_final( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) )
+    if( min_stopped( A, M ) )
         return;
     const bool gd     = M.sc->gd;
     const int li      = M.sc->local_iter;
     const int cp      = lbfgs_c2( li, MIN_MEMORY - 1 );
     const double coef = M.sc->coef;
-    const int i       = blockIdx.x * LLG_THREADS + threadIdx.x;
+    const MinSite t   = min_site( M.bpi, D, nos );
     double x[1]       = { 0 };
-    if( i < nos )
+    if( t.i < nos )
     {
         double sq = 0;
 #pragma unroll
         for( int d = 0; d < D; ++d )
         {
-            const double g = M.grad[d * nos + i];
+            const long long j = t.bd + d * nos + t.i;
+            const double g    = M.grad[j];
             double sdv;
             if( gd )
             {
@@ -456,18 +508,18 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_final( StepArgs A, MinA
 #pragma unroll
                 for( int n = 0; n < MIN_MEMORY; ++n )
                 {
-                    M.da[n][d * nos + i] = 0.0;
-                    M.dg[n][d * nos + i] = 0.0;
+                    M.da[n][j] = 0.0;
+                    M.dg[n][j] = 0.0;
                 }
             }
             else
             {
-                sdv = M.sd[d * nos + i];
-                sdv += coef * M.da[cp][d * nos + i];
+                sdv = M.sd[j];
+                sdv += coef * M.da[cp][j];
                 sdv = -sdv;
             }
-            M.grad_pr[d * nos + i] = g;
-            M.sd[d * nos + i]      = sdv;
+            M.grad_pr[j] = g;
+            M.sd[j]      = sdv;
             sq += sdv * sdv;
         }
         x[0] = sq;
@@ -478,31 +530,32 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_lbfgs_final( StepArgs A, MinA
 }
 
 // ---------------------------------------------------------------------------
-// the update of the spins and Hook_Post_Iteration's torque / effective field
+// the update of the spins; for LLG runs Hook_Post_Iteration's torque / effective field
 // ---------------------------------------------------------------------------
 template<int SOLVER>
 __global__ void __launch_bounds__( LLG_THREADS ) k_min_update( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) )
+    if( min_stopped( A, M ) )
         return;
-    const ImageDev * __restrict__ I = A.imgs;
+    constexpr int D = SOLVER == SOLVER_LBFGS_ATLAS ? 2 : 3;
+    const MinSite t = min_site( M.bpi, D, nos );
+    const ImageDev * __restrict__ I   = A.imgs + t.img;
     const MinScalars * __restrict__ S = M.sc;
-    const int i = blockIdx.x * LLG_THREADS + threadIdx.x;
-    double t_i = 0, flag = 0;
+    const int i = t.i;
+    double t_i  = 0;
     if( i < nos )
     {
-        double s[3], F[3], sn[3];
-        load3( A.s, nos, i, s );
-        load3( M.f, nos, i, F );
+        double s[3], sn[3];
+        load3( A.s + t.b3, nos, i, s );
         const double dt = I->llg.dt;
         if constexpr( SOLVER == SOLVER_VP || SOLVER == SOLVER_VP_OSO )
         {
             double gv[3], v[3];
-            load3( M.prev, nos, i, gv );
+            load3( M.prev + t.b3, nos, i, gv );
 #pragma unroll
             for( int q = 0; q < 3; ++q )
                 v[q] = S->vp_zero ? 0.0 : gv[q] * S->ratio;
-            store3( M.v, nos, i, v );
+            store3( M.v + t.b3, nos, i, v );
             if constexpr( SOLVER == SOLVER_VP )
             {
 #pragma unroll
@@ -522,69 +575,79 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_min_update( StepArgs A, MinAr
         else if constexpr( SOLVER == SOLVER_LBFGS_OSO )
         {
             double sd[3];
-            load3( M.sd, nos, i, sd );
+            load3( M.sd + t.bd, nos, i, sd );
 #pragma unroll
             for( int q = 0; q < 3; ++q )
                 sd[q] *= S->scaling;
-            store3( M.sd, nos, i, sd );
+            store3( M.sd + t.bd, nos, i, sd );
             oso_rotate( s, sd, sn );
         }
         else
         {
-            double d[2] = { M.sd[i], M.sd[nos + i] };
+            double d[2] = { M.sd[t.bd + i], M.sd[t.bd + nos + i] };
             d[0] *= S->scaling;
             d[1] *= S->scaling;
-            M.sd[i]       = d[0];
-            M.sd[nos + i] = d[1];
-            const double a3 = M.a3[i];
-            atlas_rotate( s, a3, d, sn );
-            flag = sn[2] * a3 < -0.6 ? 1.0 : 0.0; // ncg_atlas_check_coordinates, tol -0.6
+            M.sd[t.bd + i]       = d[0];
+            M.sd[t.bd + nos + i] = d[1];
+            atlas_rotate( s, M.a3[t.b1 + i], d, sn );
         }
-        store3( A.s, nos, i, sn );
-        // Hook_Post_Iteration: torque of the virtual force (s x F at the old spins; scaled by
-        // dt gamma / mu_B except for the L-BFGS solvers) projected on the new spins
-        constexpr bool LBFGS = SOLVER == SOLVER_LBFGS_OSO || SOLVER == SOLVER_LBFGS_ATLAS;
-        const double c0      = LBFGS ? 1.0 : dt * gamma_e / mu_B;
-        double c[3], fv[3], pr[3];
-        cross3( s, F, c );
-#pragma unroll
-        for( int q = 0; q < 3; ++q )
-            fv[q] = c0 * c[q];
-        const double d = dot3( fv, sn );
-#pragma unroll
-        for( int q = 0; q < 3; ++q )
-            pr[q] = fv[q] - d * sn[q];
-        t_i = dot3( pr, pr );
-        if( A.obs )
+        store3( A.s + t.b3, nos, i, sn );
+        if( !M.F_ext )
         {
-            const double d1 = dot3( F, sn );
+            // Hook_Post_Iteration: torque of the virtual force (s x F at the old spins; scaled by
+            // dt gamma / mu_B except for the L-BFGS solvers) projected on the new spins
+            double F[3];
+            load3( M.f, nos, i, F );
+            constexpr bool LBFGS = SOLVER == SOLVER_LBFGS_OSO || SOLVER == SOLVER_LBFGS_ATLAS;
+            const double c0      = LBFGS ? 1.0 : dt * gamma_e / mu_B;
+            double c[3], fv[3], pr[3];
+            cross3( s, F, c );
 #pragma unroll
             for( int q = 0; q < 3; ++q )
-                pr[q] = F[q] - d1 * sn[q];
-            store3( A.eff, nos, i, pr );
+                fv[q] = c0 * c[q];
+            const double dd = dot3( fv, sn );
+#pragma unroll
+            for( int q = 0; q < 3; ++q )
+                pr[q] = fv[q] - dd * sn[q];
+            t_i = dot3( pr, pr );
+            if( A.obs )
+            {
+                const double d1 = dot3( F, sn );
+#pragma unroll
+                for( int q = 0; q < 3; ++q )
+                    pr[q] = F[q] - d1 * sn[q];
+                store3( A.eff, nos, i, pr );
+            }
         }
     }
-    const double tmax = min_block_max( t_i );
-    if( threadIdx.x == 0 )
-        A.part_T[blockIdx.x] = tmax;
-    if constexpr( SOLVER == SOLVER_LBFGS_ATLAS )
+    if( !M.F_ext )
     {
-        __syncthreads();
-        double x[1] = { flag };
-        min_block_sums<1>( x );
+        const double tmax = min_block_max( t_i );
         if( threadIdx.x == 0 )
-            M.part[blockIdx.x] = x[0];
+            A.part_T[blockIdx.x] = tmax;
     }
 }
 
 // ---------------------------------------------------------------------------
-// atlas chart changes (lbfgs_atlas_transform_direction)
+// atlas chart changes (ncg_atlas_check_coordinates, lbfgs_atlas_transform_direction)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool atlas_leaves_chart( const MinArgs & M, const double * s, int nos, int i, double & a3n,
-                                                    double & factor )
+// spins of image 0 against each image's charts, as the reference's check reads them
+__global__ void __launch_bounds__( LLG_THREADS ) k_atlas_check( StepArgs A, MinArgs M, int nos )
 {
-    const double sz = s[2 * nos + i];
-    if( !( sz * M.a3[i] < 0 ) )
+    if( min_stopped( A, M ) )
+        return;
+    const MinSite t = min_site( M.bpi, 2, nos );
+    double x[1]     = { 0 };
+    if( t.i < nos )
+        x[0] = A.s[2 * nos + t.i] * M.a3[t.b1 + t.i] < -0.6 ? 1.0 : 0.0;
+    min_block_sums<1>( x );
+    if( threadIdx.x == 0 )
+        M.part[blockIdx.x] = x[0];
+}
+
+__device__ __forceinline__ bool atlas_leaves_chart( double sz, double a3, double & a3n, double & factor )
+{
+    if( !( sz * a3 < 0 ) )
         return false;
     a3n    = sz > 0 ? 1 : -1;
     factor = ( 1 - a3n * sz ) / ( 1 + a3n * sz );
@@ -595,17 +658,17 @@ __device__ __forceinline__ bool atlas_leaves_chart( const MinArgs & M, const dou
 // adds them into rho_n from parallel threads; here they are summed in a fixed order)
 __global__ void __launch_bounds__( LLG_THREADS ) k_atlas_flip_sums( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) || !M.sc->flip )
+    if( min_stopped( A, M ) || !M.sc->flip )
         return;
-    const int i = blockIdx.x * LLG_THREADS + threadIdx.x;
-    double x[3] = { 0, 0, 0 };
+    const MinSite t = min_site( M.bpi, 2, nos );
+    double x[3]     = { 0, 0, 0 };
     double a3n, factor;
-    if( i < nos && atlas_leaves_chart( M, A.s, nos, i, a3n, factor ) )
+    if( t.i < nos && atlas_leaves_chart( A.s[t.b3 + 2 * nos + t.i], M.a3[t.b1 + t.i], a3n, factor ) )
     {
+        const long long j0 = t.bd + t.i, j1 = t.bd + nos + t.i;
 #pragma unroll
         for( int n = 0; n < MIN_MEMORY; ++n )
-            x[n] = ( factor * factor - 1 )
-                   * ( M.da[n][i] * M.dg[n][i] + M.da[n][nos + i] * M.dg[n][nos + i] );
+            x[n] = ( factor * factor - 1 ) * ( M.da[n][j0] * M.dg[n][j0] + M.da[n][j1] * M.dg[n][j1] );
     }
     min_block_sums<3>( x );
     if( threadIdx.x == 0 )
@@ -615,22 +678,23 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_atlas_flip_sums( StepArgs A, 
 
 __global__ void __launch_bounds__( LLG_THREADS ) k_atlas_flip_apply( StepArgs A, MinArgs M, int nos )
 {
-    if( min_stopped( A ) || !M.sc->flip )
+    if( min_stopped( A, M ) || !M.sc->flip )
         return;
-    const int i = blockIdx.x * LLG_THREADS + threadIdx.x;
+    const MinSite t = min_site( M.bpi, 2, nos );
     double a3n, factor;
-    if( i < nos && atlas_leaves_chart( M, A.s, nos, i, a3n, factor ) )
+    if( t.i < nos && atlas_leaves_chart( A.s[t.b3 + 2 * nos + t.i], M.a3[t.b1 + t.i], a3n, factor ) )
     {
-        M.a3[i] = a3n;
+        M.a3[t.b1 + t.i] = a3n;
         for( int d = 0; d < 2; ++d )
         {
-            M.sd[d * nos + i] *= factor;
-            M.grad_pr[d * nos + i] *= factor;
+            const long long j = t.bd + d * nos + t.i;
+            M.sd[j] *= factor;
+            M.grad_pr[j] *= factor;
 #pragma unroll
             for( int n = 0; n < MIN_MEMORY; ++n )
             {
-                M.da[n][d * nos + i] *= factor;
-                M.dg[n][d * nos + i] *= factor;
+                M.da[n][j] *= factor;
+                M.dg[n][j] *= factor;
             }
         }
     }
@@ -683,10 +747,22 @@ static void lbfgs_iteration( const StepArgs & A, const MinArgs & M, int nos, hip
     hipLaunchKernelGGL( k_min_update<SOLVER>, g, b, 0, st, A, M, nos );
     if constexpr( SOLVER == SOLVER_LBFGS_ATLAS )
     {
+        hipLaunchKernelGGL( k_atlas_check, g, b, 0, st, A, M, nos );
         launch_scalar( A, M, PH_FLIP, 0, nos, st );
         hipLaunchKernelGGL( k_atlas_flip_sums, g, b, 0, st, A, M, nos );
         launch_scalar( A, M, PH_FLIPRHO, 0, nos, st );
         hipLaunchKernelGGL( k_atlas_flip_apply, g, b, 0, st, A, M, nos );
+    }
+}
+
+void min_launch_chain_update( int solver, const StepArgs & A, const MinArgs & M, int nos, hipStream_t st )
+{
+    switch( solver )
+    {
+        case SOLVER_VP_OSO: vp_iteration<SOLVER_VP_OSO>( A, M, nos, st ); break;
+        case SOLVER_LBFGS_OSO: lbfgs_iteration<SOLVER_LBFGS_OSO>( A, M, nos, st ); break;
+        case SOLVER_LBFGS_ATLAS: lbfgs_iteration<SOLVER_LBFGS_ATLAS>( A, M, nos, st ); break;
+        default: break;
     }
 }
 

@@ -35,8 +35,8 @@ struct MinScalars
     int flip;          // atlas: a spin left its chart (s_z a3 < -0.6)
 };
 
-// Device buffers of a minimisation run (SoA: component q of spin i at [q * nos + i]; the
-// atlas vectors have two components).
+// Device buffers of a minimisation run (per image SoA: component q of spin i of image k at
+// [(k * dim + q) * nos + i], dim = 3, or 2 for the atlas vectors).
 struct MinArgs
 {
     double * v;       // VP, VP_OSO: velocities
@@ -48,11 +48,14 @@ struct MinArgs
     double * q;       // L-BFGS two-loop vector
     double * da[MIN_MEMORY];
     double * dg[MIN_MEMORY];
-    double * a3;   // atlas: chart of each spin (+1 / -1)
-    double * part; // per-block partial sums, [MIN_SLOTS][n_blocks]
+    double * a3;          // atlas: chart of each spin (+1 / -1)
+    double * part;        // per-block partial sums, [MIN_SLOTS][n_blocks]
+    const double * F_ext; // chains (GNEB): the forces F_total of the images; nullptr: the LLG force -g
     MinScalars * sc;
-    int n_blocks;
-    int dim; // 3, or 2 for the atlas
+    int n_blocks;         // noi * bpi
+    int noi;              // images (1 for LLG)
+    int bpi;              // blocks per image
+    int dim;              // 3, or 2 for the atlas
     double maxmove;
 };
 
@@ -62,5 +65,10 @@ bool min_is_solver( int solver );
 // partials into A.part_E and max-torque partials into A.part_T (the layout k_finalize
 // reduces), the effective field into A.eff on observed iterations.
 void min_launch_iteration( int solver, const StepArgs & A, const MinArgs & M, int nos, hipStream_t st );
+
+// One update of `solver` (VP_OSO, LBFGS_OSO, LBFGS_Atlas) on the M.noi images of a chain from
+// their forces M.F_ext (Method_GNEB's F_total): spins A.s (nos per image), parameters A.imgs,
+// run-wide stop flag A.stop.  No energy, torque or effective field (the chain's hook has them).
+void min_launch_chain_update( int solver, const StepArgs & A, const MinArgs & M, int nos, hipStream_t st );
 
 } // namespace spirit_hip

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_goldens.json")))
-SOLVERS = {"vp": 0, "sib": 1, "depondt": 2, "heun": 3, "rk4": 4}
+SOLVERS = {"vp": 0, "sib": 1, "depondt": 2, "heun": 3, "rk4": 4, "lbfgs_oso": 5, "lbfgs_atlas": 6, "vp_oso": 7}
 
 
 def print_case(*a):
@@ -110,14 +110,20 @@ def check(c_gpu, c_ref, E_gpu, Rx_gpu, data, tol=1e-9):
     np.testing.assert_allclose(Rx_gpu, data["Rx"], rtol=1e-6, atol=1e-6)
 
 
-@pytest.mark.parametrize("name", ["vp", "heun", "depondt", "sib", "rk4"])
+# iterations compared per solver: where a solver amplifies rounding differences the run is
+# shortened (measured on the oracle alone with 1e-15 perturbations of the interior images:
+# VP 1e-12 after 20 and 4e-10 after 100 iterations; VP_OSO 4e-13 after 60 and 4e-8 after 100;
+# LBFGS_Atlas 3e-13 after 60; LBFGS_OSO 2e-14 after 100)
+TRAJECTORY_ITERATIONS = {"vp": 60, "vp_oso": 40, "lbfgs_atlas": 60, "lbfgs_oso": 100}
+
+
+@pytest.mark.parametrize("name", ["vp", "heun", "depondt", "sib", "rk4", "lbfgs_oso", "lbfgs_atlas", "vp_oso"])
 def test_gneb_trajectory(state, oracle, name):
     """Every solver on a 7-image skyrmion-collapse chain with a climbing, a falling and a
     stationary image.  The predictor-corrector solvers agree to ~1e-15 after 300 iterations;
-    VP's velocity projection amplifies rounding differences (measured: 2e-16 after 5
-    iterations, 1e-12 after 20, 4e-10 after 100), so VP is compared over 60 iterations."""
+    the velocity-projection and L-BFGS solvers over the iterations of TRAJECTORY_ITERATIONS."""
     print_case("gneb trajectory", name)
-    n = 60 if name == "vp" else 300
+    n = TRAJECTORY_ITERATIONS.get(name, 300)
     types = [0, 0, 1, 0, 2, 3, 0]
     _, cg, cr, Eg, Rg, data = run_both(state, oracle, SMALL, SOLVERS[name], n, types=types)
     check(cg, cr, Eg, Rg, data)
@@ -194,7 +200,7 @@ def test_gneb_single_shot_matches_full_run(state):
     np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("name", ["vp", "heun", "depondt"])
+@pytest.mark.parametrize("name", ["lbfgs_atlas", "lbfgs_oso", "vp_oso", "vp", "heun", "depondt"])
 def test_gneb_saddle_point_golden_api(state, name):
     """core/test/test_solvers.cpp:74-133 through the API: LLG minimisation of a skyrmion,
     9-image chain, 2e4 GNEB iterations, climbing image set automatically, GNEB to convergence;

@@ -323,9 +323,10 @@ def _skyrmion_minimum(oracle):
     return S, s
 
 
-@pytest.mark.parametrize("solver", [0, 3, 2])
+@pytest.mark.parametrize("solver", [6, 5, 7, 0, 3, 2])
 def test_gneb_saddle_point_golden(oracle, solver):
-    """core/test/test_solvers.cpp:74-133 (VP, Heun, Depondt): skyrmion collapse over 9 images -- homogeneous
+    """core/test/test_solvers.cpp:74-133 (LBFGS_Atlas, LBFGS_OSO, VP_OSO, VP, Heun, Depondt: the
+    reference's GNEB solver list): skyrmion collapse over 9 images -- homogeneous
     rotation to +z, 2e4 GNEB iterations, climbing image set automatically, GNEB to convergence;
     the highest interior image has the reference's energy and magnetisation (float, 1e-5)."""
     G = GOLD["solvers"]
