@@ -1,0 +1,308 @@
+// IO_* entry points: spin configurations and chains in OVF 2.0 files.
+//
+// Restated reference code (paths relative to the reference checkout):
+//   core/src/Spirit/IO.cpp:90-812 (IO_Positions_Write, IO_N_Images_In_File, IO_Image_Read /
+//   _Write / _Append, IO_Chain_Read / _Write / _Append), core/src/io/OVF_File.cpp:12-39
+#include "../../include/Spirit/IO.h"
+#include "api_common.hpp"
+#include "ovf.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace spirit_hip;
+
+namespace
+{
+
+// IO::OVF_Segment( system ), OVF_File.cpp:12-39, with the title and value labels IO.cpp sets
+ovf::Segment image_segment( const Image & img, const char * comment, const char * labels )
+{
+    const Geometry & g = *img.geometry;
+    ovf::Segment s;
+    s.title       = std::string( "SPIRIT Version " ) + Spirit_Version_Full();
+    s.comment     = comment ? comment : "";
+    s.valuedim    = 3;
+    s.valuelabels = labels;
+    s.valueunits  = "none none none";
+    s.meshtype    = "rectangular";
+    s.meshunit    = "nm";
+    s.n_cells[0]  = g.n_cells[0] * g.nb;
+    s.n_cells[1]  = g.n_cells[1];
+    s.n_cells[2]  = g.n_cells[2];
+    for( int k = 0; k < 3; ++k )
+    {
+        s.bounds_min[k] = g.bounds_min[k] * 0.1;
+        s.bounds_max[k] = g.bounds_max[k] * 0.1;
+        s.origin[k]     = 0;
+        s.step_size[k]  = g.lattice_constant * g.bravais[k][k] * 0.1;
+    }
+    return s;
+}
+
+// The format written for a requested one.  Spins and positions are doubles, and the
+// reference's double-precision writer turns every binary request into binary in the data's
+// own precision (OVF_File.cpp:127-135, ovf.cpp:371-373): Binary 8, also for bin4.
+int written_format( int format )
+{
+    if( format < IO_Fileformat_OVF_bin || format > IO_Fileformat_OVF_csv )
+        throw api_error( 2, "Invalid file format index " + std::to_string( format ) );
+    if( format == IO_Fileformat_OVF_bin4 || format == IO_Fileformat_OVF_bin8 )
+        return IO_Fileformat_OVF_bin;
+    return format;
+}
+
+void warn_extension( const char * file, int idx_image, int idx_chain )
+{
+    const std::string f = file;
+    if( f.size() < 4 || f.substr( f.size() - 4 ) != ".ovf" )
+        log_message( 3, "The file \"" + f
+                            + "\" is written in OVF format but has different extension. It is recommend to use the "
+                              "appropriate \".ovf\" extension" );
+    (void)idx_image;
+    (void)idx_chain;
+}
+
+// segment data into an image's spins: the first min(N, nos) rows, each normalised, vacancies
+// (|s| < 1e-5) set to +z (IO.cpp:231-283)
+void read_into( Image & img, const std::string & file, int idx_infile, int n_segments )
+{
+    ovf::Segment seg;
+    std::vector<double> data;
+    ovf::read_segment( file, idx_infile, seg, data );
+    long N = seg.rows();
+    if( N != img.nos() )
+        log_message( 3, "OVF file \"" + file + "\": segment " + std::to_string( idx_infile + 1 ) + "/"
+                            + std::to_string( n_segments ) + " contains " + std::to_string( N )
+                            + " spins while the system contains " + std::to_string( img.nos() ) + "." );
+    if( seg.valuedim != 3 )
+        throw api_error( 2, "Segment " + std::to_string( idx_infile + 1 ) + "/" + std::to_string( n_segments )
+                                + " in OVF file \"" + file + "\" should have 3 columns, but only has "
+                                + std::to_string( seg.valuedim ) + ". Will not read." );
+    N = std::min<long>( N, img.nos() );
+    for( long i = 0; i < N; ++i )
+    {
+        double v[3] = { data[3 * i], data[3 * i + 1], data[3 * i + 2] };
+        double n    = std::sqrt( v[0] * v[0] + v[1] * v[1] + v[2] * v[2] );
+        if( n < 1e-5 )
+            v[0] = 0, v[1] = 0, v[2] = 1;
+        else
+            v[0] /= n, v[1] /= n, v[2] /= n;
+        for( int c = 0; c < 3; ++c )
+            img.spins[3 * i + c] = v[c];
+    }
+}
+
+} // namespace
+
+extern "C" void IO_Positions_Write( State * state, const char * file, int format, const char * comment,
+                                    int idx_image, int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    format = written_format( format );
+    warn_extension( file, idx_image, idx_chain );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    auto seg = image_segment( *image, comment, "position_x position_y position_z" );
+    ovf::write_segment( file, seg, image->geometry->positions.data(), format, false );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" int IO_N_Images_In_File( State * state, const char * file, int idx_image, int idx_chain ) noexcept
+try
+{
+    (void)state;
+    auto info = ovf::inspect( file );
+    if( info.is_ovf )
+        return info.n_segments;
+    log_message( 3, std::string( "File \"" ) + file + "\" is not OVF. Cannot measure number of images." );
+    return -1;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return -1;
+}
+
+extern "C" void IO_Image_Read( State * state, const char * file, int idx_image_infile, int idx_image_inchain,
+                               int idx_chain ) noexcept
+try
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image_inchain, idx_chain, image, chain );
+    auto info = ovf::inspect( file );
+    if( !info.is_ovf )
+        throw api_error( 2, std::string( "File \"" ) + file + "\" does not seem to be in valid OVF format. Message: "
+                                + info.message + "." );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    sync_image_host( state, idx_image_inchain );
+    read_into( *image, file, idx_image_infile, info.n_segments );
+    notify_image_changed( state, idx_image_inchain, true, false, false );
+    log_message( 5, std::string( "Read image from file \"" ) + file + "\"" );
+}
+catch( ... )
+{
+    handle_exception( idx_image_inchain, idx_chain );
+}
+
+static void image_write( State * state, const char * file, int format, const char * comment, int idx_image,
+                         int idx_chain, bool append )
+{
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    format = written_format( format );
+    if( !append )
+        warn_extension( file, idx_image, idx_chain );
+    else
+    {
+        auto info = ovf::inspect( file );
+        if( info.found && !info.is_ovf )
+            throw api_error( 2, std::string( "Cannot append to non-OVF file \"" ) + file + "\"" );
+    }
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    sync_image_host( state, idx_image );
+    auto seg = image_segment( *image, comment, "spin_x spin_y spin_z" );
+    ovf::write_segment( file, seg, image->spins.data(), format, append );
+}
+
+extern "C" void IO_Image_Write( State * state, const char * file, int format, const char * comment, int idx_image,
+                                int idx_chain ) noexcept
+try
+{
+    image_write( state, file, format, comment, idx_image, idx_chain, false );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void IO_Image_Append( State * state, const char * file, int format, const char * comment, int idx_image,
+                                 int idx_chain ) noexcept
+try
+{
+    image_write( state, file, format, comment, idx_image, idx_chain, true );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+// IO.cpp:439-649.  The images from start_image_infile .. end_image_infile go to the chain
+// images insert_idx, insert_idx + 1, ...; the chain grows (copies of its last image) when
+// they do not fit.
+extern "C" void IO_Chain_Read( State * state, const char * file, int start_image_infile, int end_image_infile,
+                               int insert_idx, int idx_chain ) noexcept
+try
+{
+    int idx0 = insert_idx;
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx0, idx_chain, image, chain );
+    if( insert_idx < 0 )
+        insert_idx = 0;
+    const int noi = chain->noi();
+    if( insert_idx > noi )
+        throw api_error( 2, "IO_Chain_Read: Tried to start reading chain on invalid index (insert_idx="
+                                + std::to_string( insert_idx ) + ", but chain has " + std::to_string( noi )
+                                + " images)" );
+    auto info = ovf::inspect( file );
+    if( !info.is_ovf )
+        throw api_error( 2, std::string( "IO_Chain_Read: File \"" ) + file + "\" is not in OVF format: "
+                                + info.message );
+    const int noi_infile = info.n_segments;
+    if( start_image_infile < 0 )
+        start_image_infile = 0;
+    if( end_image_infile < 0 )
+        end_image_infile = noi_infile - 1;
+    if( end_image_infile < start_image_infile || end_image_infile >= noi_infile )
+    {
+        log_message( 3, "IO_Chain_Read: specified invalid reading range (start_image_infile="
+                            + std::to_string( start_image_infile ) + ", end_image_infile="
+                            + std::to_string( end_image_infile ) + "). Set to read entire file \"" + file + "\" ("
+                            + std::to_string( noi_infile ) + " images)." );
+        end_image_infile = noi_infile - 1;
+    }
+    if( start_image_infile >= noi_infile )
+        throw api_error( 2, "Specified starting index " + std::to_string( start_image_infile ) + ", but file \""
+                                + file + "\" contains only " + std::to_string( noi_infile ) + " images." );
+    const int noi_to_read = end_image_infile - start_image_infile + 1;
+    const int noi_to_add  = noi_to_read - ( noi - insert_idx );
+    if( noi_to_add > 0 )
+    {
+        Chain_Image_to_Clipboard( state, noi - 1, idx_chain );
+        Chain_Set_Length( state, noi + noi_to_add, idx_chain );
+    }
+    sync_all_hosts( state );
+    for( int k = 0; k < noi_to_read; ++k )
+    {
+        const int i = insert_idx + k;
+        auto & img  = *chain->images[i];
+        std::lock_guard<std::recursive_mutex> lk( img.mtx );
+        read_into( img, file, start_image_infile + k, noi_infile );
+        notify_image_changed( state, i, true, false, false );
+    }
+    State_Update( state );
+    Chain_Setup_Data( state, idx_chain );
+    log_message( 5, std::string( "IO_Chain_Read: Read chain from file \"" ) + file + "\"" );
+}
+catch( ... )
+{
+    handle_exception( insert_idx, idx_chain );
+}
+
+// IO.cpp:651-812: one segment per image, "Image i of noi. comment" (i from 1)
+static void chain_write( State * state, const char * file, int format, const char * comment, int idx_chain,
+                         bool append )
+{
+    int idx_image = 0;
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    format = written_format( format );
+    if( append )
+    {
+        auto info = ovf::inspect( file );
+        if( info.found && !info.is_ovf )
+            throw api_error( 2, std::string( "Cannot append to non-OVF file \"" ) + file + "\"" );
+    }
+    std::lock_guard<std::recursive_mutex> lk( chain->mtx );
+    sync_all_hosts( state );
+    const int noi = chain->noi();
+    for( int i = 0; i < noi; ++i )
+    {
+        const std::string c = "Image " + std::to_string( i + 1 ) + " of " + std::to_string( noi ) + ". "
+                              + ( comment ? comment : "" );
+        auto seg = image_segment( *chain->images[i], c.c_str(), "spin_x spin_y spin_z" );
+        ovf::write_segment( file, seg, chain->images[i]->spins.data(), format, append || i > 0 );
+    }
+}
+
+extern "C" void IO_Chain_Write( State * state, const char * file, int format, const char * comment,
+                                int idx_chain ) noexcept
+try
+{
+    chain_write( state, file, format, comment, idx_chain, false );
+}
+catch( ... )
+{
+    handle_exception( 0, idx_chain );
+}
+
+extern "C" void IO_Chain_Append( State * state, const char * file, int format, const char * comment,
+                                 int idx_chain ) noexcept
+try
+{
+    chain_write( state, file, format, comment, idx_chain, true );
+}
+catch( ... )
+{
+    handle_exception( 0, idx_chain );
+}
