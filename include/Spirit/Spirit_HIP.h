@@ -44,6 +44,16 @@ SPIRIT_API int Spirit_HIP_Gradient_and_Energy( State * state, double * gradient,
                                                int idx_image SPIRIT_DEFAULT( -1 ),
                                                int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* Per-spin energy contributions of the image's current spins (the reference's
+ * Hamiltonian_Heisenberg::Energy_Contributions_per_Spin, Hamiltonian_Heisenberg.cpp:262-305,
+ * which its C API reaches only through IO_Image_Write_Energy_per_Spin), evaluated on the GPU:
+ * out[k * nos + i] for the contributions System_Get_Energy_Array_Names lists.  Also updates
+ * the image's energy and System_Get_Energy_Array.  Returns the number of contributions
+ * (out may be NULL to query it). */
+SPIRIT_API int Spirit_HIP_Get_Energy_Contributions_per_Spin( State * state, double * out,
+                                                             int idx_image SPIRIT_DEFAULT( -1 ),
+                                                             int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
 /* For benchmarks: mean wall time [ms] of one step in the last Simulation_N_Shot /
  * run batch of the image's LLG method (0 if none). */
 SPIRIT_API double Spirit_HIP_Get_Last_Step_Time( State * state, int idx_image SPIRIT_DEFAULT( -1 ),

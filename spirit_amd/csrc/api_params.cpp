@@ -426,9 +426,31 @@ catch( ... )
     return 0;
 }
 
-// The energy of the image's host spins, with the effective field -dE/ds (Spin_System::UpdateEnergy /
-// UpdateEffectiveField, core/src/data/Spin_System.cpp:138-161), evaluated on the GPU.
-static double update_image_energy( Image & image, bool store_field )
+// The contributions present in a Hamiltonian (Hamiltonian_Heisenberg::Update_Energy_Contributions,
+// Hamiltonian_Heisenberg.cpp:200-260), as indices into the per-spin terms of local_energy_terms.
+static std::vector<std::pair<std::string, int>> energy_contribution_list( const Image & image )
+{
+    const Hamiltonian & h = *image.hamiltonian;
+    std::vector<std::pair<std::string, int>> c;
+    if( std::abs( h.field_magnitude ) > 1e-60 )
+        c.push_back( { "Zeeman", 0 } );
+    if( !h.aniso_idx.empty() || !h.site_K.empty() )
+        c.push_back( { "Anisotropy", 1 } );
+    if( !h.cubic_idx.empty() )
+        c.push_back( { "Cubic anisotropy", 2 } );
+    if( !h.exch_pairs.empty() )
+        c.push_back( { "Exchange", 3 } );
+    if( !h.dmi_pairs.empty() )
+        c.push_back( { "DMI", 4 } );
+    if( h.ddi_method != DDI_NONE )
+        c.push_back( { "DDI", 5 } );
+    return c;
+}
+
+// The energy of the image's host spins, with the effective field -dE/ds and the energy
+// contributions (Spin_System::UpdateEnergy / UpdateEffectiveField, core/src/data/Spin_System.cpp:138-161,
+// Hamiltonian_Heisenberg::Energy_Contributions), evaluated on the GPU.
+static double update_image_energy( Image & image, bool store_field, std::vector<double> * per_spin = nullptr )
 {
     ImageSpec sp{ image.geometry, image.hamiltonian, *image.llg, image.spins.data() };
     LLGRunner r( { sp }, SOLVER_DEPONDT );
@@ -437,6 +459,19 @@ static double update_image_energy( Image & image, bool store_field )
     if( store_field )
         for( size_t k = 0; k < g.size(); ++k )
             image.eff_field[k] = -1 * g[k];
+    const size_t nos = (size_t)image.nos();
+    std::vector<double> terms( 6 * nos );
+    r.energy_terms( 0, terms.data() );
+    image.E_array.clear();
+    for( auto & c : energy_contribution_list( image ) )
+    {
+        double sum = 0;
+        for( size_t i = 0; i < nos; ++i )
+            sum += terms[(size_t)c.second * nos + i];
+        image.E_array.push_back( { c.first, sum } );
+    }
+    if( per_spin )
+        *per_spin = std::move( terms );
     return E;
 }
 
@@ -444,10 +479,12 @@ extern "C" int System_Get_Energy_Array_Names( State * state, char * names, int i
 try
 {
     API_IMAGE();
-    const char * n = "Total";
+    std::string n;
+    for( auto & c : energy_contribution_list( *image ) )
+        n += ( n.empty() ? "" : "|" ) + c.first;
     if( names )
-        std::strcpy( names, n );
-    return (int)std::strlen( n );
+        std::strcpy( names, n.c_str() );
+    return (int)n.size();
 }
 catch( ... )
 {
@@ -455,17 +492,44 @@ catch( ... )
     return 0;
 }
 
+// System.cpp:179-208: the contributions of the last System_Update_Data (per spin with
+// divide_by_nspins); with energies == nullptr the number of contributions
 extern "C" int System_Get_Energy_Array( State * state, float * energies, bool divide_by_nspins, int idx_image,
                                         int idx_chain ) noexcept
 try
 {
     API_IMAGE();
-    double E = image->E;
-    if( divide_by_nspins )
-        E /= image->nos();
-    if( energies )
-        energies[0] = (float)E;
-    return 1;
+    if( !energies )
+        return (int)image->E_array.size();
+    const double nd = divide_by_nspins ? 1 / (double)image->nos() : 1;
+    for( size_t k = 0; k < image->E_array.size(); ++k )
+        energies[k] = (float)( nd * image->E_array[k].second );
+    return -1;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return -1;
+}
+
+// Per-spin energy contributions of the image's current spins, [contribution][nos] in the order of
+// System_Get_Energy_Array_Names; returns the number of contributions (out may be nullptr).
+extern "C" int Spirit_HIP_Get_Energy_Contributions_per_Spin( State * state, double * out, int idx_image,
+                                                             int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    sync_image_host( state, idx_image );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    auto list = energy_contribution_list( *image );
+    if( !out )
+        return (int)list.size();
+    std::vector<double> terms;
+    image->E = update_image_energy( *image, true, &terms );
+    const size_t nos = (size_t)image->nos();
+    for( size_t k = 0; k < list.size(); ++k )
+        std::memcpy( out + k * nos, terms.data() + (size_t)list[k].second * nos, sizeof( double ) * nos );
+    return (int)list.size();
 }
 catch( ... )
 {
@@ -481,7 +545,11 @@ extern "C" void System_Print_Energy_Array( State * state, int idx_image, int idx
 try
 {
     API_IMAGE();
-    std::printf( "E_tot = %.12g  ||  E_tot/N = %.12g\n", image->E, image->E / image->nos() );
+    // System.cpp:232-256
+    const double n = image->nos();
+    std::printf( "E_tot = %.12g  ||  E_tot/N = %.12g\n", image->E, image->E / n );
+    for( auto & c : image->E_array )
+        std::printf( "  %s = %.12g  ||  /N = %.12g\n", c.first.c_str(), c.second, c.second / n );
 }
 catch( ... )
 {

@@ -1064,6 +1064,21 @@ void LLGRunner::initial_observe()
     HIPCHECK( hipStreamSynchronize( stream ) );
 }
 
+void LLGRunner::energy_terms( int img, double * out )
+{
+    StepArgs A = STEP_ARGS();
+    if( any_ddi )
+        ddi_field( A.s );
+    DeviceBuffer e( sizeof( double ) * 6 * (size_t)n_total );
+    llg_launch_energy_terms( A, A.s, e.as<double>(), n_blocks, stream );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    const ImageDev & I = imgs_host[img];
+    const int nos      = specs[img].geometry->nos;
+    HIPCHECK( hipMemcpy( out, e.as<double>() + 6 * I.off, sizeof( double ) * 6 * (size_t)nos,
+                         hipMemcpyDeviceToHost ) );
+}
+
 double LLGRunner::gradient( int img, double * grad_aos )
 {
     StepArgs A = STEP_ARGS();

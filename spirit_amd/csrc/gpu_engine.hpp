@@ -161,6 +161,9 @@ public:
     void download_effective_field( int img, double * aos );
     // gradient (AoS) and energy of the current device spins of image img
     double gradient( int img, double * grad_aos );
+    // per-spin energy contributions of image img's current spins, out[t * nos + i] for the six
+    // terms Zeeman, anisotropy, cubic anisotropy, exchange, DMI, DDI
+    void energy_terms( int img, double * out );
     // injected N(0,1) noise for the next `n_steps` steps, AoS per spin over all images
     void set_injected_noise( const double * noise, long n_steps );
 

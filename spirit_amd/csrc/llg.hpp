@@ -40,6 +40,9 @@ void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, 
 // *step += by on the device (the step counter of graph-replayed step sequences)
 void llg_launch_step_advance( uint64_t * step, int by, hipStream_t st );
 void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, int n_blocks, hipStream_t st );
+// per-spin energy contributions (Zeeman, anisotropy, cubic, exchange, DMI, DDI), SoA [6][nos] per
+// image at 6*off; the DDI field of x must be in A.h
+void llg_launch_energy_terms( const StepArgs & A, const double * x, double * out, int n_blocks, hipStream_t st );
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
 // Per-image energy / max torque into obs[2*img], [2*img+1]; with check_convergence an image whose
 // torque is below force_convergence stops (img_stop, stop_step[img]); the run-wide stop flag is set

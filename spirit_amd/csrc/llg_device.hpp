@@ -334,6 +334,72 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
     }
 }
 
+// Energy contributions of spin i (Hamiltonian_Heisenberg::E_Zeeman, E_Anisotropy,
+// E_Cubic_Anisotropy, E_Exchange, E_DMI, E_DDI_FFT, :306-535), in the order
+// ENERGY_TERMS lists; the pair terms count half of each pair at each of its spins.
+constexpr int N_ENERGY_TERMS = 6; // Zeeman, Anisotropy, Cubic anisotropy, Exchange, DMI, DDI
+__device__ __forceinline__ void local_energy_terms( const HamDev & H, const double * __restrict__ x, int nos, int i,
+                                                    const double * __restrict__ hddi, double * E )
+{
+    const int cell = fast_div( i, H.div_nb );
+    const int ib   = i - cell * H.nb;
+    const int rest = fast_div( cell, H.div_Na );
+    const int a    = cell - rest * H.Na;
+    const int c    = fast_div( rest, H.div_Nb );
+    const int b    = rest - c * H.Nb;
+    const double s[3] = { x[i], x[nos + i], x[2 * nos + i] };
+    for( int t = 0; t < N_ENERGY_TERMS; ++t )
+        E[t] = 0;
+    if( H.has_zeeman )
+        E[0] -= H.mu[ib] * H.B * dot3( H.Bn, s );
+    for( int k = 0; k < H.n_aniso; ++k )
+        if( H.aniso_idx[k] == ib )
+        {
+            const double d = dot3( H.aniso_n[k], s );
+            E[1] -= H.aniso_K[k] * pow( d, 2.0 );
+        }
+    if( H.has_site_K )
+    {
+        const double d = dot3( H.site_n, s );
+        E[1] -= H.site_K[i] * pow( d, 2.0 );
+    }
+    for( int k = 0; k < H.n_cubic; ++k )
+        if( H.cubic_idx[k] == ib )
+            E[2] -= H.cubic_K[k] / 2 * ( pow( s[0], 4.0 ) + pow( s[1], 4.0 ) + pow( s[2], 4.0 ) );
+    for( int k = 0; k < H.n_exch; ++k )
+    {
+        const PairDev & p = H.exch[k];
+        if( p.i != ib )
+            continue;
+        const int j = pair_neighbour( H, a, b, c, p );
+        if( j >= 0 )
+        {
+            const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
+            E[3] -= 0.5 * p.J * dot3( s, sj );
+        }
+    }
+    for( int k = 0; k < H.n_dmi; ++k )
+    {
+        const PairDev & p = H.dmi[k];
+        if( p.i != ib )
+            continue;
+        const int j = pair_neighbour( H, a, b, c, p );
+        if( j >= 0 )
+        {
+            const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
+            double cr[3];
+            cross3( s, sj, cr );
+            E[4] -= 0.5 * p.D * dot3( p.n, cr );
+        }
+    }
+    if( hddi )
+    {
+        // the DDI gradient is -h (h = mu_i (D * mu s)_i / M): E_i = 1/2 s . (-h)
+        const double g[3] = { -hddi[i], -hddi[nos + i], -hddi[2 * nos + i] };
+        E[5] += 0.5 * dot3( s, g );
+    }
+}
+
 // Method_LLG::Calculate_Force_Virtual (no STT), Method_LLG.cpp:132-226
 __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, double mu, const double * s,
                                                const double * F, const double * xi, double * fv )

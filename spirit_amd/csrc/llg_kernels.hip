@@ -318,6 +318,31 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_gradient( StepArgs A, const d
     block_reduce_store( e_i, 0.0, A.part_E, A.part_T, blk );
 }
 
+// Energy contributions per spin (SoA [term][nos] per image at 6*off), for
+// Energy_Contributions_per_Spin / System_Update_Data.
+__global__ void __launch_bounds__( LLG_THREADS ) k_energy_terms( StepArgs A, const double * __restrict__ x0,
+                                                                double * __restrict__ out )
+{
+    const int blk           = xcd_contiguous_block();
+    const ImageDev * __restrict__ I = A.imgs + A.blk_img[blk];
+    const HamDev & H        = I->ham;
+    const int nos           = H.nos;
+    const long long base    = 3 * I->off;
+    const int i             = ( blk - I->block_begin ) * LLG_THREADS + threadIdx.x;
+    if( i < nos )
+    {
+        double E[N_ENERGY_TERMS];
+        local_energy_terms( H, x0 + base, nos, i, ( A.h && H.has_ddi ) ? A.h + base : nullptr, E );
+        for( int t = 0; t < N_ENERGY_TERMS; ++t )
+            out[N_ENERGY_TERMS * I->off + (long long)t * nos + i] = E[t];
+    }
+}
+
+void llg_launch_energy_terms( const StepArgs & A, const double * x, double * out, int n_blocks, hipStream_t st )
+{
+    hipLaunchKernelGGL( k_energy_terms, dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A, x, out );
+}
+
 // Per-image reduction of the partials (one block per image, fixed order); convergence of
 // each image (Method_Solver.hpp:232-239) with the index of the step at which it converged.
 // Once every image has stopped, the run-wide flag stops all further stages.

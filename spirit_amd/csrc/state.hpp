@@ -26,6 +26,7 @@ struct Image
     std::vector<double> spins;     // AoS host mirror, handed out by System_Get_Spin_Directions
     std::vector<double> eff_field; // AoS
     double E          = 0;
+    std::vector<std::pair<std::string, double>> E_array; // energy contributions (System_Update_Data)
     double max_torque = 0;
     double Rx         = 0;
     bool iteration_allowed  = false;

@@ -93,6 +93,18 @@ def gradient_and_energy(p_state, nos, idx_image=-1, idx_chain=-1):
     return g, e.value
 
 
+_Energy_per_Spin = _fn("Spirit_HIP_Get_Energy_Contributions_per_Spin", [_vp, _PD, _i, _i], _i)
+
+
+def energy_contributions_per_spin(p_state, nos, idx_image=-1, idx_chain=-1):
+    """Per-spin energy contributions {name: (nos,) array} of the image's current spins."""
+    from spirit_amd import system
+    names = system.get_energy_array_names(p_state, idx_image, idx_chain)
+    out = np.zeros((max(len(names), 1), nos))
+    _Energy_per_Spin(ctypes.c_void_p(p_state), out.ctypes.data_as(_PD), _i(idx_image), _i(idx_chain))
+    return {n: out[k] for k, n in enumerate(names)}
+
+
 def last_step_ms(p_state, idx_image=-1, idx_chain=-1):
     return float(_Get_Last_Step_Time(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
 

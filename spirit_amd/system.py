@@ -24,6 +24,8 @@ _Get_Energy = _fn("System_Get_Energy", [_vp, _i, _i], _f)
 _Get_Rx = _fn("System_Get_Rx", [_vp, _i, _i], _f)
 _Update_Data = _fn("System_Update_Data", [_vp, _i, _i])
 _Print_Energy_Array = _fn("System_Print_Energy_Array", [_vp, _i, _i])
+_Get_Energy_Array_Names = _fn("System_Get_Energy_Array_Names", [_vp, ctypes.c_char_p, _i, _i], _i)
+_Get_Energy_Array = _fn("System_Get_Energy_Array", [_vp, ctypes.POINTER(_f), _b, _i, _i], _i)
 
 
 def get_index(p_state):
@@ -61,3 +63,20 @@ def update_data(p_state, idx_image=-1, idx_chain=-1):
 
 def print_energy_array(p_state, idx_image=-1, idx_chain=-1):
     _Print_Energy_Array(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain))
+
+
+def get_energy_array_names(p_state, idx_image=-1, idx_chain=-1):
+    """Names of the energy contributions (System_Get_Energy_Array_Names)."""
+    n = _Get_Energy_Array_Names(ctypes.c_void_p(p_state), None, _i(idx_image), _i(idx_chain))
+    buf = ctypes.create_string_buffer(n + 1)
+    _Get_Energy_Array_Names(ctypes.c_void_p(p_state), buf, _i(idx_image), _i(idx_chain))
+    names = buf.value.decode()
+    return names.split("|") if names else []
+
+
+def get_energy_array(p_state, divide_by_nspins=True, idx_image=-1, idx_chain=-1):
+    """Energy contributions of the last update_data (System_Get_Energy_Array)."""
+    n = _Get_Energy_Array(ctypes.c_void_p(p_state), None, _b(divide_by_nspins), _i(idx_image), _i(idx_chain))
+    arr = (_f * max(n, 1))()
+    _Get_Energy_Array(ctypes.c_void_p(p_state), arr, _b(divide_by_nspins), _i(idx_image), _i(idx_chain))
+    return [float(arr[k]) for k in range(n)]

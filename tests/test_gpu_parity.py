@@ -373,3 +373,40 @@ def test_ddi_spectrum_format(engine, case, expect):
     simulation.stop(e.p)
     print(info)
     assert ("spectrum " + expect + ";") in info
+
+
+ENERGY_TERMS = {"Zeeman": (0, 1.0), "Anisotropy": (1, 0.5), "Cubic anisotropy": (2, 0.25), "Exchange": (3, 0.5),
+                "DMI": (4, 0.5), "DDI": (5, 0.5)}
+
+
+@pytest.mark.parametrize("case", ["square_neel_periodic", "film_bloch_ddi_open", "hex_neel_periodic"])
+def test_energy_contributions_per_spin(engine, oracle, case):
+    """Energy_Contributions_per_Spin (Hamiltonian_Heisenberg.cpp:262-535) against the oracle's
+    per-term gradients: a term homogeneous of degree n in s has E_i = (1/n) s_i . g_i (Zeeman
+    n = 1, anisotropy / exchange / DMI / DDI n = 2, cubic n = 4).  The contributions add up to
+    the total energy, and System_Get_Energy_Array returns their sums after System_Update_Data."""
+    from spirit_amd import hip, system
+    cfg = dict(GRAD_CASES[case])
+    if case == "square_neel_periodic":
+        cfg["cubic_anisotropy_magnitude"] = 0.3
+    print_case("energy per spin", case)
+    K = configs.site_anisotropy(int(np.prod(cfg["n_basis_cells"])), 4) if case == "film_bloch_ddi_open" else None
+    e = engine(cfg, site_K=K)
+    S = oracle.System(cfg, site_K=K)
+    s = oracle.random_spins(S.nos, 12)
+    e.set_spins(s)
+    contrib = hip.energy_contributions_per_spin(e.p, S.nos)
+    names = system.get_energy_array_names(e.p)
+    assert list(contrib) == names and len(names) >= 3
+    total = 0.0
+    for name, E_i in contrib.items():
+        term, scale = ENERGY_TERMS[name]
+        g = S.gradient_term(term, s)
+        ref = scale * np.einsum("ij,ij->i", s, g)
+        assert np.abs(E_i - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), name
+        total += E_i.sum()
+    _, E0 = S.gradient_and_energy(s)
+    assert abs(total - E0) <= 1e-11 * abs(E0)
+    system.update_data(e.p)
+    arr = system.get_energy_array(e.p, divide_by_nspins=False)
+    np.testing.assert_allclose(arr, [float(np.float32(c.sum())) for c in contrib.values()], rtol=1e-6)
