@@ -3,6 +3,7 @@
 // all exceptions, logs them and returns a default value, as the reference does
 // (core/include/utility/Exception.hpp:97-127).
 #include "api_common.hpp"
+#include "mc.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -21,10 +22,12 @@ Image::Image( const Image & o )
         : geometry( o.geometry ),
           hamiltonian( std::make_shared<Hamiltonian>( *o.hamiltonian ) ),
           llg( std::make_shared<LLGParameters>( *o.llg ) ),
+          mc( std::make_shared<MCParameters>( *o.mc ) ),
           prng( o.prng ),
           spins( o.spins ),
           eff_field( o.eff_field ),
           E( o.E ),
+          E_array( o.E_array ),
           max_torque( o.max_torque ),
           Rx( o.Rx )
 {
@@ -76,6 +79,7 @@ std::shared_ptr<Image> make_image( const std::string & cfg )
     img->geometry    = geometry_from_config( cfg );
     img->hamiltonian = hamiltonian_from_config( cfg, *img->geometry );
     img->llg         = std::make_shared<LLGParameters>( llg_parameters_from_config( cfg ) );
+    img->mc          = std::make_shared<MCParameters>( mc_parameters_from_config( cfg ) );
     img->prng        = std::mt19937( (unsigned)img->llg->seed );
     img->spins.assign( 3 * (size_t)img->geometry->nos, 0.0 );
     img->eff_field.assign( 3 * (size_t)img->geometry->nos, 0.0 );
@@ -549,10 +553,150 @@ catch( ... )
     handle_exception( -1, -1 );
 }
 
+// Colour classes for the parallel Metropolis sweep: colour(a, b, c) = (ca a + cb b + cc c) mod K
+// with no exchange or DMI pair inside a class (and consistent across periodic wraps).  One
+// basis atom; the smallest K <= 8 that works.
+static bool mc_colouring( const Geometry & g, const Hamiltonian & h, int & K, int c[3] )
+{
+    if( g.nb != 1 )
+        return false;
+    std::vector<std::array<int, 3>> ts;
+    auto add = [&]( const std::vector<Pair> & v ) {
+        for( auto & p : v )
+        {
+            bool connects = false, self = true;
+            for( int d = 0; d < 3; ++d )
+            {
+                if( p.t[d] != 0 && g.n_cells[d] > 1 && std::abs( p.t[d] ) < g.n_cells[d] )
+                    connects = true;
+                if( p.t[d] != 0 && g.n_cells[d] > 1 )
+                    self = false;
+            }
+            if( connects && !self )
+                ts.push_back( { p.t[0], p.t[1], p.t[2] } );
+        }
+    };
+    add( h.exch_pairs );
+    add( h.dmi_pairs );
+    for( K = 2; K <= 8; ++K )
+        for( c[0] = 0; c[0] < K; ++c[0] )
+            for( c[1] = 0; c[1] < K; ++c[1] )
+                for( c[2] = 0; c[2] < K; ++c[2] )
+                {
+                    bool ok = true;
+                    for( int d = 0; d < 3 && ok; ++d )
+                        if( h.bc[d] && g.n_cells[d] > 1 && ( c[d] * g.n_cells[d] ) % K != 0 )
+                            ok = false;
+                    for( auto & t : ts )
+                    {
+                        if( !ok )
+                            break;
+                        const int v = ( ( c[0] * t[0] + c[1] * t[1] + c[2] * t[2] ) % K + K ) % K;
+                        if( v == 0 )
+                            ok = false;
+                    }
+                    if( ok )
+                        return true;
+                }
+    return false;
+}
+
+// Method_MC with the Metropolis algorithm (Method_MC.cpp:40-200, Method.cpp:57-113): n_iterations
+// sweeps on the GPU; the spins, the adapted cone angle and the energy come back at the end.
 extern "C" void Simulation_MC_Start( State * state, int n_iterations, int n_iterations_log, bool singleshot,
                                      Simulation_Run_Info * info, int idx_image, int idx_chain ) noexcept
+try
 {
-    log_message( 2, "Simulation_MC_Start: Monte Carlo is not part of the HIP engine" );
+    std::shared_ptr<Image> image;
+    std::shared_ptr<Chain> chain;
+    from_indices( state, idx_image, idx_chain, image, chain );
+    if( image->iteration_allowed || chain->iteration_allowed )
+        throw api_error( 3, "Tried to use Simulation_Start on image " + std::to_string( idx_image ) + " of chain "
+                                + std::to_string( idx_chain ) + ", but there is already a simulation running." );
+    if( singleshot )
+        throw api_error( 2, "Simulation_MC_Start: single-shot Monte Carlo is not available in the HIP engine" );
+    (void)n_iterations_log;
+    MCParameters & P = *image->mc;
+    if( n_iterations > 0 )
+        P.n_iterations = n_iterations;
+    const long n = std::max( 1L, P.n_iterations );
+    int K = 0, col[3] = { 0, 0, 0 };
+    if( !mc_colouring( *image->geometry, *image->hamiltonian, K, col ) )
+        throw api_error( 2, "Simulation_MC_Start: the lattice's pairs admit no colouring into at most 8 classes "
+                            "(one basis atom is required)" );
+    sync_image_host( state, idx_image );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    image->iteration_allowed = true;
+    const auto t_start       = std::chrono::steady_clock::now();
+    long done                = 0;
+    try
+    {
+        ImageSpec sp{ image->geometry, image->hamiltonian, *image->llg, image->spins.data() };
+        LLGRunner r( { sp }, SOLVER_DEPONDT );
+        const int nos      = image->nos();
+        const int n_blocks = ( nos + 255 ) / 256;
+        DeviceBuffer cone( sizeof( double ) ), part( sizeof( double ) * K * (size_t)n_blocks ),
+            step( sizeof( uint64_t ) );
+        const double cone0 = Pi * P.metropolis_cone_angle / 180.0; // Method_MC.cpp:34
+        HIPCHECK( hipMemcpy( cone.as<void>(), &cone0, sizeof( double ), hipMemcpyHostToDevice ) );
+        HIPCHECK( hipMemset( part.as<void>(), 0, part.size() ) );
+        HIPCHECK( hipMemset( step.as<void>(), 0, sizeof( uint64_t ) ) );
+        McArgs A{};
+        A.img         = r.dev_images();
+        A.s           = r.dev_spins();
+        A.nos         = nos;
+        A.K           = K;
+        A.ca          = col[0];
+        A.cb          = col[1];
+        A.cc          = col[2];
+        A.step_cone   = P.metropolis_step_cone ? 1 : 0;
+        A.adaptive    = P.metropolis_cone_adaptive ? 1 : 0;
+        A.seed        = P.seed;
+        A.temperature = P.temperature;
+        A.kT          = k_B * P.temperature;
+        A.target      = P.acceptance_ratio_target;
+        A.cone        = cone.as<double>();
+        A.part        = part.as<double>();
+        A.n_blocks    = n_blocks;
+        A.step        = step.as<uint64_t>();
+        const long batch = 1000;
+        while( done < n && image->iteration_allowed && !stop_file_present() )
+        {
+            const long m = std::min( batch, n - done );
+            for( long k = 0; k < m; ++k )
+                mc_launch_iteration( A, step.as<uint64_t>(), r.stream );
+            HIPCHECK( hipGetLastError() );
+            HIPCHECK( hipStreamSynchronize( r.stream ) );
+            done += m;
+            if( P.max_walltime_sec > 0
+                && std::chrono::duration<double>( std::chrono::steady_clock::now() - t_start ).count()
+                       > (double)P.max_walltime_sec )
+                break;
+        }
+        r.download_spins();
+        double cone_end = cone0;
+        HIPCHECK( hipMemcpy( &cone_end, cone.as<void>(), sizeof( double ), hipMemcpyDeviceToHost ) );
+        if( P.metropolis_step_cone && P.metropolis_cone_adaptive )
+            P.metropolis_cone_angle = cone_end * 180.0 / Pi;
+    }
+    catch( ... )
+    {
+        image->iteration_allowed = false;
+        throw;
+    }
+    image->iteration_allowed = false;
+    image->E                 = update_image_energy( *image, true );
+    if( info )
+    {
+        double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t_start ).count();
+        info->total_iterations = (int)done;
+        info->total_walltime   = (int)ms;
+        info->total_ips        = ms > 0 ? float( done / ms * 1000.0 ) : 0.f;
+    }
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
 }
 
 extern "C" void Simulation_MMF_Start( State * state, int solver_type, int n_iterations, int n_iterations_log,

@@ -38,6 +38,10 @@ void sync_image_host( State * state, int idx_image );
 void sync_all_hosts( State * state );
 void notify_image_changed( State * state, int idx_image, bool spins_changed, bool ham_changed, bool llg_changed );
 bool stop_file_present();
+// energy, effective field (store_field) and energy contributions of an image's host spins,
+// evaluated on the GPU; per_spin (nullable) receives the six per-spin terms [6][nos]
+double update_image_energy( Image & image, bool store_field, std::vector<double> * per_spin = nullptr );
+std::vector<std::pair<std::string, int>> energy_contribution_list( const Image & image );
 
 // GNEB (gneb_api.cpp)
 void gneb_n_shot( State * state, int N );

@@ -428,7 +428,7 @@ catch( ... )
 
 // The contributions present in a Hamiltonian (Hamiltonian_Heisenberg::Update_Energy_Contributions,
 // Hamiltonian_Heisenberg.cpp:200-260), as indices into the per-spin terms of local_energy_terms.
-static std::vector<std::pair<std::string, int>> energy_contribution_list( const Image & image )
+std::vector<std::pair<std::string, int>> spirit_hip::energy_contribution_list( const Image & image )
 {
     const Hamiltonian & h = *image.hamiltonian;
     std::vector<std::pair<std::string, int>> c;
@@ -450,7 +450,7 @@ static std::vector<std::pair<std::string, int>> energy_contribution_list( const 
 // The energy of the image's host spins, with the effective field -dE/ds and the energy
 // contributions (Spin_System::UpdateEnergy / UpdateEffectiveField, core/src/data/Spin_System.cpp:138-161,
 // Hamiltonian_Heisenberg::Energy_Contributions), evaluated on the GPU.
-static double update_image_energy( Image & image, bool store_field, std::vector<double> * per_spin = nullptr )
+double spirit_hip::update_image_energy( Image & image, bool store_field, std::vector<double> * per_spin )
 {
     ImageSpec sp{ image.geometry, image.hamiltonian, *image.llg, image.spins.data() };
     LLGRunner r( { sp }, SOLVER_DEPONDT );
@@ -1470,4 +1470,118 @@ extern "C" const char * Spirit_HIP_Device_Name() noexcept
         return "";
     name = std::string( prop.name ) + " (" + prop.gcnArchName + ")";
     return name.c_str();
+}
+
+// ---------------------------------------------------------------------------
+// Parameters_MC (core/src/Spirit/Parameters_MC.cpp)
+// ---------------------------------------------------------------------------
+#include "../../include/Spirit/Parameters_MC.h"
+
+extern "C" void Parameters_MC_Set_N_Iterations( State * state, int n_iterations, int n_iterations_log, int idx_image,
+                                                int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    image->mc->n_iterations     = n_iterations;
+    image->mc->n_iterations_log = n_iterations_log;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Parameters_MC_Set_Temperature( State * state, float T, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    image->mc->temperature = T;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Parameters_MC_Set_Metropolis_Cone( State * state, bool cone, float cone_angle, bool adaptive_cone,
+                                                   float target_acceptance_ratio, int idx_image,
+                                                   int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    image->mc->metropolis_step_cone     = cone;
+    image->mc->metropolis_cone_angle    = cone_angle;
+    image->mc->metropolis_cone_adaptive = adaptive_cone;
+    image->mc->acceptance_ratio_target  = target_acceptance_ratio;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Parameters_MC_Set_Random_Sample( State * state, bool random_sample, int idx_image,
+                                                 int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    image->mc->metropolis_random_sample = random_sample;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" void Parameters_MC_Get_N_Iterations( State * state, int * iterations, int * iterations_log, int idx_image,
+                                                int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    *iterations     = (int)image->mc->n_iterations;
+    *iterations_log = (int)image->mc->n_iterations_log;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" float Parameters_MC_Get_Temperature( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    return (float)image->mc->temperature;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+extern "C" void Parameters_MC_Get_Metropolis_Cone( State * state, bool * cone, float * cone_angle,
+                                                   bool * adaptive_cone, float * target_acceptance_ratio,
+                                                   int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    *cone                    = image->mc->metropolis_step_cone;
+    *cone_angle              = (float)image->mc->metropolis_cone_angle;
+    *adaptive_cone           = image->mc->metropolis_cone_adaptive;
+    *target_acceptance_ratio = (float)image->mc->acceptance_ratio_target;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+extern "C" bool Parameters_MC_Get_Random_Sample( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    return image->mc->metropolis_random_sample;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return false;
 }

@@ -720,6 +720,31 @@ LLGParameters llg_parameters_from_config( const std::string & path )
     return p;
 }
 
+// Parameters_Method_MC_from_Config, core/src/io/Configparser.cpp:920-980
+MCParameters mc_parameters_from_config( const std::string & path )
+{
+    MCParameters p;
+    if( path.empty() )
+        return p;
+    ConfigFile f( path );
+    if( !f.ok() )
+        return p;
+    f.read( p.output_file_tag, "output_file_tag" );
+    f.read( p.output_folder, "mc_output_folder" );
+    std::string wt;
+    if( f.read( wt, "mc_max_walltime" ) )
+        p.max_walltime_sec = duration_from_string( wt );
+    long seed;
+    if( f.read( seed, "mc_seed" ) )
+        p.seed = (uint64_t)seed;
+    f.read( p.n_iterations, "mc_n_iterations" );
+    f.read( p.n_iterations_log, "mc_n_iterations_log" );
+    f.read( p.n_iterations_amortize, "mc_n_iterations_amortize" );
+    f.read( p.temperature, "mc_temperature" );
+    f.read( p.acceptance_ratio_target, "mc_acceptance_ratio" );
+    return p;
+}
+
 GNEBParameters gneb_parameters_from_config( const std::string & path )
 {
     GNEBParameters p;

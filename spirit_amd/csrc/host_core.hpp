@@ -151,6 +151,24 @@ struct LLGParameters
     int output_configuration_filetype = 3;
 };
 
+// Monte Carlo parameters, core/include/data/Parameters_Method_MC.hpp
+struct MCParameters
+{
+    long n_iterations             = 1000000;
+    long n_iterations_log         = 1000;
+    long n_iterations_amortize    = 1;
+    long max_walltime_sec         = 0;
+    double temperature            = 0;
+    uint64_t seed                 = 2006;
+    bool metropolis_random_sample = true;
+    bool metropolis_step_cone     = true;
+    bool metropolis_cone_adaptive = true;
+    double metropolis_cone_angle  = 30; // degrees
+    double acceptance_ratio_target = 0.5;
+    std::string output_folder = "output";
+    std::string output_file_tag = "<time>";
+};
+
 struct GNEBParameters
 {
     double spring_constant          = 1;
@@ -237,6 +255,7 @@ private:
 std::shared_ptr<Geometry> geometry_from_config( const std::string & path );
 std::shared_ptr<Hamiltonian> hamiltonian_from_config( const std::string & path, const Geometry & g );
 LLGParameters llg_parameters_from_config( const std::string & path );
+MCParameters mc_parameters_from_config( const std::string & path );
 GNEBParameters gneb_parameters_from_config( const std::string & path );
 long duration_from_string( const std::string & s );
 

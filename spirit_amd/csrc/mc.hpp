@@ -1,0 +1,35 @@
+// Metropolis Monte Carlo of one image on the GPU (Method_MC, core/src/engine/Method_MC.cpp:40-200).
+#pragma once
+
+#include "device_types.hpp"
+
+namespace spirit_hip
+{
+
+// Arguments of the Monte Carlo kernels.  Spins of one colour class share no exchange or DMI
+// pair, so a class is updated in parallel, each trial against the current neighbours.
+struct McArgs
+{
+    const ImageDev * img;
+    double * s;            // SoA spins of the image
+    int nos;
+    int K;                 // colour classes: colour(a, b, c) = (ca a + cb b + cc c) mod K
+    int ca, cb, cc;
+    int step_cone;         // trial moves inside a cone around the spin (else the reference's sphere draw)
+    int adaptive;          // adapt the cone to the target acceptance ratio
+    uint64_t seed;
+    double temperature;    // K
+    double kT;             // k_B T [meV]
+    double target;         // target acceptance ratio
+    double * cone;         // cone angle [rad], adapted on the device
+    double * part;         // per-block rejection counts, [K][n_blocks]
+    int n_blocks;
+    int pad;
+    const uint64_t * step; // Monte Carlo step index (random-number counter)
+};
+
+// One Metropolis iteration: the cone feedback from the previous iteration's rejections, then
+// one sweep over every colour class; advances *step by one.
+void mc_launch_iteration( const McArgs & A, uint64_t * step, hipStream_t st );
+
+} // namespace spirit_hip

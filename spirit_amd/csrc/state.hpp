@@ -22,6 +22,7 @@ struct Image
     std::shared_ptr<Geometry> geometry;
     std::shared_ptr<Hamiltonian> hamiltonian;
     std::shared_ptr<LLGParameters> llg;
+    std::shared_ptr<MCParameters> mc;
     std::mt19937 prng; // seeded with llg.seed, used by Configuration_Random (Parameters_Method_LLG.hpp:22-24)
     std::vector<double> spins;     // AoS host mirror, handed out by System_Get_Spin_Directions
     std::vector<double> eff_field; // AoS
