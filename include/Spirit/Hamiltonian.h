@@ -54,7 +54,7 @@ SPIRIT_API void Hamiltonian_Get_DMI_Shells( State * state, int * n_shells, float
 SPIRIT_API int Hamiltonian_Get_DMI_N_Pairs( State * state, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Hamiltonian.h:164 */
 SPIRIT_API void Hamiltonian_Get_DDI( State * state, int * ddi_method, int n_periodic_images[3], float * cutoff_radius, bool * pb_zero_padding, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
-/* Hamiltonian.h:172 (not part of the HIP engine: logs an error) */
+/* Hamiltonian.h:172: the 3N x 3N embedding Hessian of anisotropy, exchange and DMI as triplets or a dense matrix */
 SPIRIT_API void Hamiltonian_Write_Hessian( State * state, const char * filename, bool triplet_format SPIRIT_DEFAULT( true ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
 #endif

@@ -3,6 +3,7 @@
 // Parameters_LLG,Parameters_GNEB,Quantities,Constants,Version,Log}.cpp.
 #include "api_common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -336,10 +337,68 @@ catch( ... )
     handle_exception( idx_image, idx_chain );
 }
 
+// The embedding Hessian as triplets "row<TAB>col<TAB>value" per line, column-major, or as the
+// dense 3N x 3N matrix with one row per line, "value " per column and "0 " where the sparse
+// matrix holds no entry (core/src/Spirit/Hamiltonian.cpp:672-727 with Eigen's sparse-matrix
+// stream output, core/thirdparty/Eigen/src/SparseCore/SparseMatrixBase.h:218-262). Values
+// are printed as a default C++ stream prints a double (six significant digits, %g).
 extern "C" void Hamiltonian_Write_Hessian( State * state, const char * filename, bool triplet_format, int idx_image,
                                            int idx_chain ) noexcept
+try
 {
-    log_message( 2, "Hamiltonian_Write_Hessian: the Hessian is not part of the HIP engine" );
+    API_IMAGE();
+    if( filename == nullptr )
+        throw std::runtime_error( "Hamiltonian_Write_Hessian: no file name given" );
+    const auto entries = embedding_hessian( *image->geometry, *image->hamiltonian );
+    const std::string fname( filename );
+    std::printf( "Saving %s to file: %s\n", triplet_format ? "triplets" : "matrix", fname.c_str() );
+    std::fflush( stdout );
+    std::FILE * f = std::fopen( fname.c_str(), "w" );
+    if( f == nullptr )
+    {
+        std::fprintf( stderr, "Could not save matrix!" );
+        return;
+    }
+    char num[64];
+    if( triplet_format )
+    {
+        for( const auto & e : entries )
+        {
+            std::snprintf( num, sizeof( num ), "%g", e.value );
+            std::fprintf( f, "%ld\t%ld\t%s\n", e.row, e.col, num );
+        }
+    }
+    else
+    {
+        // row-major walk over the column-major entries
+        const long n = 3L * image->nos();
+        std::vector<HessianEntry> rows( entries );
+        std::stable_sort( rows.begin(), rows.end(), []( const HessianEntry & x, const HessianEntry & y ) {
+            return x.row != y.row ? x.row < y.row : x.col < y.col;
+        } );
+        size_t k = 0;
+        for( long r = 0; r < n; ++r )
+        {
+            long c = 0;
+            for( ; k < rows.size() && rows[k].row == r; ++k )
+            {
+                for( ; c < rows[k].col; ++c )
+                    std::fputs( "0 ", f );
+                std::snprintf( num, sizeof( num ), "%g", rows[k].value );
+                std::fputs( num, f );
+                std::fputc( ' ', f );
+                ++c;
+            }
+            for( ; c < n; ++c )
+                std::fputs( "0 ", f );
+            std::fputc( '\n', f );
+        }
+    }
+    std::fclose( f );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
 }
 
 // ===========================================================================

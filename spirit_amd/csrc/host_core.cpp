@@ -777,3 +777,103 @@ GNEBParameters gneb_parameters_from_config( const std::string & path )
 }
 
 } // namespace spirit_hip
+
+namespace spirit_hip
+{
+
+// Partner of spin ispin across pair p, or -1 where an open boundary cuts the pair
+// (Vectormath.hpp:437-528 with defects off, as the device-side pair_neighbour).
+static int pair_partner( const Geometry & g, const std::array<int, 3> & bc, int ispin, const Pair & p )
+{
+    const int N = g.nb;
+    const int n[3] = { g.n_cells[0], g.n_cells[1], g.n_cells[2] };
+    if( p.i != ispin % N )
+        return -1;
+    int cell = ispin / N;
+    int idx[3];
+    idx[0] = cell % n[0];
+    cell /= n[0];
+    idx[1] = cell % n[1];
+    idx[2] = cell / n[1];
+    for( int d = 0; d < 3; ++d )
+    {
+        if( std::abs( p.t[d] ) > n[d] )
+            return -1;
+        int j = idx[d] + p.t[d];
+        if( !bc[d] && ( j < 0 || j >= n[d] ) )
+            return -1;
+        idx[d] = ( ( j % n[d] ) + n[d] ) % n[d];
+    }
+    return p.j + N * ( idx[0] + n[0] * ( idx[1] + n[1] * idx[2] ) );
+}
+
+std::vector<HessianEntry> embedding_hessian( const Geometry & g, const Hamiltonian & h )
+{
+    std::vector<HessianEntry> t;
+    const int N = g.nb;
+    auto push = [&t]( long i, long j, double v ) { t.push_back( HessianEntry{ i, j, v } ); };
+
+    // single-spin terms; as the reference, every (alpha, beta) entry of the on-site block
+    // -2 K n n^T is added at (3i+alpha, 3i+alpha), so the diagonal holds the row sums
+    for( int icell = 0; icell < g.n_cells_total; ++icell )
+        for( size_t k = 0; k < h.aniso_idx.size(); ++k )
+        {
+            const long ispin = (long)icell * N + h.aniso_idx[k];
+            for( int a = 0; a < 3; ++a )
+                for( int b = 0; b < 3; ++b )
+                    push( 3 * ispin + a, 3 * ispin + a, -2.0 * h.aniso_K[k] * h.aniso_n[k][a] * h.aniso_n[k][b] );
+        }
+    // per-site K_i along one axis: treated as the uniaxial term above
+    for( size_t ispin = 0; ispin < h.site_K.size(); ++ispin )
+        for( int a = 0; a < 3; ++a )
+            for( int b = 0; b < 3; ++b )
+                push( 3 * (long)ispin + a, 3 * (long)ispin + a, -2.0 * h.site_K[ispin] * h.site_n[a] * h.site_n[b] );
+
+    // exchange and DMI over the redundant pair tables (each pair is listed in both
+    // directions, so no transposed entries are added)
+    for( int icell = 0; icell < g.n_cells_total; ++icell )
+        for( size_t k = 0; k < h.exch_pairs.size(); ++k )
+        {
+            const int ispin = h.exch_pairs[k].i + icell * N;
+            const int jspin = pair_partner( g, h.bc, ispin, h.exch_pairs[k] );
+            if( jspin < 0 )
+                continue;
+            for( int a = 0; a < 3; ++a )
+                push( 3L * ispin + a, 3L * jspin + a, -h.exch_J[k] );
+        }
+    for( int icell = 0; icell < g.n_cells_total; ++icell )
+        for( size_t k = 0; k < h.dmi_pairs.size(); ++k )
+        {
+            const int ispin = h.dmi_pairs[k].i + icell * N;
+            const int jspin = pair_partner( g, h.bc, ispin, h.dmi_pairs[k] );
+            if( jspin < 0 )
+                continue;
+            const long i = 3L * ispin, j = 3L * jspin;
+            const double D = h.dmi_D[k];
+            const Vec3 & n = h.dmi_n[k];
+            // d/ds_i of -D n.(s_i x s_j) = -D (s_j x n): the cross-product matrix of n
+            push( i + 2, j + 1, D * n[0] );
+            push( i + 1, j + 2, -D * n[0] );
+            push( i, j + 2, D * n[1] );
+            push( i + 2, j, -D * n[1] );
+            push( i + 1, j, D * n[2] );
+            push( i, j + 1, -D * n[2] );
+        }
+
+    // column-major order, duplicates summed in the order they were added
+    std::stable_sort( t.begin(), t.end(), []( const HessianEntry & x, const HessianEntry & y ) {
+        return x.col != y.col ? x.col < y.col : x.row < y.row;
+    } );
+    std::vector<HessianEntry> out;
+    out.reserve( t.size() );
+    for( const auto & e : t )
+    {
+        if( !out.empty() && out.back().row == e.row && out.back().col == e.col )
+            out.back().value += e.value;
+        else
+            out.push_back( e );
+    }
+    return out;
+}
+
+} // namespace spirit_hip

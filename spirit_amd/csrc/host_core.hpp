@@ -259,6 +259,19 @@ MCParameters mc_parameters_from_config( const std::string & path );
 GNEBParameters gneb_parameters_from_config( const std::string & path );
 long duration_from_string( const std::string & s );
 
+// The 3N x 3N embedding Hessian of the quadratic terms (Hamiltonian_Heisenberg::Sparse_Hessian,
+// core/src/engine/Hamiltonian_Heisenberg.cpp:1277-1371): uniaxial anisotropy, exchange and DMI;
+// Zeeman, cubic anisotropy and DDI contribute nothing. Entries are (row, col, value) with
+// duplicates summed in insertion order and sorted column-major, rows ascending, the order in
+// which a column-major sparse matrix built from the same triplets iterates its entries.
+// Entries whose value is zero stay in the list (they are part of the sparsity pattern).
+struct HessianEntry
+{
+    long row, col;
+    double value;
+};
+std::vector<HessianEntry> embedding_hessian( const Geometry & g, const Hamiltonian & h );
+
 // logging
 void log_message( int level, const std::string & msg );
 extern int g_log_console_level;

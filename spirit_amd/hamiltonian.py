@@ -126,3 +126,14 @@ def get_exchange_pairs(p_state, idx_image=-1, idx_chain=-1):
                         tr.ctypes.data_as(ctypes.POINTER(_i)), J.ctypes.data_as(ctypes.POINTER(_f)), _i(idx_image),
                         _i(idx_chain))
     return idx[:n], tr[:n], J[:n]
+
+
+_Write_Hessian = _fn("Hamiltonian_Write_Hessian", [_vp, ctypes.c_char_p, _b, _i, _i])
+
+
+def write_hessian(p_state, filename, triplet_format=True, idx_image=-1, idx_chain=-1):
+    """Writes the 3N x 3N embedding Hessian (anisotropy, exchange, DMI) to a file, as
+    "row col value" triplets (recommended for large, sparse Hessians) or as a dense matrix
+    (core/python/spirit/hamiltonian.py:338-358)."""
+    _Write_Hessian(ctypes.c_void_p(p_state), filename.encode("utf-8"), _b(triplet_format), _i(idx_image),
+                   _i(idx_chain))
