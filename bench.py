@@ -19,7 +19,8 @@ same replicas run one Simulation_LLG_Start after another; "adjoint" -- 500 steps
 the default workload forward with checkpoints every 50 steps plus the discrete
 adjoint of a mean-square loss (gradients for J, D, B and every K_i); "gneb" -- a
 32-image chain on the 128x128 hexagonal monolayer (skyrmion collapse, climbing
-image, VP), 2000 iterations.
+image, VP), 2000 iterations; "minimiser" -- 200 LBFGS_OSO iterations of the default
+workload; "mc" -- 1000 Metropolis sweeps on the default lattice without DDI.
 """
 import argparse
 import json
@@ -248,6 +249,65 @@ def run_gneb(n_iter, noi):
             "max_torque": float(info.max_torque)}
 
 
+def _cube_state(cfg_over, site_K=True):
+    """State on the default 256x256x128 lattice (overrides applied), per-site K_i and
+    random spins from the workload's seeds."""
+    from spirit_amd import configs, hip, state, system
+    cfg = dict(configs.CUBE_256, **cfg_over)
+    nos = 256 * 256 * 128
+    path = os.path.join(tempfile.mkdtemp(), "cube.cfg")
+    configs.write_config(cfg, path)
+    p = state.setup(path, True)
+    if not p:
+        raise RuntimeError("State_Setup failed")
+    if site_K:
+        hip.set_site_anisotropy(p, configs.site_anisotropy(nos, 20006), (0.0, 0.0, 1.0))
+    system.get_spin_directions(p)[:] = configs.random_spins(nos, 20006)
+    return p, nos
+
+
+def run_minimiser(n_iter, warmup=20):
+    """LBFGS_OSO direct minimisation of the default workload (DDI included): n_iter
+    iterations (one force evaluation plus the L-BFGS update each) through n_shot."""
+    from spirit_amd import hip, simulation, state
+    p, nos = _cube_state({})
+    hip.set_host_sync(p, 0)
+    simulation.start(p, simulation.METHOD_LLG, simulation.SOLVER_LBFGS_OSO, single_shot=True)
+    simulation.n_shot(p, warmup)
+    t0 = time.perf_counter()
+    simulation.n_shot(p, n_iter)
+    t = time.perf_counter() - t0
+    torque = float(simulation.get_max_torque_norm(p))
+    simulation.stop(p)
+    state.delete(p)
+    return {"workload": "cube256_lbfgs_oso_ddi", "spins": nos, "iterations": n_iter, "seconds": t,
+            "ms_per_iteration": 1e3 * t / n_iter, "value": nos * n_iter / t, "unit": "spin-iterations/s",
+            "max_torque_after": torque}
+
+
+def run_mc(n_short, n_long):
+    """Metropolis Monte Carlo (adaptive cone, T = 10 K) on the 256x256x128 lattice without
+    DDI (the single-spin energy has none): the rate is taken from two runs of n_short and
+    n_long sweeps, so set-up and copies cancel."""
+    from spirit_amd import simulation, state
+    from spirit_amd.parameters import mc
+    p, nos = _cube_state({"ddi_method": "none"})
+    mc.set_temperature(p, 10.0)
+    mc.set_metropolis_cone(p, True, 30.0, True, 0.5)
+    times = []
+    for n in (n_short, n_long):
+        t0 = time.perf_counter()
+        info = simulation.start(p, simulation.METHOD_MC, n_iterations=n)
+        times.append(time.perf_counter() - t0)
+        if info.total_iterations != n:
+            raise RuntimeError("MC ran %d of %d sweeps" % (info.total_iterations, n))
+    state.delete(p)
+    dt = times[1] - times[0]
+    return {"workload": "cube256_metropolis_T10", "spins": nos, "sweeps": n_long - n_short, "seconds": dt,
+            "ms_per_sweep": 1e3 * dt / (n_long - n_short), "value": nos * (n_long - n_short) / dt,
+            "unit": "spin-sweeps/s", "call_seconds": times}
+
+
 PASS_C_KERNELS = {0: "k_pass_c_conv", 1: "k_p2_c", 2: "k_p2_c1", 3: "k_p4_c1"}
 
 
@@ -320,8 +380,11 @@ def main():
     ap.add_argument("--workload", default="cube256_depondt_ddi", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,adjoint,gneb",
-                    help="comma-separated: plain (always run), batched (uneven ensemble), adjoint, gneb")
+    ap.add_argument("--legs", default="plain,batched,adjoint,gneb,minimiser,mc",
+                    help="comma-separated: plain (always run), batched (uneven ensemble), adjoint, gneb, "
+                         "minimiser (LBFGS_OSO), mc (Metropolis)")
+    ap.add_argument("--minimiser-iterations", type=int, default=200)
+    ap.add_argument("--mc-sweeps", type=int, default=1000)
     ap.add_argument("--gneb-iterations", type=int, default=2000)
     ap.add_argument("--gneb-images", type=int, default=32)
     ap.add_argument("--replicas", type=int, default=256)
@@ -387,6 +450,10 @@ def main():
                                      ms_per_step if args.workload == "cube256_depondt_ddi" else None)
     if "gneb" in legs:
         out["gneb"] = run_gneb(args.gneb_iterations, args.gneb_images)
+    if "minimiser" in legs:
+        out["minimiser"] = run_minimiser(args.minimiser_iterations)
+    if "mc" in legs:
+        out["mc"] = run_mc(max(1, args.mc_sweeps // 5), args.mc_sweeps + max(1, args.mc_sweeps // 5))
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         try:

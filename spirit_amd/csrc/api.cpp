@@ -659,6 +659,15 @@ try
         A.part        = part.as<double>();
         A.n_blocks    = n_blocks;
         A.step        = step.as<uint64_t>();
+        // per-colour sweeps where ca is invertible mod K (one basis atom is required above)
+        for( int x = 1; x < K; ++x )
+            if( ( col[0] * x ) % K == 1 )
+                A.inv_ca = x;
+        A.groups        = ( image->geometry->n_cells[0] + K - 1 ) / K;
+        A.n_blocks_rows = (int)( ( (long)A.groups * image->geometry->n_cells[1] * image->geometry->n_cells[2] + 255 )
+                                 / 256 );
+        if( A.n_blocks_rows > n_blocks )
+            A.inv_ca = 0;
         const long batch = 1000;
         while( done < n && image->iteration_allowed && !stop_file_present() )
         {

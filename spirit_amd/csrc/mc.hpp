@@ -23,8 +23,10 @@ struct McArgs
     double target;         // target acceptance ratio
     double * cone;         // cone angle [rad], adapted on the device
     double * part;         // per-block rejection counts, [K][n_blocks]
-    int n_blocks;
-    int pad;
+    int n_blocks;          // blocks of the one-thread-per-site sweep (part's row length)
+    int inv_ca;            // inverse of ca mod K with one basis atom, else 0: per-colour sweeps
+    int groups;            // ceil(Na / K): groups of K cells per row
+    int n_blocks_rows;     // blocks of the per-colour sweep (<= n_blocks)
     const uint64_t * step; // Monte Carlo step index (random-number counter)
 };
 

@@ -75,76 +75,69 @@ __device__ __forceinline__ void mc_uniforms( uint64_t seed, uint64_t step, int c
     u[3] = u53( c1[2], c1[3] );
 }
 
-__global__ void __launch_bounds__( MC_THREADS ) k_mc_colour( McArgs A, int colour )
+// Metropolis trial of site i (cell (a, b, c), basis atom ib) of colour class `colour`;
+// returns 1 when the move is rejected.  McArgs is taken by value so that its fields stay
+// kernel arguments (scalar loads).
+__device__ __forceinline__ double mc_site( McArgs A, int colour, int i, int ib, int a, int b, int c )
 {
     const HamDev & H = A.img->ham;
     const int nos    = A.nos;
-    const int i      = blockIdx.x * MC_THREADS + threadIdx.x;
-    double rejected  = 0;
-    if( i < nos )
+    double u[4];
+    mc_uniforms( A.seed, *A.step, colour, i, u );
+    const double s[3] = { A.s[i], A.s[nos + i], A.s[2 * nos + i] };
+    double sn[3];
+    if( A.step_cone )
     {
-        const int cell = fast_div( i, H.div_nb );
-        const int ib   = i - cell * H.nb;
-        const int rest = fast_div( cell, H.div_Na );
-        const int a    = cell - rest * H.Na;
-        const int c    = fast_div( rest, H.div_Nb );
-        const int b    = rest - c * H.Nb;
-        const int col  = ( A.ca * a + A.cb * b + A.cc * c ) % A.K;
-        if( col == colour )
+        // local basis (e0, e1, s) of the spin, Method_MC.cpp:116-126
+        double e0[3] = { 1, 0, 0 }, e1[3] = { 0, 1, 0 }, e2[3] = { 0, 0, 1 };
+        if( s[2] < 1 - 1e-10 )
         {
-            double u[4];
-            mc_uniforms( A.seed, *A.step, colour, i, u );
-            const double s[3] = { A.s[i], A.s[nos + i], A.s[2 * nos + i] };
-            double sn[3];
-            if( A.step_cone )
-            {
-                // local basis (e0, e1, s) of the spin, Method_MC.cpp:116-126
-                double e0[3] = { 1, 0, 0 }, e1[3] = { 0, 1, 0 }, e2[3] = { 0, 0, 1 };
-                if( s[2] < 1 - 1e-10 )
-                {
-                    const double ez[3] = { 0, 0, 1 };
-                    e2[0] = s[0], e2[1] = s[1], e2[2] = s[2];
-                    cross3( e2, ez, e0 );
-                    normalize3( e0 );
-                    cross3( e2, e0, e1 );
-                }
-                const double cos_cone = cos( *A.cone );
-                const double costheta = 1 - ( 1 - cos_cone ) * u[0];
-                const double sintheta = sqrt( 1 - costheta * costheta );
-                const double phi      = 2 * Pi * u[1];
-                const double l[3]     = { sintheta * cos( phi ), sintheta * sin( phi ), costheta };
-#pragma unroll
-                for( int q = 0; q < 3; ++q )
-                    sn[q] = e0[q] * l[0] + e1[q] * l[1] + e2[q] * l[2];
-            }
-            else
-            {
-                // Method_MC.cpp:140-149: cos(theta) drawn from [0, 1)
-                const double costheta = u[0];
-                const double sintheta = sqrt( 1 - costheta * costheta );
-                const double phi      = 2 * Pi * u[1];
-                sn[0] = sintheta * cos( phi ), sn[1] = sintheta * sin( phi ), sn[2] = costheta;
-            }
-            const double Eold  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, s );
-            const double Enew  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, sn );
-            const double Ediff = Enew - Eold;
-            bool accept        = true;
-            if( Ediff > 1e-14 )
-            {
-                if( A.temperature < 1e-12 )
-                    accept = false;
-                else if( exp( -Ediff / A.kT ) < u[2] )
-                    accept = false;
-            }
-            if( accept )
-            {
-                A.s[i] = sn[0], A.s[nos + i] = sn[1], A.s[2 * nos + i] = sn[2];
-            }
-            else
-                rejected = 1;
+            const double ez[3] = { 0, 0, 1 };
+            e2[0] = s[0], e2[1] = s[1], e2[2] = s[2];
+            cross3( e2, ez, e0 );
+            normalize3( e0 );
+            cross3( e2, e0, e1 );
         }
+        const double cos_cone = cos( *A.cone );
+        const double costheta = 1 - ( 1 - cos_cone ) * u[0];
+        const double sintheta = sqrt( 1 - costheta * costheta );
+        const double phi      = 2 * Pi * u[1];
+        const double l[3]     = { sintheta * cos( phi ), sintheta * sin( phi ), costheta };
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            sn[q] = e0[q] * l[0] + e1[q] * l[1] + e2[q] * l[2];
     }
-    // per-block rejection count in a fixed order
+    else
+    {
+        // Method_MC.cpp:140-149: cos(theta) drawn from [0, 1)
+        const double costheta = u[0];
+        const double sintheta = sqrt( 1 - costheta * costheta );
+        const double phi      = 2 * Pi * u[1];
+        sn[0] = sintheta * cos( phi ), sn[1] = sintheta * sin( phi ), sn[2] = costheta;
+    }
+    const double Eold  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, s );
+    const double Enew  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, sn );
+    const double Ediff = Enew - Eold;
+    bool accept        = true;
+    if( Ediff > 1e-14 )
+    {
+        if( A.temperature < 1e-12 )
+            accept = false;
+        else if( exp( -Ediff / A.kT ) < u[2] )
+            accept = false;
+    }
+    if( accept )
+    {
+        A.s[i] = sn[0], A.s[nos + i] = sn[1], A.s[2 * nos + i] = sn[2];
+    }
+    else
+        return 1;
+    return 0;
+}
+
+// block-wide rejection count in a fixed order into part[colour][blockIdx.x]
+__device__ __forceinline__ void mc_block_rejections( const McArgs & A, int colour, double rejected )
+{
     __shared__ double red[MC_THREADS / 64];
 #pragma unroll
     for( int off = 32; off > 0; off >>= 1 )
@@ -159,6 +152,50 @@ __global__ void __launch_bounds__( MC_THREADS ) k_mc_colour( McArgs A, int colou
             t += red[k];
         A.part[colour * A.n_blocks + blockIdx.x] = t;
     }
+}
+
+// one thread per site; the sites of other colours idle
+__global__ void __launch_bounds__( MC_THREADS ) k_mc_colour( McArgs A, int colour )
+{
+    const HamDev & H = A.img->ham;
+    const int i      = blockIdx.x * MC_THREADS + threadIdx.x;
+    double rejected  = 0;
+    if( i < A.nos )
+    {
+        const int cell = fast_div( i, H.div_nb );
+        const int ib   = i - cell * H.nb;
+        const int rest = fast_div( cell, H.div_Na );
+        const int a    = cell - rest * H.Na;
+        const int c    = fast_div( rest, H.div_Nb );
+        const int b    = rest - c * H.Nb;
+        const int col  = ( A.ca * a + A.cb * b + A.cc * c ) % A.K;
+        if( col == colour )
+            rejected = mc_site( A, colour, i, ib, a, b, c );
+    }
+    mc_block_rejections( A, colour, rejected );
+}
+
+// one thread per site of the colour (one basis atom, ca invertible mod K): along a row
+// (b, c) the colour repeats with period K, so group g of K consecutive cells holds exactly
+// one site of each colour, at a = K g + inv_ca (colour - cb b - cc c) mod K
+__global__ void __launch_bounds__( MC_THREADS ) k_mc_colour_rows( McArgs A, int colour )
+{
+    const HamDev & H = A.img->ham;
+    const int t      = blockIdx.x * MC_THREADS + threadIdx.x;
+    const int row    = t / A.groups;
+    double rejected  = 0;
+    if( row < H.Nb * H.Nc )
+    {
+        const int g  = t - row * A.groups;
+        const int c  = row / H.Nb;
+        const int b  = row - c * H.Nb;
+        const int K  = A.K;
+        const int a0 = ( ( A.inv_ca * ( colour - ( A.cb * b + A.cc * c ) % K ) ) % K + K ) % K;
+        const int a  = K * g + a0;
+        if( a < H.Na )
+            rejected = mc_site( A, colour, a + H.Na * row, 0, a, b, c );
+    }
+    mc_block_rejections( A, colour, rejected );
 }
 
 // Cone angle feedback from the rejections of the previous iteration (Method_MC.cpp:66-79)
@@ -197,7 +234,12 @@ void mc_launch_iteration( const McArgs & A, uint64_t * step, hipStream_t st )
 {
     hipLaunchKernelGGL( k_mc_cone, dim3( 1 ), dim3( 1024 ), 0, st, A );
     for( int c = 0; c < A.K; ++c )
-        hipLaunchKernelGGL( k_mc_colour, dim3( A.n_blocks ), dim3( MC_THREADS ), 0, st, A, c );
+    {
+        if( A.inv_ca > 0 )
+            hipLaunchKernelGGL( k_mc_colour_rows, dim3( A.n_blocks_rows ), dim3( MC_THREADS ), 0, st, A, c );
+        else
+            hipLaunchKernelGGL( k_mc_colour, dim3( A.n_blocks ), dim3( MC_THREADS ), 0, st, A, c );
+    }
     hipLaunchKernelGGL( k_mc_advance, dim3( 1 ), dim3( 64 ), 0, st, step );
 }
 

@@ -283,23 +283,27 @@ __global__ void __launch_bounds__( SC_THREADS ) k_min_scalar( StepArgs A, MinArg
     const int ns = ( phase == PH_RHO || phase == PH_FLIPRHO ) ? 3 : ( phase == PH_VP ? 2 : 1 );
     if( phase == PH_SCALE )
     {
-        // rms of the search direction of each image (its blocks in order), then the largest
-        double r = 0;
-        if( (int)threadIdx.x < M.noi )
+        // rms of the search direction of each image (the whole block sums its partials),
+        // then the largest
+        double rmax = 0;
+        for( int img = 0; img < M.noi; ++img )
         {
-            double ss = 0;
-            for( int k = 0; k < M.bpi; ++k )
-                ss += M.part[threadIdx.x * M.bpi + k];
-            r = sqrt( ss / nos );
-        }
-        sh[0][threadIdx.x] = r;
-        __syncthreads();
-        for( int w = blockDim.x / 2; w > 0; w >>= 1 )
-        {
-            if( (int)threadIdx.x < w )
-                sh[0][threadIdx.x] = fmax( sh[0][threadIdx.x], sh[0][threadIdx.x + w] );
+            double t = 0;
+            for( int k = threadIdx.x; k < M.bpi; k += blockDim.x )
+                t += M.part[(size_t)img * M.bpi + k];
+            sh[0][threadIdx.x] = t;
+            __syncthreads();
+            for( int w = blockDim.x / 2; w > 0; w >>= 1 )
+            {
+                if( (int)threadIdx.x < w )
+                    sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+                __syncthreads();
+            }
+            rmax = fmax( rmax, sqrt( sh[0][0] / nos ) );
             __syncthreads();
         }
+        if( threadIdx.x == 0 )
+            sh[0][0] = rmax; // every thread holds the same value; thread 0 uses it below
     }
     else
     {
