@@ -635,7 +635,7 @@ try
         LLGRunner r( { sp }, SOLVER_DEPONDT );
         const int nos      = image->nos();
         const int n_blocks = ( nos + 255 ) / 256;
-        DeviceBuffer cone( sizeof( double ) ), part( sizeof( double ) * K * (size_t)n_blocks ),
+        DeviceBuffer cone( 2 * sizeof( double ) ), part( sizeof( double ) * K * (size_t)n_blocks ),
             step( sizeof( uint64_t ) );
         const double cone0 = Pi * P.metropolis_cone_angle / 180.0; // Method_MC.cpp:34
         HIPCHECK( hipMemcpy( cone.as<void>(), &cone0, sizeof( double ), hipMemcpyHostToDevice ) );

@@ -21,7 +21,7 @@ struct McArgs
     double temperature;    // K
     double kT;             // k_B T [meV]
     double target;         // target acceptance ratio
-    double * cone;         // cone angle [rad], adapted on the device
+    double * cone;         // cone angle [rad], adapted on the device, and its cosine
     double * part;         // per-block rejection counts, [K][n_blocks]
     int n_blocks;          // blocks of the one-thread-per-site sweep (part's row length)
     int inv_ca;            // inverse of ca mod K with one basis atom, else 0: per-colour sweeps

@@ -18,8 +18,12 @@ constexpr double MC_CONE_STEP = 0.01; // Method_MC.cpp:70 (diff)
 
 // Energy_Single_Spin of spin i (basis atom ib in cell a, b, c) oriented along s, the other
 // spins as in x (redundant pair tables: each pair of the spin once, as the OpenMP build)
+// G: se holds the exchange neighbours gathered into registers (gather_exchange_neighbours,
+// zeros where a pair is cut); DMI uses them when its table coincides with exchange's
+template<bool G>
 __device__ __forceinline__ double single_spin_energy( const HamDev & H, const double * __restrict__ x, int nos, int i,
-                                                      int ib, int a, int b, int c, const double * s )
+                                                      int ib, int a, int b, int c, const double * s,
+                                                      const double ( &se )[LG_MAX_NB][3] )
 {
     double E = 0;
     if( H.has_zeeman )
@@ -32,17 +36,38 @@ __device__ __forceinline__ double single_spin_energy( const HamDev & H, const do
     for( int k = 0; k < H.n_cubic; ++k )
         if( H.cubic_idx[k] == ib )
             E -= H.cubic_K[k] / 2 * ( pow( s[0], 4.0 ) + pow( s[1], 4.0 ) + pow( s[2], 4.0 ) );
-    for( int k = 0; k < H.n_exch; ++k )
+    if constexpr( G )
     {
-        const PairDev & p = H.exch[k];
-        if( p.i != ib )
-            continue;
-        const int j = pair_neighbour( H, a, b, c, p );
-        if( j >= 0 )
+#pragma unroll
+        for( int k = 0; k < LG_MAX_NB; ++k )
+            if( k < H.n_exch )
+                E -= H.exch[k].J * dot3( s, se[k] );
+    }
+    else
+        for( int k = 0; k < H.n_exch; ++k )
         {
-            const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
-            E -= p.J * dot3( s, sj );
+            const PairDev & p = H.exch[k];
+            if( p.i != ib )
+                continue;
+            const int j = pair_neighbour( H, a, b, c, p );
+            if( j >= 0 )
+            {
+                const double sj[3] = { x[j], x[nos + j], x[2 * nos + j] };
+                E -= p.J * dot3( s, sj );
+            }
         }
+    if( G && H.dmi_same )
+    {
+#pragma unroll
+        for( int k = 0; k < LG_MAX_NB; ++k )
+            if( k < H.n_dmi )
+            {
+                const PairDev & p = H.dmi[k];
+                double cr[3];
+                cross3( s, se[k], cr );
+                E -= p.D * dot3( p.n, cr );
+            }
+        return E;
     }
     for( int k = 0; k < H.n_dmi; ++k )
     {
@@ -82,6 +107,11 @@ __device__ __forceinline__ double mc_site( McArgs A, int colour, int i, int ib, 
 {
     const HamDev & H = A.img->ham;
     const int nos    = A.nos;
+    // the neighbours' loads go out first: they do not depend on the trial move
+    const bool gather = H.n_exch <= LG_MAX_NB;
+    double se[LG_MAX_NB][3];
+    if( gather )
+        gather_exchange_neighbours( H, A.s, nos, i, ib, a, b, c, se );
     double u[4];
     mc_uniforms( A.seed, *A.step, colour, i, u );
     const double s[3] = { A.s[i], A.s[nos + i], A.s[2 * nos + i] };
@@ -98,11 +128,13 @@ __device__ __forceinline__ double mc_site( McArgs A, int colour, int i, int ib, 
             normalize3( e0 );
             cross3( e2, e0, e1 );
         }
-        const double cos_cone = cos( *A.cone );
+        const double cos_cone = A.cone[1];
         const double costheta = 1 - ( 1 - cos_cone ) * u[0];
         const double sintheta = sqrt( 1 - costheta * costheta );
         const double phi      = 2 * Pi * u[1];
-        const double l[3]     = { sintheta * cos( phi ), sintheta * sin( phi ), costheta };
+        double sp, cp;
+        sincos( phi, &sp, &cp );
+        const double l[3] = { sintheta * cp, sintheta * sp, costheta };
 #pragma unroll
         for( int q = 0; q < 3; ++q )
             sn[q] = e0[q] * l[0] + e1[q] * l[1] + e2[q] * l[2];
@@ -113,10 +145,21 @@ __device__ __forceinline__ double mc_site( McArgs A, int colour, int i, int ib, 
         const double costheta = u[0];
         const double sintheta = sqrt( 1 - costheta * costheta );
         const double phi      = 2 * Pi * u[1];
-        sn[0] = sintheta * cos( phi ), sn[1] = sintheta * sin( phi ), sn[2] = costheta;
+        double sp, cp;
+        sincos( phi, &sp, &cp );
+        sn[0] = sintheta * cp, sn[1] = sintheta * sp, sn[2] = costheta;
     }
-    const double Eold  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, s );
-    const double Enew  = single_spin_energy( H, A.s, nos, i, ib, a, b, c, sn );
+    double Eold, Enew;
+    if( gather )
+    {
+        Eold = single_spin_energy<true>( H, A.s, nos, i, ib, a, b, c, s, se );
+        Enew = single_spin_energy<true>( H, A.s, nos, i, ib, a, b, c, sn, se );
+    }
+    else
+    {
+        Eold = single_spin_energy<false>( H, A.s, nos, i, ib, a, b, c, s, se );
+        Enew = single_spin_energy<false>( H, A.s, nos, i, ib, a, b, c, sn, se );
+    }
     const double Ediff = Enew - Eold;
     bool accept        = true;
     if( Ediff > 1e-14 )
@@ -213,15 +256,19 @@ __global__ void __launch_bounds__( 1024 ) k_mc_cone( McArgs A )
             sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if( threadIdx.x != 0 || !( A.step_cone && A.adaptive ) )
+    if( threadIdx.x != 0 )
         return;
-    const double acc = 1 - sh[0] / (double)A.nos;
-    double cone      = *A.cone;
-    if( acc < A.target && cone > MC_CONE_STEP )
-        cone -= MC_CONE_STEP;
-    if( acc > A.target && cone < Pi - MC_CONE_STEP )
-        cone += MC_CONE_STEP;
-    *A.cone = cone;
+    double cone = A.cone[0];
+    if( A.step_cone && A.adaptive )
+    {
+        const double acc = 1 - sh[0] / (double)A.nos;
+        if( acc < A.target && cone > MC_CONE_STEP )
+            cone -= MC_CONE_STEP;
+        if( acc > A.target && cone < Pi - MC_CONE_STEP )
+            cone += MC_CONE_STEP;
+        A.cone[0] = cone;
+    }
+    A.cone[1] = cos( cone ); // for every trial move of the iteration
 }
 
 __global__ void k_mc_advance( uint64_t * step )
