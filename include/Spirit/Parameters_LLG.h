@@ -35,6 +35,14 @@ SPIRIT_API const char * Parameters_LLG_Get_Output_Tag( State * state, int idx_im
 /* Parameters_LLG.h:138 */
 SPIRIT_API const char * Parameters_LLG_Get_Output_Folder( State * state, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Parameters_LLG.h:141 */
+/* Parameters_LLG.h:49 (stored; energy files are not written by the HIP engine) */
+SPIRIT_API void Parameters_LLG_Set_Output_Energy( State * state, bool energy_step, bool energy_archive, bool energy_spin_resolved, bool energy_divide_by_nos, bool energy_add_readability_lines, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Parameters_LLG.h:60: spin configuration files per log step (one file each, or appended to an archive) */
+SPIRIT_API void Parameters_LLG_Set_Output_Configuration( State * state, bool configuration_step, bool configuration_archive, int configuration_filetype SPIRIT_DEFAULT( 3 ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Parameters_LLG.h:145 */
+SPIRIT_API void Parameters_LLG_Get_Output_Energy( State * state, bool * energy_step, bool * energy_archive, bool * energy_spin_resolved, bool * energy_divide_by_nos, bool * energy_add_readability_lines, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Parameters_LLG.h:150 */
+SPIRIT_API void Parameters_LLG_Get_Output_Configuration( State * state, bool * configuration_step, bool * configuration_archive, int * configuration_filetype, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 SPIRIT_API void Parameters_LLG_Get_Output_General( State * state, bool * any, bool * initial, bool * final, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Parameters_LLG.h:155 */
 SPIRIT_API void Parameters_LLG_Get_N_Iterations( State * state, int * iterations, int * iterations_log, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;

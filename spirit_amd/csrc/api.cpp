@@ -321,6 +321,23 @@ static const char * solver_name( int s )
     }
 }
 
+// Solver::SolverFullName of the reference's solvers (core/include/engine/Solver_*.hpp)
+static std::string solver_full_name( int s )
+{
+    switch( s )
+    {
+        case SOLVER_SIB: return "Semi-implicit B";
+        case SOLVER_HEUN: return "Heun";
+        case SOLVER_DEPONDT: return "Depondt";
+        case SOLVER_RK4: return "Runge Kutta (4th order)";
+        case SOLVER_VP: return "Velocity Projection";
+        case SOLVER_VP_OSO: return "Velocity Projection using exponential transforms";
+        case SOLVER_LBFGS_OSO: return "Limited memory Broyden-Fletcher-Goldfarb-Shanno using exponential transforms";
+        case SOLVER_LBFGS_ATLAS: return "Limited memory Broyden-Fletcher-Goldfarb-Shanno using stereographic atlas";
+        default: return "unknown";
+    }
+}
+
 // Finish a method: copy results to the host mirror and release the device copy.
 static void llg_finalize( MethodLLG & m )
 {
@@ -400,13 +417,25 @@ try
         return;
 
     // Method::Iterate, Method.cpp:57-113: convergence is tested every n_iterations_amortize steps,
-    // on the device; the host checks the stop requests between batches.
+    // on the device; the host checks the stop requests between batches.  With configuration
+    // files per log step, batches end on the log steps and the spins come back there.
+    const int idx_img_abs   = idx_image < 0 ? state->idx_active_image : idx_image;
+    const std::string start = current_date_time();
+    const bool step_files   = image->llg->output_any
+                            && ( image->llg->output_configuration_step || image->llg->output_configuration_archive );
+    llg_save_configuration( *image, idx_img_abs, start, 0, m->n_iterations, image->max_torque,
+                            solver_full_name( solver_type ), true, false );
     m->iteration  = 0;
     const long am = m->amortize;
     while( llg_continue( *m ) && !walltime_expired( *m ) )
     {
         long outer_left = ( m->n_iterations - m->iteration + am - 1 ) / am;
         long outer      = std::min( outer_left, std::max( 1L, 2000 / am ) );
+        if( step_files )
+        {
+            const long to_log = m->n_iterations_log - m->iteration % m->n_iterations_log;
+            outer             = std::min( outer, std::max( 1L, to_log / am ) );
+        }
         auto t0         = std::chrono::steady_clock::now();
         long done       = m->runner->run( outer * am, (int)am, true, image->llg->force_convergence );
         auto t1         = std::chrono::steady_clock::now();
@@ -417,8 +446,16 @@ try
         m->picoseconds += done * image->llg->dt;
         image->E          = m->runner->energy( 0 );
         image->max_torque = m->runner->max_torque( 0 );
+        if( step_files && m->iteration > 0 && m->iteration % m->n_iterations_log == 0 )
+        {
+            m->runner->download_spins();
+            llg_save_configuration( *image, idx_img_abs, start, m->iteration, m->n_iterations, image->max_torque,
+                                    solver_full_name( solver_type ), false, false );
+        }
     }
     llg_finalize( *m );
+    llg_save_configuration( *image, idx_img_abs, start, m->iteration, m->n_iterations, image->max_torque,
+                            solver_full_name( solver_type ), false, true );
     if( info )
     {
         info->max_torque       = (float)image->max_torque;

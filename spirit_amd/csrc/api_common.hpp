@@ -42,6 +42,10 @@ bool stop_file_present();
 // evaluated on the GPU; per_spin (nullable) receives the six per-spin terms [6][nos]
 double update_image_energy( Image & image, bool store_field, std::vector<double> * per_spin = nullptr );
 std::vector<std::pair<std::string, int>> energy_contribution_list( const Image & image );
+void llg_save_configuration( const Image & img, int idx_image, const std::string & starttime, long iteration,
+                             long n_iterations, double max_torque, const std::string & solver_full_name,
+                             bool initial, bool final );
+std::string current_date_time(); // "%Y-%m-%d_%H-%M-%S", local time
 
 // GNEB (gneb_api.cpp)
 void gneb_n_shot( State * state, int N );

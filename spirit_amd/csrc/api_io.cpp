@@ -8,8 +8,11 @@
 #include "ovf.hpp"
 
 #include <algorithm>
+#include <charconv>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <ctime>
 
 using namespace spirit_hip;
 
@@ -306,3 +309,64 @@ catch( ... )
 {
     handle_exception( 0, idx_chain );
 }
+
+// Spin configuration files of an LLG run (Method_LLG::Save_Current, Method_LLG.cpp:310-495):
+// <folder>/<tag>_Image-<ii>_Spins<suffix>.ovf with the suffixes "-initial", "-final",
+// "_<iteration>" (one file per log step) and "-archive" (one segment appended per log step).
+// The tag is the run's start time ("<time>"), the given tag, or nothing.  Energy files are
+// not written.
+void spirit_hip::llg_save_configuration( const Image & img, int idx_image, const std::string & starttime,
+                                         long iteration, long n_iterations, double max_torque,
+                                         const std::string & solver_full_name, bool initial, bool final )
+{
+    const LLGParameters & P = *img.llg;
+    if( !P.output_any )
+        return;
+    char s_img[32];
+    std::snprintf( s_img, sizeof( s_img ), "%02d", idx_image );
+    // iteration zero-padded to floor(log10(n_iterations)) digits
+    const int base = n_iterations > 0 ? (int)std::floor( std::log10( (double)n_iterations ) ) : 0;
+    char s_iter[64];
+    std::snprintf( s_iter, sizeof( s_iter ), "%0*ld", base, iteration );
+    std::string tag = P.output_file_tag == "<time>" ? starttime + "_"
+                      : P.output_file_tag.empty()   ? std::string()
+                                                    : P.output_file_tag + "_";
+    const std::string pre = P.output_folder + "/" + tag + "Image-" + s_img + "_Spins";
+    char torque[64];
+    {
+        auto r = std::to_chars( torque, torque + sizeof( torque ) - 1, max_torque );
+        *r.ptr = 0;
+    }
+    const std::string comment = "LLG simulation (" + solver_full_name + " solver)\n# Desc:      Iteration: "
+                                + std::to_string( iteration ) + "\n# Desc:      Maximum torque: " + torque;
+    auto write = [&]( const std::string & suffix, bool append ) {
+        try
+        {
+            auto seg = image_segment( img, comment.c_str(), "spin_x spin_y spin_z" );
+            ovf::write_segment( pre + suffix + ".ovf", seg, img.spins.data(),
+                                written_format( P.output_configuration_filetype ), append );
+        }
+        catch( const std::exception & e )
+        {
+            log_message( 2, std::string( "LLG output failed: " ) + e.what() );
+        }
+    };
+    if( initial && P.output_initial )
+        write( "-initial", false );
+    else if( final && P.output_final )
+        write( "-final", false );
+    if( P.output_configuration_step )
+        write( std::string( "_" ) + s_iter, false );
+    if( P.output_configuration_archive )
+        write( "-archive", true );
+}
+
+std::string spirit_hip::current_date_time()
+{
+    const std::time_t t = std::time( nullptr );
+    const std::tm tm_s  = *std::localtime( &t );
+    char b[64];
+    std::strftime( b, sizeof( b ), "%Y-%m-%d_%H-%M-%S", &tm_s );
+    return b;
+}
+

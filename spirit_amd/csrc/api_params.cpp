@@ -1001,6 +1001,22 @@ extern "C" void Parameters_LLG_Set_Output_General( State * state, bool any, bool
 {
     LLG_SET( p.output_any = any; p.output_initial = initial; p.output_final = final );
 }
+extern "C" void Parameters_LLG_Set_Output_Energy( State * state, bool energy_step, bool energy_archive,
+                                                  bool energy_spin_resolved, bool energy_divide_by_nos,
+                                                  bool energy_add_readability_lines, int idx_image,
+                                                  int idx_chain ) noexcept
+{
+    LLG_SET( p.output_energy_step = energy_step; p.output_energy_archive = energy_archive;
+             p.output_energy_spin_resolved = energy_spin_resolved; p.output_energy_divide_by_nspins = energy_divide_by_nos;
+             p.output_energy_add_readability_lines = energy_add_readability_lines );
+}
+extern "C" void Parameters_LLG_Set_Output_Configuration( State * state, bool configuration_step,
+                                                         bool configuration_archive, int configuration_filetype,
+                                                         int idx_image, int idx_chain ) noexcept
+{
+    LLG_SET( p.output_configuration_step = configuration_step; p.output_configuration_archive = configuration_archive;
+             p.output_configuration_filetype = configuration_filetype );
+}
 extern "C" void Parameters_LLG_Set_N_Iterations( State * state, int n_iterations, int n_iterations_log, int idx_image,
                                                  int idx_chain ) noexcept
 {
@@ -1066,6 +1082,38 @@ extern "C" const char * Parameters_LLG_Get_Output_Tag( State * state, int idx_im
 extern "C" const char * Parameters_LLG_Get_Output_Folder( State * state, int idx_image, int idx_chain ) noexcept
 {
     LLG_GET( "", p.output_folder.c_str() );
+}
+extern "C" void Parameters_LLG_Get_Output_Energy( State * state, bool * energy_step, bool * energy_archive,
+                                                  bool * energy_spin_resolved, bool * energy_divide_by_nos,
+                                                  bool * energy_add_readability_lines, int idx_image,
+                                                  int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    const auto & p                = *image->llg;
+    *energy_step                  = p.output_energy_step;
+    *energy_archive               = p.output_energy_archive;
+    *energy_spin_resolved         = p.output_energy_spin_resolved;
+    *energy_divide_by_nos         = p.output_energy_divide_by_nspins;
+    *energy_add_readability_lines = p.output_energy_add_readability_lines;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+extern "C" void Parameters_LLG_Get_Output_Configuration( State * state, bool * configuration_step,
+                                                         bool * configuration_archive, int * configuration_filetype,
+                                                         int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    *configuration_step     = image->llg->output_configuration_step;
+    *configuration_archive  = image->llg->output_configuration_archive;
+    *configuration_filetype = image->llg->output_configuration_filetype;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
 }
 extern "C" void Parameters_LLG_Get_Output_General( State * state, bool * any, bool * initial, bool * final,
                                                    int idx_image, int idx_chain ) noexcept

@@ -696,6 +696,14 @@ LLGParameters llg_parameters_from_config( const std::string & path )
     f.read( p.output_any, "llg_output_any" );
     f.read( p.output_initial, "llg_output_initial" );
     f.read( p.output_final, "llg_output_final" );
+    f.read( p.output_energy_step, "llg_output_energy_step" );
+    f.read( p.output_energy_archive, "llg_output_energy_archive" );
+    f.read( p.output_energy_spin_resolved, "llg_output_energy_spin_resolved" );
+    f.read( p.output_energy_divide_by_nspins, "llg_output_energy_divide_by_nspins" );
+    f.read( p.output_energy_add_readability_lines, "llg_output_energy_add_readability_lines" );
+    f.read( p.output_configuration_step, "llg_output_configuration_step" );
+    f.read( p.output_configuration_archive, "llg_output_configuration_archive" );
+    f.read( p.output_configuration_filetype, "llg_output_configuration_filetype" );
     std::string wt;
     if( f.read( wt, "llg_max_walltime" ) )
         p.max_walltime_sec = duration_from_string( wt );

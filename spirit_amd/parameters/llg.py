@@ -82,3 +82,22 @@ def get_damping(p_state, idx_image=-1, idx_chain=-1):
 
 def get_temperature(p_state, idx_image=-1, idx_chain=-1):
     return float(_Get_Temperature(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+_Set_Output_Configuration = _fn("Parameters_LLG_Set_Output_Configuration", [_vp, _b, _b, _i, _i, _i])
+_Get_Output_Configuration = _fn("Parameters_LLG_Get_Output_Configuration",
+                                [_vp, ctypes.POINTER(_b), ctypes.POINTER(_b), ctypes.POINTER(_i), _i, _i])
+
+
+def set_output_configuration(p_state, step=False, archive=True, filetype=3, idx_image=-1, idx_chain=-1):
+    """Spin configuration files per log step: one file each (`step`) and/or one archive file
+    with a segment appended per log step (`archive`), in OVF format `filetype`
+    (core/python/spirit/parameters/llg.py set_output_configuration)."""
+    _Set_Output_Configuration(_vp(p_state), _b(step), _b(archive), _i(filetype), _i(idx_image), _i(idx_chain))
+
+
+def get_output_configuration(p_state, idx_image=-1, idx_chain=-1):
+    step, archive, ft = _b(), _b(), _i()
+    _Get_Output_Configuration(_vp(p_state), ctypes.byref(step), ctypes.byref(archive), ctypes.byref(ft),
+                              _i(idx_image), _i(idx_chain))
+    return step.value, archive.value, ft.value

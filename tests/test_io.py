@@ -261,3 +261,59 @@ def test_reference_reader_and_writer_interoperate(make_state, tmp_path):
     for k in range(3):
         ref = imgs[2 - k] / np.linalg.norm(imgs[2 - k], axis=1)[:, None]
         assert np.abs(spins(q, k) - ref).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_llg_configuration_output_files(tmp_path):
+    """Method_LLG::Save_Current (Method_LLG.cpp:310-495): <tag>_Image-00_Spins-initial.ovf,
+    -final.ovf, _<iteration>.ovf per save and -archive.ovf with one segment per save (the
+    initial and final saves included), written by Simulation_LLG_Start; their spins are the
+    run's (binary files; reading renormalises, so to 1e-15)."""
+    from spirit_amd import io, simulation, state, system
+    from spirit_amd.parameters import llg
+    print("CASE llg output files", flush=True)
+    out = tmp_path / "out"
+    out.mkdir()
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[8, 6, 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+               external_field_magnitude=1.0, external_field_normal=[0, 0, 1], anisotropy_magnitude=0.1,
+               anisotropy_normal=[0, 0, 1], n_shells_exchange=1, jij=[5.0], ddi_method="none", llg_dt=1e-3,
+               llg_damping=0.3, llg_temperature=0.0, llg_force_convergence=1e-14)
+    text = configs.config_text(cfg) + "\n".join([
+        "output_file_tag run", "llg_output_folder %s" % out, "llg_output_any 1", "llg_output_initial 1",
+        "llg_output_final 1", "llg_output_configuration_step 1", "llg_output_configuration_archive 0",
+        "llg_output_configuration_filetype 0"]) + "\n"
+    path = tmp_path / "input.cfg"
+    path.write_text(text)
+    p = state.setup(str(path), True)
+    assert p
+    try:
+        assert llg.get_output_configuration(p) == (True, False, 0)
+        llg.set_output_configuration(p, True, True, 0)
+        rng = np.random.default_rng(3)
+        s0 = unit(rng, 48)
+        set_spins(p, s0)
+        simulation.start(p, simulation.METHOD_LLG, simulation.SOLVER_DEPONDT, n_iterations=30, n_iterations_log=10)
+        s_end = spins(p).copy()
+        names = sorted(f.name for f in out.iterdir())
+        # the initial and final saves write the step and archive outputs as well (iterations 0 and 30)
+        want = ["run_Image-00_Spins-archive.ovf", "run_Image-00_Spins-final.ovf", "run_Image-00_Spins-initial.ovf",
+                "run_Image-00_Spins_0.ovf", "run_Image-00_Spins_10.ovf", "run_Image-00_Spins_20.ovf",
+                "run_Image-00_Spins_30.ovf"]
+        assert names == want
+        assert io.n_images_in_file(p, str(out / "run_Image-00_Spins-archive.ovf")) == 5
+        io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), 0)
+        np.testing.assert_allclose(spins(p), s0, rtol=0, atol=1e-15)
+        io.image_read(p, str(out / "run_Image-00_Spins-initial.ovf"))
+        np.testing.assert_allclose(spins(p), s0, rtol=0, atol=1e-15)
+        for f in ("run_Image-00_Spins-final.ovf", "run_Image-00_Spins_30.ovf"):
+            io.image_read(p, str(out / f))
+            np.testing.assert_allclose(spins(p), s_end, rtol=0, atol=1e-15)
+        for k in (3, 4):
+            io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), k)
+            np.testing.assert_allclose(spins(p), s_end, rtol=0, atol=1e-15)
+        io.image_read(p, str(out / "run_Image-00_Spins_10.ovf"))
+        assert np.abs(spins(p) - s0).max() > 1e-6 and np.abs(spins(p) - s_end).max() > 1e-9
+        head = open(out / "run_Image-00_Spins_20.ovf", "rb").read(2000).decode("latin-1")
+        assert "# Desc: LLG simulation (Depondt solver)" in head and "# Desc:      Iteration: 20" in head
+    finally:
+        state.delete(p)
