@@ -6,7 +6,9 @@ convolution on a 256x256x128 simple-cubic lattice (open boundaries, padded grid
 512x512x256), J=5 meV, Bloch DMI 1 meV, per-site uniaxial K_i ~ U[0.02, 0.1] meV
 along z, B=0.2 T along z, mu_s=2, dt=5e-4 ps, alpha=0.3, T=0, FP64, random initial
 spins from a seed.  A "step" is one full Depondt iteration (two force evaluations,
-each with the FFT convolution) on the spins already resident in HBM.
+each with the FFT convolution) followed by the method's post-iteration hook (energy,
+maximum torque and effective field of every step, as Simulation_N_Shot runs it) on the
+spins already resident in HBM.
 
 Prints one JSON line: spin-steps/s, the roofline of the dominant kernel (the DDI
 convolution pass, timed live with HIP events on the engine's stream) and the CPU
@@ -60,6 +62,30 @@ def alg_bytes_per_spin_step(dims, solver, ddi):
     C = 96 * H_PN + 96 * H_PP
     per_fe = (2 * A + 2 * B + C) / N
     return local + n_fe * (per_fe + 24.0), C
+
+
+def ddi_bytes_per_fe(dims):
+    """Algorithmic bytes per spin of one DDI convolution (2A + 2B + C, without the stage's read
+    of the field): pack + R2C along a, C2C along b, the c pass with the kernel multiply, and
+    the inverse passes."""
+    Na, Nb, Nc = dims
+    Pa, Pb, Pc = 2 * Na if Na > 1 else 1, 2 * Nb if Nb > 1 else 1, 2 * Nc if Nc > 1 else 1
+    Ha = Pa // 2 + 1
+    N = Na * Nb * Nc
+    H_NN, H_PN, H_PP = Ha * Nb * Nc, Ha * Pb * Nc, Ha * Pb * Pc
+    return (2 * (24 * N + 48 * H_NN) + 2 * (48 * H_NN + 48 * H_PN) + 96 * H_PN + 96 * H_PP) / N
+
+
+def adjoint_bytes_per_spin_step(dims, ckpt):
+    """Algorithmic bytes per spin and step of the differentiable Depondt+DDI run (forward with
+    checkpoints, then the backward pass), 24 B per vector field read or written:
+    forward 216 + 2 DDI (the plain step) + 24/ckpt (checkpoint);
+    recompute 312 + 2 DDI (the plain step's 9 field accesses plus the tape: F1, fv1, F2, fv2 and
+    the step's spins) + 48/ckpt (the segment's checkpoint copied to the tape);
+    adjoint 768 + 2 DDI (k_adj_predictor 3, k_adj_final 10, k_adj_predictor_back 9 and two
+    k_adj_hessian 5 field accesses, and the two transposed convolutions)."""
+    ddi = ddi_bytes_per_fe(dims)
+    return 216 + 312 + 768 + 6 * ddi + 72.0 / ckpt
 
 
 def run_gpu(args, wl):
@@ -158,11 +184,16 @@ def run_batched(n_steps, n_replicas, sequential=True):
     done = all(i.total_iterations == n_steps for i in infos)
     for p in states:
         state.delete(p)
+    value = nos * n_steps / t_bat
+    ach = 168.0 * value / 1e9
     return {"workload": "ensemble%d_depondt" % n_replicas, "replicas": n_replicas, "spins": nos, "steps": n_steps,
-            "value": nos * n_steps / t_bat, "sequential_value": nos * n_steps / t_seq if t_seq else None,
+            "value": value, "sequential_value": nos * n_steps / t_seq if t_seq else None,
             "unit": "spin-steps/s", "speedup_vs_sequential": t_seq / t_bat if t_seq else None, "batched_s": t_bat,
             "sequential_s": t_seq,
-            "all_replicas_completed": done}
+            "all_replicas_completed": done,
+            "roofline": {"bound": "hbm", "alg_bytes_per_spin_step": 168.0, "achieved": ach, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                         "timed": "the whole Spirit_HIP_Simulation_LLG_Start_Batch call (set-up, copies, run)"}}
 
 
 def run_adjoint(n_steps, ckpt, plain_ms_per_step):
@@ -206,8 +237,13 @@ def run_adjoint(n_steps, ckpt, plain_ms_per_step):
     sess.close()
     state.delete(p)
     plain = plain_ms_per_step * 1e-3 * n_steps if plain_ms_per_step else None
+    b_alg = adjoint_bytes_per_spin_step((256, 256, 128), ckpt)
+    value = nos * n_steps / (fwd + bwd)
+    ach = b_alg * value / 1e9
     return {"workload": "cube256_depondt_ddi_adjoint", "steps": n_steps, "checkpoint_every": ckpt, "spins": nos,
-            "value": nos * n_steps / (fwd + bwd), "unit": "spin-steps/s (forward + backward)",
+            "value": value, "unit": "spin-steps/s (forward + backward)",
+            "roofline": {"bound": "hbm", "alg_bytes_per_spin_step": b_alg, "achieved": ach, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": ach / HBM_PEAK_GBS},
             "forward_s": fwd, "backward_s": bwd, "backward_over_forward": bwd / fwd,
             "gradient_over_plain_forward": (fwd + bwd) / plain if plain else None,
             "loss": loss, "dL_dJ": float(g["J"][0]), "dL_dD": float(g["D"][0]), "dL_dB": float(g["B"][0])}
@@ -338,14 +374,17 @@ def measured_traffic(kernel):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(wl, threads, steps):
+def cpu_baseline(wl, threads, steps, fft_parallel=True):
     """The CPU oracle (C + OpenMP restatement of the reference algorithm) on the host cores,
-    timed on `steps` steps of the same workload after its set-up."""
+    timed on `steps` steps of the workload `wl` after its set-up.  fft_parallel=False runs
+    each 3-D transform on one thread, as the reference's kissFFT build does (the rest of the
+    step still uses every thread, as the reference's OpenMP loops do)."""
     code = r"""
 import sys, time, json
 sys.path[:0] = [%r, %r]
 import oracle as O
 from spirit_amd import configs
+O.set_fft_parallel(%r)
 cfg = dict(getattr(configs, %r)); n = %d
 K = configs.site_anisotropy(n, 20006) if %r else None
 S = O.System(cfg, site_K=K)
@@ -353,13 +392,44 @@ s = configs.random_spins(n, 20006)
 t0 = time.perf_counter()
 s, e, t, _ = S.llg(s, %d, %d, dt=cfg["llg_dt"], damping=cfg["llg_damping"], temperature=cfg["llg_temperature"], seed=cfg["llg_seed"])
 print(json.dumps({"seconds": time.perf_counter() - t0}))
-""" % (ROOT, os.path.join(ROOT, "oracle"), wl["cfg"], wl["dims"][0] * wl["dims"][1] * wl["dims"][2], wl["site_K"],
-       wl["solver"], steps)
+""" % (ROOT, os.path.join(ROOT, "oracle"), bool(fft_parallel), wl["cfg"], wl["dims"][0] * wl["dims"][1] * wl["dims"][2],
+       wl["site_K"], wl["solver"], steps)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=1200)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-2000:])
     return json.loads(r.stdout.strip().splitlines()[-1])["seconds"]
+
+
+def host_threads():
+    """Threads for the CPU baseline: the CPUs this process may run on, at most 16 (the host's
+    share for one GPU), and the machine's count as reported."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return min(16, usable), usable, os.cpu_count() or 1
+
+
+def cpu_baselines(main_wl, main_steps, threads):
+    """The CPU baselines, each timed on the host cores with `threads` OpenMP threads:
+    the headline workload (main_steps steps, threaded FFT: a reference-faithful serial-FFT step
+    of the 512x512x256 grid takes over a minute); the 64x64 Depondt monolayer (2000 steps) and
+    the 128x128x32 Heun+DDI film (1 step) with each transform on one thread, as the reference's
+    kissFFT build runs them (the reference-faithful baseline), and the film once more with the
+    threaded FFT."""
+    out = {}
+    sec = cpu_baseline(main_wl, threads, main_steps, True)
+    n = main_wl["dims"][0] * main_wl["dims"][1] * main_wl["dims"][2]
+    out["main"] = (n * main_steps / sec, sec)
+    for key, wl, steps, par in (("square64_depondt_2000", WORKLOADS["square64_depondt"], 2000, False),
+                                ("film128_heun_ddi_serial_fft", WORKLOADS["film128_heun_ddi"], 1, False),
+                                ("film128_heun_ddi_threaded_fft", WORKLOADS["film128_heun_ddi"], 1, True)):
+        n = wl["dims"][0] * wl["dims"][1] * wl["dims"][2]
+        sec = cpu_baseline(wl, threads, steps, par)
+        out[key] = {"value": n * steps / sec, "unit": "spin-steps/s", "steps": steps, "seconds": sec,
+                    "fft": "threaded" if par else "serial per transform (reference-faithful)"}
+    return out
 
 
 def cpu_model():
@@ -455,12 +525,19 @@ def main():
     if "mc" in legs:
         out["mc"] = run_mc(max(1, args.mc_sweeps // 5), args.mc_sweeps + max(1, args.mc_sweeps // 5))
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads, usable, nproc = host_threads()
         try:
-            sec = cpu_baseline(wl, threads, args.cpu_steps)
-            out["cpu_baseline"] = {"value": g["nos"] * args.cpu_steps / sec, "unit": "spin-steps/s", "cores": threads,
-                                   "kind": "port", "sample": "%d step(s) of the same workload after set-up, CPU %s" % (
-                                       args.cpu_steps, cpu_model())}
+            cb = cpu_baselines(wl, args.cpu_steps, threads)
+            value, _ = cb.pop("main")
+            out["cpu_baseline"] = {
+                "value": value, "unit": "spin-steps/s", "cores": threads, "kind": "port",
+                "sample": "%d step(s) of the same workload after set-up, FFT lines threaded; %d OpenMP threads "
+                          "(OMP_PROC_BIND=close) of %d usable CPUs (%d reported), CPU %s" % (
+                              args.cpu_steps, threads, usable, nproc, cpu_model()),
+                "reference_faithful": cb,
+                "reference_faithful_note": "the oracle with each FFT on one thread, as the reference's kissFFT "
+                                           "build runs its transforms; the baseline for the 64x64 and 128x128x32 "
+                                           "configurations"}
         except Exception as e:  # reported, never fatal for the GPU number
             out["cpu_baseline"] = {"value": None, "unit": "spin-steps/s", "cores": threads, "kind": "port",
                                    "sample": "failed: %s" % str(e)[-300:]}

@@ -33,7 +33,7 @@ typedef struct Simulation_Run_Info
 
 /* Simulation.h:73 */
 SPIRIT_API void free_run_info( Simulation_Run_Info info ) SPIRIT_NOEXCEPT;
-/* Simulation.h:81 (not part of the HIP engine: logs an error) */
+/* Simulation.h:81 (Metropolis sweeps of colour classes on the GPU; one basis atom; no single shot) */
 SPIRIT_API void Simulation_MC_Start( State * state, int n_iterations SPIRIT_DEFAULT( -1 ), int n_iterations_log SPIRIT_DEFAULT( -1 ), bool singleshot SPIRIT_DEFAULT( false ), Simulation_Run_Info * info SPIRIT_DEFAULT( 0 ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Simulation.h:86 */
 SPIRIT_API void Simulation_LLG_Start( State * state, int solver_type, int n_iterations SPIRIT_DEFAULT( -1 ), int n_iterations_log SPIRIT_DEFAULT( -1 ), bool singleshot SPIRIT_DEFAULT( false ), Simulation_Run_Info * info SPIRIT_DEFAULT( 0 ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;

@@ -132,6 +132,7 @@ def lib():
         L.orc_llg_run.restype = ctypes.c_long
         L.orc_normal3.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(d)]
         L.orc_fft.argtypes = [P(d), i, i]
+        L.orc_set_fft_parallel.argtypes = [i]
         L.orc_rotate.argtypes = [P(d), P(d), d, P(d)]
         L.orc_sib_transform.argtypes = [P(d), P(d), P(d)]
         L.orc_gneb_run.argtypes = [vp, i, P(OrcGNEB), i, P(i), P(d), ctypes.c_long, i, P(d), P(d), P(d), P(d), P(d),
@@ -407,6 +408,12 @@ def fft(x, sign=-1):
     a = np.ascontiguousarray(x, dtype=np.complex128).copy()
     lib().orc_fft(a.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(a), sign)
     return a
+
+
+def set_fft_parallel(on):
+    """True: the lines of each 3-D transform run on all OpenMP threads; False: each transform
+    runs on one thread, as the reference's kissFFT build does."""
+    lib().orc_set_fft_parallel(1 if on else 0)
 
 
 def normal3(seed, step, site):

@@ -472,6 +472,16 @@ void orc_fft( double * data, int n, int sign )
     free( scr );
 }
 
+/* Whether the lines of a 3-D transform are spread over the OpenMP threads (1, the default)
+   or run on one thread, as the reference's kissFFT build transforms (kiss_fftndr runs each
+   transform serially; the reference's batch_Four_3D loops over the transforms in turn). */
+static int g_fft_parallel = 1;
+
+void orc_set_fft_parallel( int on )
+{
+    g_fft_parallel = on ? 1 : 0;
+}
+
 /* 3-D complex FFT over grid [c][b][a] (a fastest), `count` consecutive arrays of M points. */
 static void fft3d( cpx * data, int count, const int dims[3], int sign )
 {
@@ -485,7 +495,7 @@ static void fft3d( cpx * data, int count, const int dims[3], int sign )
         cpx * W      = make_roots( n, sign );
         long stride  = axis == 0 ? 1 : ( axis == 1 ? Pa : (long)Pa * Pb );
         long n_lines = M / n;
-#pragma omp parallel
+#pragma omp parallel if( g_fft_parallel )
         {
             cpx * buf = (cpx *)malloc( sizeof( cpx ) * (size_t)n * 3 );
             cpx * out = buf + n;

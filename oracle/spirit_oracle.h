@@ -201,6 +201,9 @@ void orc_homogeneous_rotation( double * chain, int nos, int idx_1, int idx_2 );
 /* ---------------- FFT (used by the DDI convolution) ---------------- */
 /* In-place complex DFT of length n (interleaved re,im), sign -1 forward, +1 backward, unnormalised. */
 void orc_fft( double * data, int n, int sign );
+/* 1: the lines of each 3-D transform run on all OpenMP threads; 0: on one thread (the
+   reference's kissFFT build transforms serially) */
+void orc_set_fft_parallel( int on );
 
 /* ---------------- utilities ---------------- */
 /* Rodrigues rotation, core/src/engine/Vectormath.cpp:474-479 */

@@ -79,6 +79,8 @@ class Session:
         _Get_Sizes(self.h, ctypes.byref(nos), ctypes.byref(ne), ctypes.byref(nd), ctypes.byref(na))
         self.nos, self.n_exchange, self.n_dmi, self.n_aniso = nos.value, ne.value, nd.value, na.value
         self.n_steps = n_steps
+        # counts forward runs: backward differentiates the last one (its checkpoints)
+        self.generation = 0
 
     def close(self):
         if self.h:
@@ -146,6 +148,7 @@ class Session:
         out = self._out_for(s0, dev)
         if _Forward(self.h, p, self._ptr(out), dev) != 0:
             raise RuntimeError("Spirit_HIP_Adjoint_Forward failed")
+        self.generation += 1
         del keep
         return out
 
@@ -191,11 +194,22 @@ def _torch_function():
                 sN = torch.from_numpy(sN)
             ctx.session = session
             ctx.devices = (s0.device, J.device)
+            # what this node ran, so that its backward differentiates this run even when the
+            # session has run forward again since (two applications before one backward)
+            ctx.generation = session.generation
+            ctx.s0 = s0.detach().clone()
+            ctx.params = tuple(x.detach().clone() for x in (J, D, B, K_site))
             return sN
 
         @staticmethod
         def backward(ctx, grad_sN):
-            g = ctx.session.backward(grad_sN.contiguous())
+            session = ctx.session
+            if session.generation != ctx.generation:
+                # the checkpoints belong to a later forward: record this node's run again
+                J, D, B, K_site = ctx.params
+                session.set_parameters(J=J, D=D, B=B, K_site=K_site)
+                session.forward(ctx.s0)
+            g = session.backward(grad_sN.contiguous())
             ds0 = g["s0"] if torch.is_tensor(g["s0"]) else torch.from_numpy(g["s0"])
             dev = ctx.devices[1]
             t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)

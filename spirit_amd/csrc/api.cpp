@@ -366,6 +366,89 @@ static bool walltime_expired( const MethodLLG & m )
     return s > (double)m.image->llg->max_walltime_sec;
 }
 
+// A run that failed part-way: the image is free again and the device copy is dropped
+// (its spins stay as the host mirror last held them).
+static void llg_abandon( MethodLLG & m )
+{
+    m.runner.reset();
+    m.device_newer              = false;
+    m.image->iteration_allowed  = false;
+    m.image->singleshot_allowed = false;
+}
+
+// Method_LLG::Save_Current (Method_LLG.cpp:310-495): the history entry, then the configuration
+// and energy files the output parameters ask for.  The host mirror must hold the spins.
+static void llg_save_current( MethodLLG & m, long iteration, bool initial, bool final )
+{
+    m.history_iteration.push_back( (int)iteration );
+    m.history_max_torque.push_back( (float)m.image->max_torque );
+    m.history_energy.push_back( (float)m.image->E );
+    if( !m.image->llg->output_any )
+        return;
+    llg_save_configuration( *m.image, m.idx_image, m.starttime, iteration, m.n_iterations, m.image->max_torque,
+                            solver_full_name( m.solver ), initial, final );
+    llg_save_energy( *m.image, m.idx_image, m.starttime, iteration, m.n_iterations, initial, final );
+}
+
+// LLG dynamics the engine does not evaluate refuse to start rather than run other physics.
+static void check_llg_supported( const LLGParameters & P, int solver )
+{
+    const bool dynamics = !P.direct_minimization && ( solver == SOLVER_SIB || solver == SOLVER_DEPONDT
+                                                      || solver == SOLVER_HEUN || solver == SOLVER_RK4 );
+    if( dynamics && P.stt_magnitude > 0 )
+        throw api_error( 2, "Simulation_LLG_Start: spin-transfer torque is not available in the HIP engine" );
+    if( dynamics && P.temperature_gradient_inclination != 0 )
+        throw api_error( 2, "Simulation_LLG_Start: temperature gradients are not available in the HIP engine" );
+}
+
+// Whether the step files of the output parameters are written at log steps.
+static bool llg_step_files( const LLGParameters & P )
+{
+    return P.output_any
+           && ( P.output_configuration_step || P.output_configuration_archive || P.output_energy_step
+                || P.output_energy_archive );
+}
+
+// The iteration loop of Method::Iterate (Method.cpp:71-102) saves after the body that starts
+// at iteration `it` when it > 0 and it is a multiple of n_iterations_log.  Of the next
+// `chunks` bodies of `am` steps from `iteration` on, the number up to and including the first
+// such body (chunks if there is none).
+static long chunks_to_log_body( long iteration, long chunks, long am, long log )
+{
+    for( long k = 1; k <= chunks; ++k )
+    {
+        const long it = iteration + ( k - 1 ) * am;
+        if( it > 0 && it % log == 0 )
+            return k;
+    }
+    return chunks;
+}
+
+static void fill_run_info( Simulation_Run_Info * info, const MethodLLG & m )
+{
+    if( !info )
+        return;
+    info->max_torque       = (float)m.image->max_torque;
+    info->total_iterations = (int)m.iteration;
+    double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - m.t_start ).count();
+    info->total_walltime = (int)ms;
+    info->total_ips      = ms > 0 ? float( m.iteration / ms * 1000.0 ) : 0.f;
+    // Simulation.cpp:49-70: arrays owned by the caller, released by free_run_info
+    const int n = (int)m.history_iteration.size();
+    if( n > 0 )
+    {
+        info->n_history_iteration  = n;
+        info->history_iteration    = new int[n];
+        info->n_history_max_torque = n;
+        info->history_max_torque   = new float[n];
+        info->n_history_energy     = n;
+        info->history_energy       = new float[n];
+        std::copy( m.history_iteration.begin(), m.history_iteration.end(), info->history_iteration );
+        std::copy( m.history_max_torque.begin(), m.history_max_torque.end(), info->history_max_torque );
+        std::copy( m.history_energy.begin(), m.history_energy.end(), info->history_energy );
+    }
+}
+
 extern "C" void Simulation_LLG_Start( State * state, int solver_type, int n_iterations, int n_iterations_log,
                                       bool singleshot, Simulation_Run_Info * info, int idx_image,
                                       int idx_chain ) noexcept
@@ -377,6 +460,7 @@ try
     if( image->iteration_allowed || chain->iteration_allowed )
         throw api_error( 3, "Tried to use Simulation_Start on image " + std::to_string( idx_image ) + " of chain "
                                 + std::to_string( idx_chain ) + ", but there is already a simulation running." );
+    check_llg_supported( *image->llg, solver_type );
     auto m = std::make_shared<MethodLLG>();
     {
         std::lock_guard<std::recursive_mutex> lk( image->mtx );
@@ -413,57 +497,61 @@ try
         m->t_start        = std::chrono::steady_clock::now();
     }
     state->method_image[idx_image] = m;
-    if( singleshot )
-        return;
+    m->idx_image = idx_image < 0 ? state->idx_active_image : idx_image;
+    m->starttime = current_date_time();
+    m->iteration = 0;
+    try
+    {
+        // initial save, also in single-shot mode (Simulation.cpp:24-38)
+        llg_save_current( *m, 0, true, false );
+        if( singleshot )
+            return;
 
-    // Method::Iterate, Method.cpp:57-113: convergence is tested every n_iterations_amortize steps,
-    // on the device; the host checks the stop requests between batches.  With configuration
-    // files per log step, batches end on the log steps and the spins come back there.
-    const int idx_img_abs   = idx_image < 0 ? state->idx_active_image : idx_image;
-    const std::string start = current_date_time();
-    const bool step_files   = image->llg->output_any
-                            && ( image->llg->output_configuration_step || image->llg->output_configuration_archive );
-    llg_save_configuration( *image, idx_img_abs, start, 0, m->n_iterations, image->max_torque,
-                            solver_full_name( solver_type ), true, false );
-    m->iteration  = 0;
-    const long am = m->amortize;
-    while( llg_continue( *m ) && !walltime_expired( *m ) )
-    {
-        long outer_left = ( m->n_iterations - m->iteration + am - 1 ) / am;
-        long outer      = std::min( outer_left, std::max( 1L, 2000 / am ) );
-        if( step_files )
+        // Method::Iterate, Method.cpp:57-113: n_iterations_amortize steps per body, then
+        // Hook_Post_Iteration (energy, max torque and the convergence test, evaluated on the
+        // device after every body).  The host submits batches of bodies and checks the stop
+        // requests (Simulation_Stop, the STOP file, the wall time) between them; a batch lasts
+        // about 0.1 s.  With step files, a batch ends on the next log body and the spins come back.
+        const long am       = m->amortize;
+        const long log      = m->n_iterations_log;
+        const bool step_out = llg_step_files( *image->llg );
+        long chunks         = std::max( 1L, 16 / am );
+        while( llg_continue( *m ) && !walltime_expired( *m ) )
         {
-            const long to_log = m->n_iterations_log - m->iteration % m->n_iterations_log;
-            outer             = std::min( outer, std::max( 1L, to_log / am ) );
+            const long left = ( m->n_iterations - m->iteration + am - 1 ) / am;
+            long outer      = std::min( left, chunks );
+            if( step_out )
+                outer = chunks_to_log_body( m->iteration, outer, am, log );
+            auto t0    = std::chrono::steady_clock::now();
+            long done  = m->runner->run( outer * am, (int)am, true, image->llg->force_convergence );
+            auto t1    = std::chrono::steady_clock::now();
+            double sec = std::chrono::duration<double>( t1 - t0 ).count();
+            if( sec > 0 && done > 0 )
+            {
+                m->ips          = done / sec;
+                const double pb = sec / (double)( ( done + am - 1 ) / am ); // seconds per body
+                chunks          = std::max( 1L, std::min( 1L << 20, (long)( 0.1 / pb ) ) );
+            }
+            m->iteration += done;
+            m->picoseconds += done * image->llg->dt;
+            image->E          = m->runner->energy( 0 );
+            image->max_torque = m->runner->max_torque( 0 );
+            const long it     = m->iteration - am; // first iteration of the last body
+            if( step_out && done > 0 && it > 0 && it % log == 0 )
+            {
+                m->runner->download_spins();
+                llg_save_current( *m, it, false, false );
+            }
         }
-        auto t0         = std::chrono::steady_clock::now();
-        long done       = m->runner->run( outer * am, (int)am, true, image->llg->force_convergence );
-        auto t1         = std::chrono::steady_clock::now();
-        double sec      = std::chrono::duration<double>( t1 - t0 ).count();
-        if( sec > 0 )
-            m->ips = done / sec;
-        m->iteration += done;
-        m->picoseconds += done * image->llg->dt;
-        image->E          = m->runner->energy( 0 );
-        image->max_torque = m->runner->max_torque( 0 );
-        if( step_files && m->iteration > 0 && m->iteration % m->n_iterations_log == 0 )
-        {
-            m->runner->download_spins();
-            llg_save_configuration( *image, idx_img_abs, start, m->iteration, m->n_iterations, image->max_torque,
-                                    solver_full_name( solver_type ), false, false );
-        }
+        llg_finalize( *m );
     }
-    llg_finalize( *m );
-    llg_save_configuration( *image, idx_img_abs, start, m->iteration, m->n_iterations, image->max_torque,
-                            solver_full_name( solver_type ), false, true );
-    if( info )
+    catch( ... )
     {
-        info->max_torque       = (float)image->max_torque;
-        info->total_iterations = (int)m->iteration;
-        double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - m->t_start ).count();
-        info->total_walltime = (int)ms;
-        info->total_ips      = ms > 0 ? float( m->iteration / ms * 1000.0 ) : 0.f;
+        llg_abandon( *m );
+        throw;
     }
+    llg_save_current( *m, m->iteration, false, true );
+    fill_run_info( info, *m );
 }
 catch( ... )
 {
@@ -498,6 +586,7 @@ try
         for( int j = 0; j < k; ++j )
             if( E[j].image == E[k].image )
                 throw api_error( 3, "Spirit_HIP_Simulation_LLG_Start_Batch: an image appears twice" );
+        check_llg_supported( *E[k].image->llg, solver_type );
     }
     std::vector<ImageSpec> specs;
     long am = 1;
@@ -540,40 +629,54 @@ try
         E[k].image->E          = runner->energy( k );
         E[k].image->max_torque = runner->max_torque( k );
     }
-    // Method::Iterate per image; convergence is tested on the device every am steps
-    while( true )
+    // Method::Iterate per image; convergence is tested on the device every am steps.  Batches
+    // last about 0.1 s, so stop requests are seen at least that often.
+    long chunks = std::max( 1L, 16 / am );
+    try
     {
-        long batch = 0;
-        for( int k = 0; k < n_states; ++k )
+        while( true )
         {
-            auto & img = *E[k].image;
-            const bool walltime
-                = img.llg->max_walltime_sec > 0
-                  && std::chrono::duration<double>( std::chrono::steady_clock::now() - t_start ).count()
-                         > (double)img.llg->max_walltime_sec;
-            E[k].active = E[k].active && E[k].iteration < E[k].n_iterations && img.iteration_allowed
-                          && !stop_file_present() && !( img.max_torque < img.llg->force_convergence ) && !walltime;
-            runner->set_image_stopped( k, !E[k].active );
-            if( E[k].active )
+            long batch = 0;
+            for( int k = 0; k < n_states; ++k )
             {
-                long left = ( E[k].n_iterations - E[k].iteration + am - 1 ) / am * am;
-                batch     = batch == 0 ? left : std::min( batch, left );
+                auto & img = *E[k].image;
+                const bool walltime
+                    = img.llg->max_walltime_sec > 0
+                      && std::chrono::duration<double>( std::chrono::steady_clock::now() - t_start ).count()
+                             > (double)img.llg->max_walltime_sec;
+                E[k].active = E[k].active && E[k].iteration < E[k].n_iterations && img.iteration_allowed
+                              && !stop_file_present() && !( img.max_torque < img.llg->force_convergence ) && !walltime;
+                runner->set_image_stopped( k, !E[k].active );
+                if( E[k].active )
+                {
+                    long left = ( E[k].n_iterations - E[k].iteration + am - 1 ) / am * am;
+                    batch     = batch == 0 ? left : std::min( batch, left );
+                }
+            }
+            if( batch == 0 )
+                break;
+            batch       = std::min( batch, chunks * am );
+            auto t0     = std::chrono::steady_clock::now();
+            long done   = runner->run( batch, (int)am, true, 0.0 );
+            double sec  = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
+            if( sec > 0 && done > 0 )
+                chunks = std::max( 1L, std::min( 1L << 20, (long)( 0.1 / ( sec / (double)( ( done + am - 1 ) / am ) ) ) ) );
+            for( int k = 0; k < n_states; ++k )
+            {
+                E[k].iteration += runner->steps_last_run( k );
+                E[k].image->E          = runner->energy( k );
+                E[k].image->max_torque = runner->max_torque( k );
             }
         }
-        if( batch == 0 )
-            break;
-        batch = std::min( batch, std::max( 1L, 2000 / am ) * am );
-        runner->run( batch, (int)am, true, 0.0 );
+        runner->download_spins();
         for( int k = 0; k < n_states; ++k )
-        {
-            E[k].iteration += runner->steps_last_run( k );
-            E[k].image->E          = runner->energy( k );
-            E[k].image->max_torque = runner->max_torque( k );
-        }
+            runner->download_effective_field( k, E[k].image->eff_field.data() );
     }
-    runner->download_spins();
-    for( int k = 0; k < n_states; ++k )
-        runner->download_effective_field( k, E[k].image->eff_field.data() );
+    catch( ... )
+    {
+        release();
+        throw;
+    }
     release();
     double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t_start ).count();
     if( info )
@@ -781,21 +884,51 @@ try
         throw api_error( 3, "Tried to use Simulation_SingleShot on image " + std::to_string( idx_image )
                                 + " but no SingleShot simulation has been started." );
     auto m = state->method_image[idx_image];
-    if( llg_continue( *m ) && !walltime_expired( *m ) )
+    try
     {
-        std::lock_guard<std::recursive_mutex> lk( image->mtx );
-        long n = std::max( 0, N );
-        m->runner->run( n, 0, false, image->llg->force_convergence );
-        m->iteration += n;
-        m->picoseconds += n * image->llg->dt;
-        image->E          = m->runner->energy( 0 );
-        image->max_torque = m->runner->max_torque( 0 );
-        m->device_newer   = true;
-        if( state->host_sync )
-            sync_image_host( state, idx_image );
+        if( llg_continue( *m ) && !walltime_expired( *m ) )
+        {
+            // Simulation.cpp:477-504: N iterations, each followed by Hook_Post_Iteration (energy,
+            // max torque and effective field of every step; convergence is only tested after the
+            // N steps), saving after the iteration it when it > 0 is a multiple of n_iterations_log
+            std::lock_guard<std::recursive_mutex> lk( image->mtx );
+            const bool step_out = llg_step_files( *image->llg );
+            long left           = std::max( 0, N );
+            while( left > 0 )
+            {
+                const long n = step_out ? chunks_to_log_body( m->iteration, left, 1, m->n_iterations_log ) : left;
+                auto t0      = std::chrono::steady_clock::now();
+                m->runner->run( n, 1, false, image->llg->force_convergence );
+                double sec = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
+                if( sec > 0 )
+                    m->ips = n / sec;
+                m->iteration += n;
+                m->picoseconds += n * image->llg->dt;
+                image->E          = m->runner->energy( 0 );
+                image->max_torque = m->runner->max_torque( 0 );
+                m->device_newer   = true;
+                left -= n;
+                const long it = m->iteration - 1;
+                if( step_out && it > 0 && it % m->n_iterations_log == 0 )
+                {
+                    sync_image_host( state, idx_image );
+                    llg_save_current( *m, it, false, false );
+                }
+            }
+            if( state->host_sync )
+                sync_image_host( state, idx_image );
+        }
+        if( !llg_continue( *m ) || walltime_expired( *m ) )
+        {
+            llg_finalize( *m );
+            llg_save_current( *m, m->iteration, false, true );
+        }
     }
-    if( !llg_continue( *m ) || walltime_expired( *m ) )
-        llg_finalize( *m );
+    catch( ... )
+    {
+        llg_abandon( *m );
+        throw;
+    }
 }
 catch( ... )
 {

@@ -45,6 +45,10 @@ std::vector<std::pair<std::string, int>> energy_contribution_list( const Image &
 void llg_save_configuration( const Image & img, int idx_image, const std::string & starttime, long iteration,
                              long n_iterations, double max_torque, const std::string & solver_full_name,
                              bool initial, bool final );
+// the energy files of an LLG save (and, spin-resolved, the per-spin energies, which update the
+// image's energy contributions)
+void llg_save_energy( Image & img, int idx_image, const std::string & starttime, long iteration, long n_iterations,
+                      bool initial, bool final );
 std::string current_date_time(); // "%Y-%m-%d_%H-%M-%S", local time
 
 // GNEB (gneb_api.cpp)

@@ -361,6 +361,180 @@ void spirit_hip::llg_save_configuration( const Image & img, int idx_image, const
         write( "-archive", true );
 }
 
+namespace
+{
+
+// fmt's "{:^w}": centred, the odd padding character on the right
+std::string centred( const std::string & s, int w )
+{
+    const int pad = w - (int)s.size();
+    if( pad <= 0 )
+        return s;
+    return std::string( pad / 2, ' ' ) + s + std::string( pad - pad / 2, ' ' );
+}
+
+std::string fixed10( double v )
+{
+    char b[64];
+    std::snprintf( b, sizeof( b ), "%.10f", v );
+    return b;
+}
+
+void put_text( const std::string & text, const std::string & path, bool append )
+{
+    std::FILE * f = std::fopen( path.c_str(), append ? "ab" : "wb" );
+    if( !f )
+    {
+        log_message( 2, "Unable to write to file \"" + path + "\"" );
+        return;
+    }
+    std::fwrite( text.data(), 1, text.size(), f );
+    std::fclose( f );
+}
+
+// IO::Write_Energy_Header (Datawriter.cpp:116-160) with the columns "iteration", "E_tot"
+std::string energy_header( const Image & img, bool readability )
+{
+    std::string separator, line;
+    for( const char * column : { "iteration", "E_tot" } )
+    {
+        if( readability )
+            separator += "----------------------++";
+        line += " " + centred( column, 20 ) + " ||";
+    }
+    bool first = true;
+    for( const auto & pair : img.E_array )
+    {
+        if( first )
+            first = false;
+        else
+        {
+            line += "|";
+            if( readability )
+                separator += "+";
+        }
+        line += " " + centred( pair.first, 20 ) + " ";
+        if( readability )
+            separator += "----------------------";
+    }
+    line += "\n";
+    separator += "\n";
+    std::string header = readability ? separator + line + separator : line;
+    if( !readability )
+        std::replace( header.begin(), header.end(), '|', ' ' );
+    return header;
+}
+
+// IO::Append_Image_Energy (Datawriter.cpp:162-185)
+std::string energy_line( const Image & img, long iteration, bool normalize, bool readability )
+{
+    const double norm = normalize ? 1.0 / (double)img.nos() : 1.0;
+    std::string line  = " " + centred( std::to_string( iteration ), 20 ) + " || "
+                       + centred( fixed10( img.E * norm ), 20 ) + " |";
+    for( const auto & pair : img.E_array )
+        line += "| " + centred( fixed10( pair.second * norm ), 20 ) + " ";
+    line += "\n";
+    if( !readability )
+        std::replace( line.begin(), line.end(), '|', ' ' );
+    return line;
+}
+
+} // namespace
+
+// The energy files of Method_LLG::Save_Current (Method_LLG.cpp:383-495): "<pre>_Energy<suffix>.txt"
+// holds the energy header and one line (image energy and the contributions of its last
+// update); a fresh file (initial, final, step) also writes the per-spin energies
+// "<pre>_Energy-perSpin<suffix>.ovf" when spin-resolved output is asked for (energies from
+// the GPU, contributions updated); archive files get a header once and a line per save.
+void spirit_hip::llg_save_energy( Image & img, int idx_image, const std::string & starttime, long iteration,
+                                  long n_iterations, bool initial, bool final )
+{
+    const LLGParameters & P = *img.llg;
+    if( !P.output_any )
+        return;
+    char s_img[32];
+    std::snprintf( s_img, sizeof( s_img ), "%02d", idx_image );
+    const int base = n_iterations > 0 ? (int)std::floor( std::log10( (double)n_iterations ) ) : 0;
+    char s_iter[64];
+    std::snprintf( s_iter, sizeof( s_iter ), "%0*ld", base, iteration );
+    const std::string tag = P.output_file_tag == "<time>" ? starttime + "_"
+                            : P.output_file_tag.empty()   ? std::string()
+                                                          : P.output_file_tag + "_";
+    const std::string pre = P.output_folder + "/" + tag + "Image-" + s_img + "_Energy";
+    const bool normalize  = P.output_energy_divide_by_nspins;
+    const bool readable   = P.output_energy_add_readability_lines;
+    auto write = [&]( const std::string & suffix, bool append ) {
+        const std::string file = pre + suffix + ".txt";
+        if( append )
+        {
+            std::FILE * f = std::fopen( file.c_str(), "rb" );
+            if( f )
+                std::fclose( f );
+            else
+                put_text( energy_header( img, readable ), file, false );
+            put_text( energy_line( img, iteration, normalize, readable ), file, true );
+            return;
+        }
+        put_text( energy_header( img, readable ), file, false );
+        put_text( energy_line( img, iteration, normalize, readable ), file, true );
+        if( !P.output_energy_spin_resolved )
+            return;
+        try
+        {
+            std::vector<double> terms;
+            img.E           = update_image_energy( img, false, &terms );
+            const auto list = energy_contribution_list( img );
+            const size_t nos = (size_t)img.nos(), dim = 1 + list.size();
+            std::vector<double> data( dim * nos, 0.0 );
+            for( size_t i = 0; i < nos; ++i )
+            {
+                double sum = 0;
+                for( size_t k = 0; k < list.size(); ++k )
+                {
+                    const double e       = terms[(size_t)list[k].second * nos + i];
+                    data[i * dim + 1 + k] = e;
+                    sum += e;
+                }
+                data[i * dim] = sum;
+            }
+            ovf::Segment seg = image_segment( img, "", "" );
+            char total[64];
+            {
+                auto r = std::to_chars( total, total + sizeof( total ) - 1, img.E );
+                *r.ptr = 0;
+            }
+            std::string comment = std::string( "Energy per spin. Total=" ) + total + "meV";
+            std::string labels  = "Total";
+            for( const auto & c : img.E_array )
+            {
+                char v[64];
+                auto r = std::to_chars( v, v + sizeof( v ) - 1, c.second );
+                *r.ptr = 0;
+                comment += ", " + c.first + "=" + v + "meV";
+                labels += " " + c.first;
+            }
+            seg.comment     = comment;
+            seg.valuedim    = (int)dim;
+            seg.valuelabels = labels;
+            seg.valueunits  = "";
+            ovf::write_segment( pre + "-perSpin" + suffix + ".ovf", seg, data.data(),
+                                written_format( P.output_configuration_filetype ), false );
+        }
+        catch( const std::exception & e )
+        {
+            log_message( 2, std::string( "LLG energy output failed: " ) + e.what() );
+        }
+    };
+    if( initial && P.output_initial )
+        write( "-initial", false );
+    else if( final && P.output_final )
+        write( "-final", false );
+    if( P.output_energy_step )
+        write( std::string( "_" ) + s_iter, false );
+    if( P.output_energy_archive )
+        write( "-archive", true );
+}
+
 std::string spirit_hip::current_date_time()
 {
     const std::time_t t = std::time( nullptr );

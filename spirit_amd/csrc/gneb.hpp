@@ -1,10 +1,10 @@
 // Launch interface of the GNEB chain kernels (FP64).
 //
 // The chain's images are stored batched, image k's SoA field at 3*k*nos, all images with
-// the same number of spins.  Per-spin kernels write per-block partial sums; single-block
-// "chain" kernels reduce them per image in a fixed order and run the chain-level logic
-// (reaction coordinate, tangent weights, springs, VP projection, convergence) on the
-// device, so an iteration needs no host round trip.
+// the same number of spins.  Per-spin kernels write per-block partial sums; the last block
+// of each pass to finish reduces them per image in a fixed order and runs the chain-level
+// logic (reaction coordinate, tangent weights, springs, VP projection, convergence) on the
+// device, so an iteration is one launch per pass and needs no host round trip.
 #pragma once
 
 #include "device_types.hpp"
@@ -15,7 +15,7 @@ namespace spirit_hip
 constexpr int GNEB_THREADS = 256;
 constexpr int GNEB_NPART   = 8;  // partial-sum slots per block
 constexpr int GNEB_NCH     = 32; // scalar slots per image
-constexpr int GNEB_CH_LDS  = 128; // chains up to this many images keep their scalars in LDS in the chain kernels
+constexpr int GNEB_CH_LDS  = 64; // chains up to this many images keep their scalars in LDS in the chain phases
 
 // partial-sum slots
 enum GnebPart
@@ -95,6 +95,7 @@ struct GnebArgs
     const int * types;           // image types (0 normal, 1 climbing, 2 falling, 3 stationary)
     int * stop;                  // 0 running, 1 converged, 2 two images coincide
     long long * iter;            // iterations done
+    int * count;                 // blocks of the running pass that have stored their partials (0 between passes)
     GnebParamsDev P;
 };
 
@@ -114,7 +115,7 @@ enum GnebPass
     GPASS_HOOK,         // Hook_Post_Iteration (non-VP solvers)
 };
 
-// chain phases (single block)
+// chain phases (run by the last block of the pass before them)
 enum GnebPhase
 {
     GPH_DIST = 0, // energies, Rx, tangent rules
@@ -128,8 +129,10 @@ enum GnebPhase
     GPH_HOOK,     // max torque, convergence, iteration count
 };
 
+// One per-spin pass over the chain, whose last block to finish runs the chain phase that
+// consumes the pass's partial sums (GRAD -> DIST, TANGENT -> TANGENT, TRANSLATE -> TRANSLATE,
+// SHRINKn -> SHRINKn, FORCE_VP -> VP, VP_UPDATE and HOOK -> HOOK; FORCE_FV has none).
 void gneb_launch_pass( int pass, const GnebArgs & A, hipStream_t st );
-void gneb_launch_chain( int phase, const GnebArgs & A, hipStream_t st );
 // Solver stages of the predictor-corrector solvers on the chain (virtual forces already computed):
 // stage 0 writes the predictor, stage 1 the new configuration (RK4: stages 0..3).
 struct GnebSolverArgs

@@ -50,6 +50,8 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
     b_types.resize( sizeof( int ) * noi );
     b_stop.resize( sizeof( int ) );
     b_iter.resize( sizeof( long long ) );
+    b_count.resize( sizeof( int ) );
+    HIPCHECK( hipMemsetAsync( b_count.as<void>(), 0, sizeof( int ), st ) );
     HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), st ) );
     HIPCHECK( hipMemsetAsync( b_iter.as<void>(), 0, sizeof( long long ), st ) );
 
@@ -72,6 +74,7 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
     A.types    = b_types.as<int>();
     A.stop     = b_stop.as<int>();
     A.iter     = b_iter.as<long long>();
+    A.count    = b_count.as<int>();
     set_types( types );
     set_parameters( par );
     if( min_is_solver( solver ) && solver != SOLVER_VP )
@@ -173,24 +176,17 @@ void GNEBRunner::calc_force( double * x, bool vp )
     field->compute_ddi_field( x );
     A.h = field->dev_ddi_field();
     gneb_launch_pass( GPASS_GRAD, A, st );
-    gneb_launch_chain( GPH_DIST, A, st );
     gneb_launch_pass( GPASS_TANGENT, A, st );
-    gneb_launch_chain( GPH_TANGENT, A, st );
     if( A.P.moving_endpoints && A.P.translating_endpoints )
     {
         gneb_launch_pass( GPASS_TRANSLATE, A, st );
-        gneb_launch_chain( GPH_TRANSLATE, A, st );
     }
     if( A.P.path_shortening_constant > 0 )
     {
         gneb_launch_pass( GPASS_SHRINK1, A, st );
-        gneb_launch_chain( GPH_SHRINK1, A, st );
         gneb_launch_pass( GPASS_SHRINK2, A, st );
-        gneb_launch_chain( GPH_SHRINK2, A, st );
         gneb_launch_pass( GPASS_SHRINK3, A, st );
-        gneb_launch_chain( GPH_SHRINK3, A, st );
         gneb_launch_pass( GPASS_SHRINK4, A, st );
-        gneb_launch_chain( GPH_SHRINK4, A, st );
     }
     gneb_launch_pass( vp ? GPASS_FORCE_VP : GPASS_FORCE_FV, A, st );
 }
@@ -203,10 +199,8 @@ void GNEBRunner::iteration()
     if( solver == SOLVER_VP )
     {
         calc_force( xs, true );
-        gneb_launch_chain( GPH_VP, A, st );
         A.x = xs;
         gneb_launch_pass( GPASS_VP_UPDATE, A, st );
-        gneb_launch_chain( GPH_HOOK, A, st );
         return;
     }
     if( min_is_solver( solver ) )
@@ -223,7 +217,6 @@ void GNEBRunner::iteration()
         min_launch_chain_update( solver, SA, min_args, nos, st );
         A.x = xs;
         gneb_launch_pass( GPASS_HOOK, A, st );
-        gneb_launch_chain( GPH_HOOK, A, st );
         return;
     }
     GnebSolverArgs S{};
@@ -250,7 +243,6 @@ void GNEBRunner::iteration()
     }
     A.x = xs;
     gneb_launch_pass( GPASS_HOOK, A, st );
-    gneb_launch_chain( GPH_HOOK, A, st );
 }
 
 long GNEBRunner::run( long n, bool check_convergence )

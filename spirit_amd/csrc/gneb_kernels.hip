@@ -227,7 +227,13 @@ __device__ __forceinline__ void total_force( const GnebArgs & A, int img, long l
     }
 }
 
-template<int PASS>
+template<int PHASE>
+__device__ __forceinline__ void pass_tail( const GnebArgs & A );
+
+// One per-spin pass over every image of the chain; with PHASE >= 0 the last block to finish
+// runs that chain phase on the pass's partial sums (pass_tail), so a pass and the chain step
+// that consumes it are one launch.
+template<int PASS, int PHASE>
 __global__ void __launch_bounds__( GNEB_THREADS ) k_gneb_pass( GnebArgs A )
 {
     if( *A.stop )
@@ -515,6 +521,8 @@ __global__ void __launch_bounds__( GNEB_THREADS ) k_gneb_pass( GnebArgs A )
         }
         block_max_store( t2, part + GP_C );
     }
+    if constexpr( PHASE >= 0 )
+        pass_tail<PHASE>( A );
 }
 
 // Cubic Hermite interpolation point k of the chain (Cubic_Hermite_Spline.cpp:10-47)
@@ -543,15 +551,15 @@ __device__ void hermite_point( const double * ch, int noi, int n_interp, int k, 
 template<int PHASE>
 __device__ __forceinline__ void chain_phase( const GnebArgs & A, double * ch, const double ( *red )[256] );
 
-// Per-image sums of partial slots (one wave per image, fixed order), then the chain logic.
-// The chain scalars are staged in LDS (chains of up to GNEB_CH_LDS images): the sequential
-// parts of the logic then pay LDS latency instead of L2 latency per dependent read.
+// The chain step that follows a per-spin pass, run by the last block of that pass to finish
+// (every thread of the block calls it): per-image sums of the pass's partial slots (one wave
+// per image at a time, fixed order), then the chain logic of the phase.  The chain scalars
+// are staged in LDS (chains of up to GNEB_CH_LDS images): the sequential parts of the logic
+// then pay LDS latency instead of L2 latency per dependent read.
 template<int PHASE>
-__global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
+__device__ void chain_tail( const GnebArgs & A )
 {
     extern __shared__ double ch_lds[];
-    if( *A.stop )
-        return;
     const int noi       = A.noi;
     const bool staged   = noi <= GNEB_CH_LDS;
     const int n_scalars = noi * GNEB_NCH;
@@ -604,6 +612,28 @@ __global__ void __launch_bounds__( 1024 ) k_gneb_chain( GnebArgs A )
     if( staged )
         for( int e = threadIdx.x; e < n_scalars; e += blockDim.x )
             A.ch[e] = ch_lds[e];
+}
+
+// Arrival of a block at the end of a pass: its partials are made visible device-wide, the
+// arrival counter is advanced, and the block that arrives last (all partials written) runs
+// the chain phase, then rearms the counter for the next pass.  Every thread of the block
+// calls it, after the block's partial slots were stored by thread 0.
+template<int PHASE>
+__device__ __forceinline__ void pass_tail( const GnebArgs & A )
+{
+    __shared__ int last;
+    if( threadIdx.x == 0 )
+    {
+        __threadfence();
+        last = atomicAdd( A.count, 1 ) == (int)gridDim.x - 1 ? 1 : 0;
+    }
+    __syncthreads();
+    if( !last )
+        return;
+    __threadfence();
+    chain_tail<PHASE>( A );
+    if( threadIdx.x == 0 )
+        atomicExch( A.count, 0 );
 }
 
 // The chain logic of one phase, run by the whole block: per-image work on one thread per
@@ -945,42 +975,29 @@ __global__ void __launch_bounds__( GNEB_THREADS ) k_gneb_stage( GnebSolverArgs S
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+template<int PASS, int PHASE>
+static void launch_pass( const GnebArgs & A, dim3 grid, hipStream_t st )
+{
+    const size_t lds = ( PHASE >= 0 && A.noi <= GNEB_CH_LDS ) ? sizeof( double ) * A.noi * GNEB_NCH : 0;
+    hipLaunchKernelGGL( ( k_gneb_pass<PASS, PHASE> ), grid, dim3( GNEB_THREADS ), lds, st, A );
+}
+
 void gneb_launch_pass( int pass, const GnebArgs & A, hipStream_t st )
 {
-    const dim3 blk( GNEB_THREADS );
     const dim3 all( A.n_blocks ), first( A.bpi );
     switch( pass )
     {
-        case GPASS_GRAD: hipLaunchKernelGGL( k_gneb_pass<GPASS_GRAD>, all, blk, 0, st, A ); break;
-        case GPASS_TANGENT: hipLaunchKernelGGL( k_gneb_pass<GPASS_TANGENT>, all, blk, 0, st, A ); break;
-        case GPASS_TRANSLATE: hipLaunchKernelGGL( k_gneb_pass<GPASS_TRANSLATE>, first, blk, 0, st, A ); break;
-        case GPASS_SHRINK1: hipLaunchKernelGGL( k_gneb_pass<GPASS_SHRINK1>, all, blk, 0, st, A ); break;
-        case GPASS_SHRINK2: hipLaunchKernelGGL( k_gneb_pass<GPASS_SHRINK2>, all, blk, 0, st, A ); break;
-        case GPASS_SHRINK3: hipLaunchKernelGGL( k_gneb_pass<GPASS_SHRINK3>, all, blk, 0, st, A ); break;
-        case GPASS_SHRINK4: hipLaunchKernelGGL( k_gneb_pass<GPASS_SHRINK4>, all, blk, 0, st, A ); break;
-        case GPASS_FORCE_VP: hipLaunchKernelGGL( k_gneb_pass<GPASS_FORCE_VP>, all, blk, 0, st, A ); break;
-        case GPASS_FORCE_FV: hipLaunchKernelGGL( k_gneb_pass<GPASS_FORCE_FV>, all, blk, 0, st, A ); break;
-        case GPASS_VP_UPDATE: hipLaunchKernelGGL( k_gneb_pass<GPASS_VP_UPDATE>, all, blk, 0, st, A ); break;
-        case GPASS_HOOK: hipLaunchKernelGGL( k_gneb_pass<GPASS_HOOK>, all, blk, 0, st, A ); break;
-        default: break;
-    }
-}
-
-void gneb_launch_chain( int phase, const GnebArgs & A, hipStream_t st )
-{
-    const dim3 g( 1 ), b( 1024 );
-    const size_t lds = A.noi <= GNEB_CH_LDS ? sizeof( double ) * A.noi * GNEB_NCH : 0;
-    switch( phase )
-    {
-        case GPH_DIST: hipLaunchKernelGGL( k_gneb_chain<GPH_DIST>, g, b, lds, st, A ); break;
-        case GPH_TANGENT: hipLaunchKernelGGL( k_gneb_chain<GPH_TANGENT>, g, b, lds, st, A ); break;
-        case GPH_TRANSLATE: hipLaunchKernelGGL( k_gneb_chain<GPH_TRANSLATE>, g, b, lds, st, A ); break;
-        case GPH_SHRINK1: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK1>, g, b, lds, st, A ); break;
-        case GPH_SHRINK2: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK2>, g, b, lds, st, A ); break;
-        case GPH_SHRINK3: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK3>, g, b, lds, st, A ); break;
-        case GPH_SHRINK4: hipLaunchKernelGGL( k_gneb_chain<GPH_SHRINK4>, g, b, lds, st, A ); break;
-        case GPH_VP: hipLaunchKernelGGL( k_gneb_chain<GPH_VP>, g, b, lds, st, A ); break;
-        case GPH_HOOK: hipLaunchKernelGGL( k_gneb_chain<GPH_HOOK>, g, b, lds, st, A ); break;
+        case GPASS_GRAD: launch_pass<GPASS_GRAD, GPH_DIST>( A, all, st ); break;
+        case GPASS_TANGENT: launch_pass<GPASS_TANGENT, GPH_TANGENT>( A, all, st ); break;
+        case GPASS_TRANSLATE: launch_pass<GPASS_TRANSLATE, GPH_TRANSLATE>( A, first, st ); break;
+        case GPASS_SHRINK1: launch_pass<GPASS_SHRINK1, GPH_SHRINK1>( A, all, st ); break;
+        case GPASS_SHRINK2: launch_pass<GPASS_SHRINK2, GPH_SHRINK2>( A, all, st ); break;
+        case GPASS_SHRINK3: launch_pass<GPASS_SHRINK3, GPH_SHRINK3>( A, all, st ); break;
+        case GPASS_SHRINK4: launch_pass<GPASS_SHRINK4, GPH_SHRINK4>( A, all, st ); break;
+        case GPASS_FORCE_VP: launch_pass<GPASS_FORCE_VP, GPH_VP>( A, all, st ); break;
+        case GPASS_FORCE_FV: launch_pass<GPASS_FORCE_FV, -1>( A, all, st ); break;
+        case GPASS_VP_UPDATE: launch_pass<GPASS_VP_UPDATE, GPH_HOOK>( A, all, st ); break;
+        case GPASS_HOOK: launch_pass<GPASS_HOOK, GPH_HOOK>( A, all, st ); break;
         default: break;
     }
 }

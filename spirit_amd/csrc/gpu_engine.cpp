@@ -351,11 +351,31 @@ static FastDiv make_fast_div( int d )
 
 void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
 {
+    HamDev H{};
+    std::vector<PairDev> pv = build_ham_dev( g, h, H );
+    pairs.resize( sizeof( PairDev ) * std::max<size_t>( 1, pv.size() ) );
+    if( !pv.empty() )
+        HIPCHECK( hipMemcpy( pairs.as<void>(), pv.data(), sizeof( PairDev ) * pv.size(), hipMemcpyHostToDevice ) );
+    H.exch = pairs.as<PairDev>();
+    H.dmi  = pairs.as<PairDev>() + H.n_exch;
+    if( H.has_site_K )
+    {
+        site_K.resize( sizeof( double ) * g.nos );
+        HIPCHECK( hipMemcpy( site_K.as<void>(), h.site_K.data(), sizeof( double ) * g.nos, hipMemcpyHostToDevice ) );
+        H.site_K = site_K.as<double>();
+    }
+    dev         = H;
+    version     = h.version;
+    ddi_version = h.ddi_version;
+}
+
+std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H )
+{
     if( g.nb > MAX_BASIS )
         throw std::runtime_error( "too many basis atoms (max " + std::to_string( MAX_BASIS ) + ")" );
     if( (int)h.aniso_idx.size() > MAX_BASIS || (int)h.cubic_idx.size() > MAX_BASIS )
         throw std::runtime_error( "too many anisotropy entries" );
-    HamDev H{};
+    H     = HamDev{};
     H.Na  = g.n_cells[0];
     H.Nb  = g.n_cells[1];
     H.Nc  = g.n_cells[2];
@@ -430,26 +450,19 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
         if( e.i != m.i || e.j != m.j || e.da != m.da || e.db != m.db || e.dc != m.dc )
             H.dmi_same = 0;
     }
-    pairs.resize( sizeof( PairDev ) * std::max<size_t>( 1, pv.size() ) );
-    if( !pv.empty() )
-        HIPCHECK( hipMemcpy( pairs.as<void>(), pv.data(), sizeof( PairDev ) * pv.size(), hipMemcpyHostToDevice ) );
-    H.exch = pairs.as<PairDev>();
-    H.dmi  = pairs.as<PairDev>() + H.n_exch;
+    H.exch = nullptr;
+    H.dmi  = nullptr;
     if( !h.site_K.empty() )
     {
         if( (int)h.site_K.size() != g.nos )
             throw std::runtime_error( "per-site anisotropy has the wrong length" );
-        site_K.resize( sizeof( double ) * g.nos );
-        HIPCHECK( hipMemcpy( site_K.as<void>(), h.site_K.data(), sizeof( double ) * g.nos, hipMemcpyHostToDevice ) );
         H.has_site_K = 1;
-        H.site_K     = site_K.as<double>();
+        H.site_K     = nullptr;
         for( int c = 0; c < 3; ++c )
             H.site_n[c] = h.site_n[c];
     }
-    H.has_ddi   = h.ddi_method == DDI_FFT ? 1 : 0;
-    dev         = H;
-    version     = h.version;
-    ddi_version = h.ddi_version;
+    H.has_ddi = h.ddi_method == DDI_FFT ? 1 : 0;
+    return pv;
 }
 
 // ---------------------------------------------------------------------------
@@ -524,7 +537,8 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
         else
         {
             if( sp.hamiltonian->ddi_method != DDI_NONE )
-                log_message( 3, "DDI method other than FFT is not available in the HIP engine; DDI ignored" );
+                throw std::runtime_error( "DDI method " + std::to_string( sp.hamiltonian->ddi_method )
+                                          + " is not available in the HIP engine (use FFT or none)" );
             ddi.emplace_back( nullptr );
         }
     }
@@ -753,11 +767,19 @@ void LLGRunner::download_spins()
 
 void LLGRunner::download_effective_field( int img, double * aos )
 {
-    const int nos = specs[img].geometry->nos;
+    const int nos        = specs[img].geometry->nos;
+    const long long base = 3 * imgs_host[img].off;
+    const double * src   = b_eff.as<double>() + base;
+    if( eff_raw )
+    {
+        // P_perp(s) F1 with the spins of the last observed step (every run ends on one)
+        llg_launch_project_field( src, b_s.as<double>() + base, b_efft.as<double>() + base, nos, stream );
+        HIPCHECK( hipGetLastError() );
+        src = b_efft.as<double>() + base;
+    }
     std::vector<double> soa( 3 * (size_t)nos );
     HIPCHECK( hipStreamSynchronize( stream ) );
-    HIPCHECK( hipMemcpy( soa.data(), b_eff.as<double>() + 3 * imgs_host[img].off, sizeof( double ) * soa.size(),
-                         hipMemcpyDeviceToHost ) );
+    HIPCHECK( hipMemcpy( soa.data(), src, sizeof( double ) * soa.size(), hipMemcpyDeviceToHost ) );
     soa_to_aos( soa.data(), aos, nos );
 }
 
@@ -808,7 +830,7 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
     A.aux2    = aux2.as<double>();
     A.h       = h.as<double>();
     A.eff     = eff.as<double>();
-    A.eff_tmp = efft.as<double>();
+    (void)efft;
     A.part_E  = pE.as<double>();
     A.part_T  = pT.as<double>();
     A.stop     = stop.as<int>();
@@ -997,6 +1019,7 @@ long LLGRunner::run( long n, int observe_every, bool check_convergence, double f
         ++it;
     }
     dev_step = false;
+    eff_raw  = !min_is_solver( solver );
     HIPCHECK( hipGetLastError() );
     std::vector<long long> stop_step( n_img, 0 );
     HIPCHECK( hipMemcpyAsync( img_stopped.data(), b_img_stop.as<void>(), sizeof( int ) * n_img, hipMemcpyDeviceToHost,
@@ -1056,6 +1079,7 @@ void LLGRunner::initial_observe()
         ddi_field( A.s );
     HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), stream ) );
     llg_launch_observe( A, solver, n_blocks, stream );
+    eff_raw = true;
     llg_launch_finalize( A, (int)specs.size(), b_obs.as<double>(), b_img_stop.as<int>(), b_nstop.as<int>(),
                          b_stopstep.as<long long>(), -1, 0.0, 0, stream );
     HIPCHECK( hipGetLastError() );

@@ -97,6 +97,12 @@ private:
 // Padded DDI grid, Hamiltonian_Heisenberg.cpp:1501-1530 (without the kissFFT-only length-1 workaround).
 std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h );
 
+// The POD image description a kernel reads, built on the host: H gets every field but the
+// device pointers (exch, dmi, site_K are null; has_site_K says whether the Hamiltonian has
+// per-site K_i, from h.site_K); returns the pair table (n_exch exchange pairs, then n_dmi DMI
+// pairs) that H.exch / H.dmi must point to on the device.
+std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H );
+
 // Builds the POD image description a kernel reads (pair tables are uploaded to `pairs`).
 struct HamUpload
 {
@@ -268,6 +274,10 @@ private:
     std::vector<long> steps_last; // steps each image advanced in the last run
     int stop_host = 0, nstop_host = 0;
     bool any_ddi = false;
+    // b_eff holds the unprojected force F1 of the last observed step (LLG stages and the
+    // initial observation), projected onto the spins when downloaded; the direct-minimisation
+    // kernels store the projected field
+    bool eff_raw = true;
     long noise_steps = 0, noise_pos = 0;
     // direct-minimisation solvers (VP, VP_OSO, LBFGS_OSO, LBFGS_Atlas): state of the run
     void setup_minimiser();
@@ -380,7 +390,7 @@ private:
     int solver = 0, noi = 0, nos = 0, bpi = 0;
     GnebArgs A{};
     DeviceBuffer b_eff, b_Fg, b_tau, b_Ft, b_ftr, b_fv, b_fv0, b_v, b_fprev, b_pred, b_tmp, b_part, b_ch, b_types,
-        b_stop, b_iter;
+        b_stop, b_iter, b_count;
     // VP_OSO, LBFGS_OSO, LBFGS_Atlas on the chain's forces F_total
     void setup_minimiser();
     DeviceBuffer b_mv, b_mprev, b_mgrad, b_mgpr, b_msd, b_mq, b_ma3, b_mpart, b_msc;

@@ -20,8 +20,8 @@ struct StepArgs
     double * aux;          // first virtual force (Depondt/Heun, SIB on observed steps) or RK4 accumulator
     double * aux2;         // RK4: first virtual force on observed steps
     const double * h;      // DDI field of the stage's input configuration (nullable)
-    double * eff;          // effective field output (observed steps)
-    double * eff_tmp;      // first force, kept for the effective field (observed steps)
+    double * eff;          // observed steps: the first evaluation's force F1 (LLG stages) or the
+                           // projected effective field (direct-minimisation kernels)
     const double * noise;  // injected N(0,1) thermal noise of this step, AoS per spin (nullable)
     double * part_E;       // per-block energy partial sums
     double * part_T;       // per-block max squared torque
@@ -44,6 +44,8 @@ void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, i
 // image at 6*off; the DDI field of x must be in A.h
 void llg_launch_energy_terms( const StepArgs & A, const double * x, double * out, int n_blocks, hipStream_t st );
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
+// out = F - (F.s) s per spin (one image's SoA fields)
+void llg_launch_project_field( const double * F, const double * s, double * out, int nos, hipStream_t st );
 // Per-image energy / max torque into obs[2*img], [2*img+1]; with check_convergence an image whose
 // torque is below force_convergence stops (img_stop, stop_step[img]); the run-wide stop flag is set
 // when n_stopped reaches n_images.

@@ -77,8 +77,11 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
             store3( A.tape_F + base, nos, i, F );
             store3( A.tape_fv + base, nos, i, fv );
         }
+        // Hook_Post_Iteration's effective field is P_perp(s_new) F1: the force of the first
+        // evaluation is stored here and projected onto the new spins when the field is read
+        // (llg_launch_project_field), so an observed step adds 24 B per spin, not 72.
         if( STAGE == 0 && A.obs )
-            store3( A.eff_tmp + base, nos, i, F );
+            store3( A.eff + base, nos, i, F );
 
         if( STAGE == 0 )
         {
@@ -232,20 +235,13 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 
         if( LAST && A.obs )
         {
-            // MaxTorque_on_Image: |P_perp(s_new) F_v1|; effective field P_perp(s_new) F1
+            // MaxTorque_on_Image: |P_perp(s_new) F_v1|
             double d = dot3( fv1, snew );
             double pr[3];
 #pragma unroll
             for( int q = 0; q < 3; ++q )
                 pr[q] = fv1[q] - d * snew[q];
             t_i = dot3( pr, pr );
-            double F1[3];
-            load3( A.eff_tmp + base, nos, i, F1 );
-            double d1 = dot3( F1, snew );
-#pragma unroll
-            for( int q = 0; q < 3; ++q )
-                pr[q] = F1[q] - d1 * snew[q];
-            store3( A.eff + base, nos, i, pr );
         }
     }
     if( LAST && A.obs )
@@ -254,7 +250,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 
 // The method constructor's force evaluation and Hook_Post_Iteration (Method_LLG.cpp:22-63,
 // 247-301) on the current spins, without a thermal field: energy, max torque of the
-// virtual force projected on s, effective field P_perp(s) F.
+// virtual force projected on s, and the force F (projected when the field is read).
 __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solver )
 {
     const int blk           = xcd_contiguous_block();
@@ -282,14 +278,36 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solv
 #pragma unroll
         for( int q = 0; q < 3; ++q )
             pr[q] = fv[q] - d * si[q];
-        t_i       = dot3( pr, pr );
-        double d1 = dot3( F, si );
-#pragma unroll
-        for( int q = 0; q < 3; ++q )
-            pr[q] = F[q] - d1 * si[q];
-        store3( A.eff + base, nos, i, pr );
+        t_i = dot3( pr, pr );
+        store3( A.eff + base, nos, i, F );
     }
     block_reduce_store( e_i, t_i, A.part_E, A.part_T, blk );
+}
+
+// Effective field of one image as Hook_Post_Iteration forms it (Method_LLG.cpp:284-285):
+// out = P_perp(s) F = F - (F.s) s, from the force F the last observed step stored and that
+// step's new spins s.
+__global__ void __launch_bounds__( LLG_THREADS ) k_project_field( const double * __restrict__ F,
+                                                                 const double * __restrict__ s,
+                                                                 double * __restrict__ out, int nos )
+{
+    const int i = blockIdx.x * LLG_THREADS + threadIdx.x;
+    if( i >= nos )
+        return;
+    double f[3], si[3], pr[3];
+    load3( F, nos, i, f );
+    load3( s, nos, i, si );
+    const double d1 = dot3( f, si );
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        pr[q] = f[q] - d1 * si[q];
+    store3( out, nos, i, pr );
+}
+
+void llg_launch_project_field( const double * F, const double * s, double * out, int nos, hipStream_t st )
+{
+    hipLaunchKernelGGL( k_project_field, dim3( ( nos + LLG_THREADS - 1 ) / LLG_THREADS ), dim3( LLG_THREADS ), 0, st,
+                        F, s, out, nos );
 }
 
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st )

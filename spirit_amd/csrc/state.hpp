@@ -6,6 +6,7 @@
 #include "gpu_engine.hpp"
 #include "host_core.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <map>
 #include <memory>
@@ -30,8 +31,9 @@ struct Image
     std::vector<std::pair<std::string, double>> E_array; // energy contributions (System_Update_Data)
     double max_torque = 0;
     double Rx         = 0;
-    bool iteration_allowed  = false;
-    bool singleshot_allowed = false;
+    // written by Simulation_Stop from any thread while a run polls them
+    std::atomic<bool> iteration_allowed{ false };
+    std::atomic<bool> singleshot_allowed{ false };
     std::recursive_mutex mtx;
 
     Image() = default;
@@ -56,8 +58,8 @@ struct Chain
     std::vector<int> image_type;
     std::shared_ptr<GNEBParameters> gneb;
     std::vector<double> Rx, Rx_interpolated, E_interpolated;
-    bool iteration_allowed  = false;
-    bool singleshot_allowed = false;
+    std::atomic<bool> iteration_allowed{ false };
+    std::atomic<bool> singleshot_allowed{ false };
     std::recursive_mutex mtx;
     int noi() const
     {
@@ -80,6 +82,11 @@ struct MethodLLG
     bool device_newer      = false; // device spins newer than the host mirror
     std::chrono::steady_clock::time_point t_start;
     std::string solver_name;
+    std::string starttime; // date and time of the start (output file tags)
+    int idx_image = 0;
+    // appended at every Save_Current (Method_LLG.cpp:313-315), handed out by Simulation_Run_Info
+    std::vector<int> history_iteration;
+    std::vector<float> history_max_torque, history_energy;
 };
 
 // A running GNEB method on the chain, core/src/engine/Method_GNEB.cpp

@@ -153,3 +153,37 @@ def test_torch_autograd_function(state):
     np.testing.assert_allclose(D.grad.cpu().numpy(), ref["D"], rtol=1e-12)
     np.testing.assert_allclose(B.grad.cpu().numpy(), ref["B"], rtol=1e-12)
     np.testing.assert_allclose(Ks.grad.cpu().numpy(), ref["K_site"], rtol=1e-12, atol=1e-18)
+
+
+def test_torch_function_applied_twice(state):
+    """Two applications of LLGFunction on one session before a single backward (a loss that
+    sums two runs from different initial states and parameters): each node differentiates
+    its own run, so the gradients equal the sums of two fresh single-application runs."""
+    import torch
+    from spirit_amd import adjoint
+    print_case("adjoint", "torch function applied twice")
+    n = 10 * 8 * 3
+    p = state(FILM, site_K=np.full(n, 0.03))
+    sess = adjoint.Session(p, solver=2, n_steps=10, checkpoint_every=4)
+    dev = torch.device("cuda")
+    t = torch.tensor(configs.random_spins(n, 9), dtype=torch.float64, device=dev)
+    base = sess.parameter_tensors(requires_grad=False, device=dev)
+
+    def run(s0, scale):
+        J, D, B, Ks = (x.clone().mul_(scale).requires_grad_(True) for x in base)
+        return J, D, B, Ks, adjoint.LLGFunction.apply(sess, s0, J, D, B, Ks)
+
+    starts = [torch.tensor(configs.random_spins(n, 20 + k), dtype=torch.float64, device=dev, requires_grad=True)
+              for k in range(2)]
+    scales = (1.0, 1.3)
+    # both runs first, then one backward through their sum
+    outs = [run(s0, c) for s0, c in zip(starts, scales)]
+    loss = sum(((o[4] - t) ** 2).sum(dim=1).mean() for o in outs)
+    loss.backward()
+    for k, (s0, c) in enumerate(zip(starts, scales)):
+        ref_s0 = s0.detach().clone().requires_grad_(True)
+        J, D, B, Ks, sN = run(ref_s0, c)
+        ((sN - t) ** 2).sum(dim=1).mean().backward()
+        np.testing.assert_allclose(s0.grad.cpu().numpy(), ref_s0.grad.cpu().numpy(), rtol=1e-12, atol=1e-15)
+        for a, b in zip(outs[k][:4], (J, D, B, Ks)):
+            np.testing.assert_allclose(a.grad.cpu().numpy(), b.grad.cpu().numpy(), rtol=1e-12, atol=1e-18)

@@ -111,3 +111,58 @@ def test_batch_with_dipolar_replicas(states):
     hip.llg_start_batch(bat, 3, n_iterations=60)
     for k in range(len(cfgs)):
         np.testing.assert_array_equal(spins(bat[k]), spins(seq[k]))
+
+
+def test_ensemble_replicas_match_oracle(states):
+    """The cfg-4 uneven ensemble (seeded sides, B, T and seeds) against the oracle: one replica
+    of each side 32, 48, 64 and 256 taken from configs.ensemble(256), advanced together by one
+    batched Depondt run of 200 steps at their temperatures; each equals the oracle's run of
+    that replica on the same counter-based thermal stream (its seed, steps 0..199) to 1e-9."""
+    import oracle as O
+    from spirit_amd import hip
+    print_case("ensemble256 replicas vs oracle")
+    ens = configs.ensemble(256)
+    pick = []
+    for L in (32, 48, 64, 256):
+        pick.append(next(c for c in ens if c["n_basis_cells"][0] == L))
+    assert all(c["llg_temperature"] > 0 for c in pick)
+    bat = states(pick, spins_seed=500)
+    infos = hip.llg_start_batch(bat, 2, n_iterations=200)
+    assert [i.total_iterations for i in infos] == [200] * 4
+    for k, cfg in enumerate(pick):
+        n = cfg["n_basis_cells"][0] ** 2
+        s0 = configs.random_spins(n, 500 + k)
+        ref, _, _, _ = O.System(cfg).llg(s0, O.SOLVER_DEPONDT, 200, dt=cfg["llg_dt"], damping=cfg["llg_damping"],
+                                        temperature=cfg["llg_temperature"], seed=cfg["llg_seed"])
+        err = float(np.abs(spins(bat[k]) - ref).max())
+        print("  L=%d B=%.3f T=%.2f seed=%d: max |s - s_oracle| = %.3e" % (
+            cfg["n_basis_cells"][0], cfg["external_field_magnitude"], cfg["llg_temperature"], cfg["llg_seed"], err),
+            flush=True)
+        assert err < 1e-9
+
+
+def test_ensemble256_full_properties(states):
+    """The full 256-replica ensemble (about 4.8 million spins) through one batched Depondt run
+    of 500 steps: every replica completes, every spin stays a unit vector to 1e-12, and the
+    replicas of a sample (the first of each side) are bit-identical to their lone runs."""
+    from spirit_amd import hip, simulation
+    print_case("ensemble256 full batch")
+    ens = configs.ensemble(256)
+    bat = states(ens, spins_seed=900)
+    infos = hip.llg_start_batch(bat, 2, n_iterations=500)
+    assert all(i.total_iterations == 500 for i in infos)
+    worst = 0.0
+    for p in bat:
+        s = spins(p)
+        worst = max(worst, float(np.abs(np.linalg.norm(s, axis=1) - 1).max()))
+    assert worst < 1e-12, worst
+    seen = set()
+    for k, cfg in enumerate(ens):
+        L = cfg["n_basis_cells"][0]
+        if L in seen:
+            continue
+        seen.add(L)
+        lone = states([cfg], spins_seed=900 + k)[0]
+        simulation.start(lone, simulation.METHOD_LLG, 2, n_iterations=500)
+        np.testing.assert_array_equal(spins(lone), spins(bat[k]))
+    assert len(seen) == 7

@@ -265,10 +265,16 @@ def test_reference_reader_and_writer_interoperate(make_state, tmp_path):
 
 @pytest.mark.gpu
 def test_llg_configuration_output_files(tmp_path):
-    """Method_LLG::Save_Current (Method_LLG.cpp:310-495): <tag>_Image-00_Spins-initial.ovf,
-    -final.ovf, _<iteration>.ovf per save and -archive.ovf with one segment per save (the
-    initial and final saves included), written by Simulation_LLG_Start; their spins are the
-    run's (binary files; reading renormalises, so to 1e-15)."""
+    """Method_LLG::Save_Current (Method_LLG.cpp:310-495) as Method::Iterate calls it
+    (Method.cpp:57-113): <tag>_Image-00_Spins-initial.ovf, -final.ovf, _<iteration>.ovf per
+    save and -archive.ovf with one segment per save.  The loop saves after the body that
+    starts at iteration 10 and 20 (so those files hold the spins after 11 and 21 steps), the
+    initial and final saves also write the step and archive outputs: archive segments 0, 10,
+    20, 30.  The energy files (_Energy-initial/-final/_<iteration>.txt, -archive.txt) carry
+    the header and one line per save, energies per spin to 10 decimals.  Spins are checked
+    against the oracle's run (binary files; reading renormalises, so 1e-15 against the run's
+    own final spins)."""
+    import oracle as O
     from spirit_amd import io, simulation, state, system
     from spirit_amd.parameters import llg
     print("CASE llg output files", flush=True)
@@ -281,7 +287,7 @@ def test_llg_configuration_output_files(tmp_path):
     text = configs.config_text(cfg) + "\n".join([
         "output_file_tag run", "llg_output_folder %s" % out, "llg_output_any 1", "llg_output_initial 1",
         "llg_output_final 1", "llg_output_configuration_step 1", "llg_output_configuration_archive 0",
-        "llg_output_configuration_filetype 0"]) + "\n"
+        "llg_output_configuration_filetype 0", "llg_output_energy_step 1", "llg_output_energy_archive 1"]) + "\n"
     path = tmp_path / "input.cfg"
     path.write_text(text)
     p = state.setup(str(path), True)
@@ -294,13 +300,16 @@ def test_llg_configuration_output_files(tmp_path):
         set_spins(p, s0)
         simulation.start(p, simulation.METHOD_LLG, simulation.SOLVER_DEPONDT, n_iterations=30, n_iterations_log=10)
         s_end = spins(p).copy()
+        E_end = system.get_energy(p)
         names = sorted(f.name for f in out.iterdir())
-        # the initial and final saves write the step and archive outputs as well (iterations 0 and 30)
-        want = ["run_Image-00_Spins-archive.ovf", "run_Image-00_Spins-final.ovf", "run_Image-00_Spins-initial.ovf",
+        want = ["run_Image-00_Energy-archive.txt", "run_Image-00_Energy-final.txt", "run_Image-00_Energy-initial.txt",
+                "run_Image-00_Energy_0.txt", "run_Image-00_Energy_10.txt", "run_Image-00_Energy_20.txt",
+                "run_Image-00_Energy_30.txt",
+                "run_Image-00_Spins-archive.ovf", "run_Image-00_Spins-final.ovf", "run_Image-00_Spins-initial.ovf",
                 "run_Image-00_Spins_0.ovf", "run_Image-00_Spins_10.ovf", "run_Image-00_Spins_20.ovf",
                 "run_Image-00_Spins_30.ovf"]
         assert names == want
-        assert io.n_images_in_file(p, str(out / "run_Image-00_Spins-archive.ovf")) == 5
+        assert io.n_images_in_file(p, str(out / "run_Image-00_Spins-archive.ovf")) == 4
         io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), 0)
         np.testing.assert_allclose(spins(p), s0, rtol=0, atol=1e-15)
         io.image_read(p, str(out / "run_Image-00_Spins-initial.ovf"))
@@ -308,12 +317,25 @@ def test_llg_configuration_output_files(tmp_path):
         for f in ("run_Image-00_Spins-final.ovf", "run_Image-00_Spins_30.ovf"):
             io.image_read(p, str(out / f))
             np.testing.assert_allclose(spins(p), s_end, rtol=0, atol=1e-15)
-        for k in (3, 4):
-            io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), k)
-            np.testing.assert_allclose(spins(p), s_end, rtol=0, atol=1e-15)
-        io.image_read(p, str(out / "run_Image-00_Spins_10.ovf"))
-        assert np.abs(spins(p) - s0).max() > 1e-6 and np.abs(spins(p) - s_end).max() > 1e-9
+        io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), 3)
+        np.testing.assert_allclose(spins(p), s_end, rtol=0, atol=1e-15)
+        sysO = O.System(cfg)
+        for label, k in (("10", 11), ("20", 21)):
+            ref, _, _, _ = sysO.llg(s0, O.SOLVER_DEPONDT, k, dt=1e-3, damping=0.3)
+            io.image_read(p, str(out / ("run_Image-00_Spins_%s.ovf" % label)))
+            assert np.abs(spins(p) - ref).max() < 1e-9, label
+            io.image_read(p, str(out / "run_Image-00_Spins-archive.ovf"), int(label) // 10)
+            assert np.abs(spins(p) - ref).max() < 1e-9, label
         head = open(out / "run_Image-00_Spins_20.ovf", "rb").read(2000).decode("latin-1")
         assert "# Desc: LLG simulation (Depondt solver)" in head and "# Desc:      Iteration: 20" in head
+        # energy files: Write_Energy_Header / Append_Image_Energy without readability lines
+        arch = (out / "run_Image-00_Energy-archive.txt").read_text().splitlines()
+        assert len(arch) == 5
+        assert arch[0].split() == ["iteration", "E_tot"]
+        assert [int(l.split()[0]) for l in arch[1:]] == [0, 10, 20, 30]
+        last = arch[-1].split()
+        assert len(last[1].split(".")[1]) == 10
+        assert abs(float(last[1]) - E_end / 48) < 1e-6 * max(1.0, abs(E_end / 48))
+        assert (out / "run_Image-00_Energy_20.txt").read_text().splitlines()[1].split()[0] == "20"
     finally:
         state.delete(p)

@@ -420,3 +420,166 @@ def _skyrmion(pos, center, p):
                       math.cos(th) * -dr])
         s[i] = v / np.linalg.norm(v)
     return s
+
+
+# ---------------------------------------------------------------------------
+# Closed forms that fix the dipole tensor, the Neel DMI normals and the thermal amplitude
+# ---------------------------------------------------------------------------
+# core/include/utility/Constants.hpp, typed here independently of the oracle's copies
+MU_0_T2M3_PER_MEV = 2.0133545e-28
+MU_B_MEV_PER_T = 0.057883817555
+K_B_MEV_PER_K = 0.08617330350
+
+
+def _dipole_energy_closed_form(pos, spins, mu):
+    """E = -sum_{i<j} mu0 muB^2 mu_i mu_j / (4 pi |r|^3) [3 (s_i.r^)(s_j.r^) - s_i.s_j], r in
+    Angstrom (1e-10 m), and the gradient dE/ds_i = -sum_j mu0 muB^2 mu_i mu_j / (4 pi |r|^3)
+    [3 r^ (r^.s_j) - s_j]: the classical energy of point dipoles mu_i muB s_i."""
+    pref = MU_0_T2M3_PER_MEV * MU_B_MEV_PER_T ** 2 / (4 * math.pi * 1e-30)
+    n = len(pos)
+    E = 0.0
+    g = np.zeros((n, 3))
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            r = pos[j] - pos[i]
+            d = np.linalg.norm(r)
+            u = r / d
+            c = pref * mu[i] * mu[j] / d ** 3
+            g[i] -= c * (3 * u * np.dot(u, spins[j]) - spins[j])
+            if i < j:
+                E -= c * (3 * np.dot(spins[i], u) * np.dot(spins[j], u) - np.dot(spins[i], spins[j]))
+    return E, g
+
+
+def test_dipole_two_moments_closed_form(oracle):
+    """Two moments 3 Angstrom apart along x (lattice constant 3, mu_s 2): head-to-tail along x
+    gives E = -2 c, side by side along z gives E = +c, orthogonal x/y gives 0, with
+    c = mu0 muB^2 mu^2 / (4 pi r^3 1e-30) -- fixes the prefactor and the sign of the tensor
+    (Hamiltonian_Heisenberg.cpp:1409, :1487-1492) in the FFT and the direct-sum paths."""
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[2, 1, 1], boundary_conditions=[0, 0, 0], mu_s=[2.0],
+               lattice_constant=3.0, ddi_method="fft", ddi_n_periodic_images=[0, 0, 0], ddi_pb_zero_padding=1)
+    S = oracle.System(cfg)
+    c = MU_0_T2M3_PER_MEV * MU_B_MEV_PER_T ** 2 * 4.0 / (4 * math.pi * 27.0 * 1e-30)
+    x, y, z = np.eye(3)
+    for s, want in (((x, x), -2 * c), ((z, z), c), ((x, y), 0.0), ((y, -y), -c)):
+        spins = np.array(s, dtype=float)
+        for term in (5, 6):  # FFT convolution, direct sum
+            g = S.gradient_term(term, spins)
+            E = 0.5 * float(np.sum(g * spins))
+            assert abs(E - want) <= 1e-12 * c, (term, s, E, want)
+
+
+def test_dipole_lattice_closed_form(oracle):
+    """A 3x2x2 open block with a two-atom basis and lattice constant 2.5: energy and gradient
+    of the FFT convolution and of the direct sum equal the brute-force point-dipole sum."""
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[3, 2, 2], boundary_conditions=[0, 0, 0], mu_s=[2.0, 1.5],
+               basis=[[0.0, 0.0, 0.0], [0.5, 0.5, 0.25]], lattice_constant=2.5, ddi_method="fft",
+               ddi_n_periodic_images=[0, 0, 0], ddi_pb_zero_padding=1)
+    S = oracle.System(cfg)
+    spins = oracle.random_spins(S.nos, 5)
+    pos = S.positions()
+    mu = np.array([2.0, 1.5] * (S.nos // 2))
+    E_ref, g_ref = _dipole_energy_closed_form(pos, spins, mu)
+    for term in (5, 6):
+        g = S.gradient_term(term, spins)
+        E = 0.5 * float(np.sum(g * spins))
+        assert abs(E - E_ref) <= 1e-11 * abs(E_ref), term
+        np.testing.assert_allclose(g, g_ref, rtol=0, atol=1e-11 * np.abs(g_ref).max())
+
+
+# Neel normals d = r^ x z (chirality 2) and z x r^ (chirality -2), Neighbours.cpp:249-286,
+# written out for the first shells (hex2d: a1 = (sqrt3/2, -1/2, 0), a2 = (sqrt3/2, 1/2, 0))
+_H = math.sqrt(3) / 2
+NEEL_NORMALS = {
+    "sc": {(1, 0, 0): (0, -1, 0), (0, 1, 0): (1, 0, 0), (0, -1, 0): (-1, 0, 0), (-1, 0, 0): (0, 1, 0)},
+    "hex2d": {(1, 0, 0): (-0.5, -_H, 0), (1, -1, 0): (-1, 0, 0), (0, 1, 0): (0.5, -_H, 0), (0, -1, 0): (-0.5, _H, 0),
+              (-1, 1, 0): (1, 0, 0), (-1, 0, 0): (0.5, _H, 0)},
+}
+
+
+@pytest.mark.parametrize("lattice", ["sc", "hex2d"])
+@pytest.mark.parametrize("chirality", [2, -2])
+def test_neel_normals_explicit(oracle, lattice, chirality):
+    cfg = dict(bravais_lattice=lattice, n_basis_cells=[8, 8, 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+               n_shells_exchange=0, n_shells_dmi=1, dij=[1.0], dm_chirality=chirality)
+    p, _ = oracle.params_from_config(cfg)
+    want = NEEL_NORMALS[lattice]
+    got = {}
+    for k in range(p.n_dmi):
+        t = tuple(p.dmi[k][2:5])
+        got[t] = tuple(p.dmi_normal[k])
+    assert sorted(got) == sorted(want)
+    sign = 1 if chirality == 2 else -1
+    for t, d in want.items():
+        np.testing.assert_allclose(got[t], sign * np.array(d, float), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("lattice", ["sc", "hex2d"])
+@pytest.mark.parametrize("chirality", [2, -2])
+def test_neel_cycloid_energy(oracle, lattice, chirality):
+    """The DMI energy of a cycloid s = (sin qx, 0, cos qx) along x: with the Neel normal every
+    x-offset r_x of a pair adds D r_x sin(q r_x) (half per spin), so the square lattice gives
+    E = N D sin q and the hexagonal one E = N sqrt3 D sin(q sqrt3/2); chirality -2 flips the
+    sign.  This is what makes the interfacial DMI favour one rotation sense."""
+    D = 1.7
+    n = 8
+    cfg = dict(bravais_lattice=lattice, n_basis_cells=[n, n, 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+               n_shells_exchange=0, n_shells_dmi=1, dij=[D], dm_chirality=chirality)
+    S = oracle.System(cfg)
+    x = S.positions()[:, 0]
+    if lattice == "sc":
+        q = 2 * math.pi / n
+        want = S.nos * D * math.sin(q)
+    else:
+        q = 2 * math.pi / n / _H  # q sqrt3/2 = 2 pi / n: periodic with the lattice
+        want = S.nos * math.sqrt(3) * D * math.sin(q * _H)
+    spins = np.stack([np.sin(q * x), np.zeros_like(x), np.cos(q * x)], axis=1)
+    _, E = S.gradient_and_energy(spins)
+    sign = 1 if chirality == 2 else -1
+    assert abs(E - sign * want) < 1e-12 * abs(want)
+
+
+def langevin(x):
+    return 1.0 / math.tanh(x) - 1.0 / x
+
+
+def paramagnet_mz(run, n_spins, dt, n_equil, n_gap, n_samples):
+    """<m_z> of independent spins sampled n_samples times, n_gap steps apart, after n_equil
+    steps; returns the mean and its standard error (samples n_gap apart are independent when
+    n_gap spans a few relaxation times)."""
+    run(n_equil)
+    m = []
+    for _ in range(n_samples):
+        s = run(n_gap)
+        m.append(float(np.mean(s[:, 2])))
+    m = np.array(m)
+    return float(m.mean()), float(m.std(ddof=1) / math.sqrt(len(m)))
+
+
+PARAMAGNET = dict(bravais_lattice="sc", n_basis_cells=[32, 32, 1], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+                  external_field_magnitude=20.0, external_field_normal=[0, 0, 1], n_shells_exchange=0,
+                  n_shells_dmi=0, ddi_method="none", llg_dt=0.004, llg_damping=0.3, llg_seed=77)
+
+
+@pytest.mark.parametrize("temperature", [20.0, 60.0])
+def test_thermal_amplitude_langevin(oracle, temperature):
+    """Method_LLG::Prepare_Thermal_Field's amplitude eps = sqrt(2 alpha dt gamma k_B / mu_B) /
+    (1 + alpha^2) times sqrt(T / mu_s) (Method_LLG.cpp:66-110) is the one for which free spins
+    in a field B equilibrate to the Boltzmann distribution: <m_z> = L(mu_s mu_B B / k_B T),
+    the Langevin function.  Heun, alpha = 0.3, B = 20 T, two temperatures; a factor 2 in the
+    noise variance moves <m_z> by more than 20 standard errors."""
+    cfg = dict(PARAMAGNET, llg_temperature=temperature)
+    S = oracle.System(cfg)
+    state = {"s": np.tile([0.0, 0.0, 1.0], (S.nos, 1)), "step": 0}
+
+    def run(k):
+        s, _, _, _ = S.llg(state["s"], oracle.SOLVER_HEUN, k, dt=cfg["llg_dt"], damping=0.3, temperature=temperature,
+                           seed=cfg["llg_seed"], step0=state["step"])
+        state["s"], state["step"] = s, state["step"] + k
+        return s
+
+    mean, se = paramagnet_mz(run, S.nos, cfg["llg_dt"], 1500, 300, 40)
+    want = langevin(2.0 * MU_B_MEV_PER_T * 20.0 / (K_B_MEV_PER_K * temperature))
+    assert abs(mean - want) < max(5 * se, 0.01), (mean, want, se)
