@@ -44,6 +44,20 @@ SPIRIT_API int Spirit_HIP_Gradient_and_Energy( State * state, double * gradient,
                                                int idx_image SPIRIT_DEFAULT( -1 ),
                                                int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* The dense 3N x 3N embedding Hessian of the image's Hamiltonian (row-major, index
+ * 3*ispin + component), evaluated on the GPU: the matrix Hamiltonian_Heisenberg::Hessian
+ * fills (Hamiltonian_Heisenberg.cpp:1119-1245; EMA, MMF and HTST read it), including its
+ * anisotropy row sums on the diagonal and the dipolar block over the positions without
+ * periodic images.  hessian holds 9*nos*nos doubles.  Returns 0 on success. */
+SPIRIT_API int Spirit_HIP_Hamiltonian_Get_Hessian( State * state, double * hessian, int idx_image SPIRIT_DEFAULT( -1 ),
+                                                   int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
+/* out = H v (AoS, 3*nos) for the same matrix, evaluated on the GPU without forming it.
+ * Returns 0 on success. */
+SPIRIT_API int Spirit_HIP_Hamiltonian_Hessian_Vector_Product( State * state, const double * v, double * out,
+                                                              int idx_image SPIRIT_DEFAULT( -1 ),
+                                                              int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
 /* Per-spin energy contributions of the image's current spins (the reference's
  * Hamiltonian_Heisenberg::Energy_Contributions_per_Spin, Hamiltonian_Heisenberg.cpp:262-305,
  * which its C API reaches only through IO_Image_Write_Energy_per_Spin), evaluated on the GPU:

@@ -61,6 +61,9 @@ class OrcParams(ctypes.Structure):
         ("ddi_n_periodic_images", ctypes.c_int * 3),
         ("ddi_pb_zero_padding", ctypes.c_int),
         ("ddi_radius", ctypes.c_double),
+        ("n_quad", ctypes.c_int),
+        ("quad", (ctypes.c_int * 13) * 64),
+        ("quad_Q", ctypes.c_double * 64),
     ]
 
 
@@ -127,6 +130,9 @@ def lib():
         L.orc_gradient_and_energy.argtypes = [vp, P(d), P(d), P(d)]
         L.orc_gradient.argtypes = [vp, P(d), P(d)]
         L.orc_gradient_term.argtypes = [vp, i, P(d), P(d)]
+        L.orc_hessian.argtypes = [vp, P(d), P(d)]
+        L.orc_ddi_cutoff_pairs.argtypes = [vp, P(i), P(d), P(d), i]
+        L.orc_ddi_cutoff_pairs.restype = i
         L.orc_llg_iterate.argtypes = [vp, i, P(OrcLLG), P(d), i, ctypes.c_uint64, P(d), P(d), P(d)]
         L.orc_llg_run.argtypes = [vp, i, P(OrcLLG), P(d), ctypes.c_long, d, P(d), P(d), P(d)]
         L.orc_llg_run.restype = ctypes.c_long
@@ -270,6 +276,14 @@ def params_from_config(cfg, site_K=None):
         p.ddi_n_periodic_images[c] = int(im[c])
     p.ddi_pb_zero_padding = 1 if int(cfg.get("ddi_pb_zero_padding", 1)) else 0
     p.ddi_radius = float(cfg.get("ddi_radius", 0.0))
+    nq = 0
+    for row in cfg.get("quadruplets", []):
+        if float(row[13]) == 0:
+            continue
+        p.quad[nq][:] = [int(x) for x in row[:13]]
+        p.quad_Q[nq] = float(row[13])
+        nq += 1
+    p.n_quad = nq
     return p, keep
 
 
@@ -321,6 +335,13 @@ class System:
         g = np.zeros((self.nos, 3))
         lib().orc_gradient_term(self.h, int(term), _dp(s), _dp(g))
         return g
+
+    def hessian(self, spins=None):
+        """Dense embedding Hessian (3N x 3N), Hamiltonian_Heisenberg::Hessian."""
+        s = np.zeros((self.nos, 3)) if spins is None else np.ascontiguousarray(spins, dtype=np.float64)
+        H = np.zeros((3 * self.nos, 3 * self.nos))
+        lib().orc_hessian(self.h, _dp(s), _dp(H))
+        return H
 
     def llg(self, spins, solver, n_steps, dt=1e-3, damping=0.3, temperature=0.0, seed=0, step0=0,
             direct_minimization=False, noise=None):

@@ -75,6 +75,11 @@ struct orc_sys
     double * d_hat;   /* complex [n_inter][6][M] interleaved */
     double * s_hat;   /* complex [n_atoms][3][M] */
     double * res_hat; /* complex [n_atoms][3][M] */
+    /* DDI (cutoff): pairs within ddi_radius with their distances and unit vectors */
+    int n_ddi_pairs;
+    int * ddi_pairs;    /* [n][5]: i, j, da, db, dc */
+    double * ddi_mag;   /* [n] */
+    double * ddi_n;     /* [n][3] */
     /* scratch for LLG */
     double *f1, *fv1, *f2, *fv2, *sp, *tmp, *xi, *k1, *k2, *k3, *k4, *g;
 };
@@ -774,6 +779,227 @@ static void gradient_ddi_direct( orc_sys * s, const double * spins, double * gra
     }
 }
 
+/* Neighbours::Get_Pairs_in_Radius, Neighbours.cpp:158-246, and DDI_from_Pair, :288-307:
+   every (iatom, jatom, translation) with 0 < |r_j + t - r_i| < radius, iatom outer, then the
+   translations a, b, c ascending, jatom inner; the translation ranges from the lattice's
+   bounding box (Geometry::calculateBounds, Geometry.cpp:729-744, which starts at the origin). */
+static void ddi_cutoff_prepare( orc_sys * s )
+{
+    const orc_params * p = &s->p;
+    const double radius  = p->ddi_radius;
+    s->n_ddi_pairs       = 0;
+    if( fabs( radius ) < 1e-6 )
+        return;
+    double bmin[3] = { 0, 0, 0 }, bmax[3] = { 0, 0, 0 };
+    for( int i = 0; i < s->nos; ++i )
+        for( int q = 0; q < 3; ++q )
+        {
+            double v = s->positions[3 * i + q];
+            if( v < bmin[q] )
+                bmin[q] = v;
+            if( v > bmax[q] )
+                bmax[q] = v;
+        }
+    int mx[3] = { 0, 0, 0 };
+    double tv[3][3];
+    for( int k = 0; k < 3; ++k )
+    {
+        for( int q = 0; q < 3; ++q )
+            tv[k][q] = p->lattice_constant * p->bravais[k][q];
+        double diff = bmax[k] - bmin[k];
+        if( radius > 0 )
+        {
+            if( diff > 0 )
+            {
+                int m = (int)( 1.1 * radius * p->n_cells[k] / diff );
+                mx[k] = p->n_cells[k] - 1 < m ? p->n_cells[k] - 1 : m;
+            }
+        }
+        else
+            mx[k] = p->n_cells[k] - 1;
+        if( sqrt( v3_sqnorm( tv[k] ) ) == 0.0 )
+            mx[k] = 0;
+    }
+    long cap = (long)p->n_cell_atoms * p->n_cell_atoms * ( 2 * mx[0] + 1 ) * ( 2 * mx[1] + 1 ) * ( 2 * mx[2] + 1 );
+    s->ddi_pairs = (int *)malloc( sizeof( int ) * 5 * (size_t)cap );
+    s->ddi_mag   = (double *)malloc( sizeof( double ) * (size_t)cap );
+    s->ddi_n     = (double *)malloc( sizeof( double ) * 3 * (size_t)cap );
+    int n        = 0;
+    for( int ia = 0; ia < p->n_cell_atoms; ++ia )
+    {
+        double pi_[3];
+        basis_position( p, ia, pi_ );
+        for( int i = -mx[0]; i <= mx[0]; ++i )
+            for( int j = -mx[1]; j <= mx[1]; ++j )
+                for( int k = -mx[2]; k <= mx[2]; ++k )
+                    for( int ja = 0; ja < p->n_cell_atoms; ++ja )
+                    {
+                        double pj[3], d[3];
+                        basis_position( p, ja, pj );
+                        for( int q = 0; q < 3; ++q )
+                            pj[q] = ( ( pj[q] + i * tv[0][q] ) + j * tv[1][q] ) + k * tv[2][q];
+                        for( int q = 0; q < 3; ++q )
+                            d[q] = pi_[q] - pj[q];
+                        double dist = sqrt( v3_sqnorm( d ) );
+                        if( dist < radius && dist > 2.220446049250313e-16 )
+                        {
+                            int * e = s->ddi_pairs + 5 * n;
+                            e[0] = ia, e[1] = ja, e[2] = i, e[3] = j, e[4] = k;
+                            /* DDI_from_Pair: r_ij = r_j + t - r_i, its norm and direction */
+                            double r[3];
+                            for( int q = 0; q < 3; ++q )
+                                r[q] = pj[q] - pi_[q];
+                            double m      = sqrt( v3_sqnorm( r ) );
+                            s->ddi_mag[n] = m;
+                            for( int q = 0; q < 3; ++q )
+                                s->ddi_n[3 * n + q] = m > 0 ? r[q] / m : 0.0;
+                            ++n;
+                        }
+                    }
+    }
+    s->n_ddi_pairs = n;
+}
+
+/* Gradient_DDI_Cutoff, Hamiltonian_Heisenberg.cpp:880-918 (pairs in pair order, cells inner) */
+static void gradient_ddi_cutoff( orc_sys * s, const double * spins, double * grad )
+{
+    const orc_params * p = &s->p;
+    const double mult    = ddi_mult();
+    for( int ip = 0; ip < s->n_ddi_pairs; ++ip )
+    {
+        if( !( s->ddi_mag[ip] > 0.0 ) )
+            continue;
+        const double c   = mult / pow( s->ddi_mag[ip], 3.0 );
+        const int * pr   = s->ddi_pairs + 5 * ip;
+        const double * n = s->ddi_n + 3 * ip;
+#pragma omp parallel for
+        for( int icell = 0; icell < s->n_cells_total; ++icell )
+        {
+            int i = pr[0] + icell * p->n_cell_atoms;
+            int j = orc_idx_from_pair( i, p->bc, p->n_cells, p->n_cell_atoms, pr );
+            if( j >= 0 )
+            {
+                const double * sj = spins + 3 * j;
+                double sn         = v3_dot( sj, n );
+                double f          = s->mu_s[j] * s->mu_s[i] * c;
+                for( int q = 0; q < 3; ++q )
+                    grad[3 * i + q] -= f * ( 3 * n[q] * sn - sj[q] );
+            }
+        }
+    }
+}
+
+/* Hamiltonian_Heisenberg::Hessian, Hamiltonian_Heisenberg.cpp:1119-1245: the dense 3N x 3N
+   embedding Hessian (row-major; index 3*ispin + component).  Anisotropy: every (alpha, beta)
+   entry of -2 K n n^T lands on (3i+alpha, 3i+alpha) (:1133-1144), so the diagonal holds the
+   row sums; exchange (:1150-1172) and DMI (:1174-1207) per redundant pair at (i, j); the
+   dipolar block (:1210-1245) as the direct sum over the positions without periodic images,
+   for any DDI method.  The per-site K_i extension is treated as a uniaxial term. */
+void orc_hessian( orc_sys * s, const double * spins, double * H )
+{
+    (void)spins;
+    const orc_params * p = &s->p;
+    const long n3        = 3L * s->nos;
+    const int N          = p->n_cell_atoms;
+    memset( H, 0, sizeof( double ) * (size_t)n3 * (size_t)n3 );
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+    {
+        for( int k = 0; k < p->n_aniso; ++k )
+        {
+            long ispin = (long)icell * N + p->aniso_idx[k];
+            for( int a = 0; a < 3; ++a )
+                for( int b = 0; b < 3; ++b )
+                {
+                    long r = 3 * ispin + a;
+                    H[r * n3 + r] += -2.0 * p->aniso_K[k] * p->aniso_normal[k][a] * p->aniso_normal[k][b];
+                }
+        }
+        if( s->site_K )
+            for( int ib = 0; ib < N; ++ib )
+            {
+                long ispin = (long)icell * N + ib;
+                for( int a = 0; a < 3; ++a )
+                    for( int b = 0; b < 3; ++b )
+                    {
+                        long r = 3 * ispin + a;
+                        H[r * n3 + r] += -2.0 * s->site_K[ispin] * p->site_aniso_normal[a] * p->site_aniso_normal[b];
+                    }
+            }
+    }
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+        for( int k = 0; k < p->n_exch; ++k )
+        {
+            int ispin = p->exch[k][0] + icell * N;
+            int jspin = orc_idx_from_pair( ispin, p->bc, p->n_cells, N, p->exch[k] );
+            if( jspin >= 0 )
+                for( int a = 0; a < 3; ++a )
+                    H[( 3L * ispin + a ) * n3 + 3L * jspin + a] += -p->exch_J[k];
+        }
+    for( int icell = 0; icell < s->n_cells_total; ++icell )
+        for( int k = 0; k < p->n_dmi; ++k )
+        {
+            int ispin = p->dmi[k][0] + icell * N;
+            int jspin = orc_idx_from_pair( ispin, p->bc, p->n_cells, N, p->dmi[k] );
+            if( jspin >= 0 )
+            {
+                long i = 3L * ispin, j = 3L * jspin;
+                double D = p->dmi_D[k];
+                const double * n = p->dmi_normal[k];
+                H[( i + 2 ) * n3 + j + 1] += D * n[0];
+                H[( i + 1 ) * n3 + j + 2] += -D * n[0];
+                H[( i ) * n3 + j + 2] += D * n[1];
+                H[( i + 2 ) * n3 + j] += -D * n[1];
+                H[( i + 1 ) * n3 + j] += D * n[2];
+                H[( i ) * n3 + j + 1] += -D * n[2];
+            }
+        }
+    if( p->ddi_method != ORC_DDI_NONE )
+    {
+        const double mult = ddi_mult();
+#pragma omp parallel for
+        for( int i1 = 0; i1 < s->nos; ++i1 )
+            for( int i2 = 0; i2 < s->nos; ++i2 )
+            {
+                double diff[3];
+                for( int q = 0; q < 3; ++q )
+                    diff[q] = s->positions[3 * i2 + q] - s->positions[3 * i1 + q];
+                double d = sqrt( v3_sqnorm( diff ) );
+                double Dxx = 0, Dxy = 0, Dxz = 0, Dyy = 0, Dyz = 0, Dzz = 0;
+                if( d > 1e-10 )
+                {
+                    double d3 = d * d * d, d5 = d * d * d * d * d;
+                    Dxx += mult * ( 3 * diff[0] * diff[0] / d5 - 1 / d3 );
+                    Dxy += mult * 3 * diff[0] * diff[1] / d5;
+                    Dxz += mult * 3 * diff[0] * diff[2] / d5;
+                    Dyy += mult * ( 3 * diff[1] * diff[1] / d5 - 1 / d3 );
+                    Dyz += mult * 3 * diff[1] * diff[2] / d5;
+                    Dzz += mult * ( 3 * diff[2] * diff[2] / d5 - 1 / d3 );
+                }
+                const double m = -s->mu_s[i1] * s->mu_s[i2];
+                const double T[3][3] = { { Dxx, Dxy, Dxz }, { Dxy, Dyy, Dyz }, { Dxz, Dyz, Dzz } };
+                for( int a = 0; a < 3; ++a )
+                    for( int b = 0; b < 3; ++b )
+                        H[( 3L * i1 + a ) * n3 + 3L * i2 + b] += m * T[a][b];
+            }
+    }
+}
+
+int orc_ddi_cutoff_pairs( const orc_sys * s, int * pairs, double * magnitudes, double * normals, int max_pairs )
+{
+    for( int k = 0; k < s->n_ddi_pairs && k < max_pairs; ++k )
+    {
+        if( pairs )
+            for( int q = 0; q < 5; ++q )
+                pairs[5 * k + q] = s->ddi_pairs[5 * k + q];
+        if( magnitudes )
+            magnitudes[k] = s->ddi_mag[k];
+        if( normals )
+            for( int q = 0; q < 3; ++q )
+                normals[3 * k + q] = s->ddi_n[3 * k + q];
+    }
+    return s->n_ddi_pairs;
+}
+
 /* ------------------------------------------------------------------ */
 /* system                                                              */
 /* ------------------------------------------------------------------ */
@@ -796,6 +1022,8 @@ orc_sys * orc_create( const orc_params * p )
     s->p.site_aniso_K = NULL;
     if( p->ddi_method == ORC_DDI_FFT )
         ddi_prepare( s );
+    else if( p->ddi_method == ORC_DDI_CUTOFF && p->ddi_radius >= 0 )
+        ddi_cutoff_prepare( s );
     size_t n3 = sizeof( double ) * 3 * (size_t)s->nos;
     s->f1     = (double *)malloc( n3 );
     s->fv1    = (double *)malloc( n3 );
@@ -822,6 +1050,9 @@ void orc_destroy( orc_sys * s )
     free( s->d_hat );
     free( s->s_hat );
     free( s->res_hat );
+    free( s->ddi_pairs );
+    free( s->ddi_mag );
+    free( s->ddi_n );
     free( s->f1 );
     free( s->fv1 );
     free( s->f2 );
@@ -938,12 +1169,77 @@ static void grad_dmi( orc_sys * s, const double * sp, double * g )
         }
 }
 
+/* the spins of quadruplet q anchored at cell icell (idx_from_pair from i's spin for j, k, l);
+   returns 0 when one of them leaves an open boundary */
+static int quad_spins( const orc_sys * s, int q, int icell, int out[4] )
+{
+    const orc_params * p = &s->p;
+    const int * Q        = p->quad[q];
+    int ispin            = Q[0] + icell * p->n_cell_atoms;
+    int pj[5]            = { Q[0], Q[1], Q[4], Q[5], Q[6] };
+    int pk[5]            = { Q[0], Q[2], Q[7], Q[8], Q[9] };
+    int pl[5]            = { Q[0], Q[3], Q[10], Q[11], Q[12] };
+    out[0]               = ispin;
+    out[1]               = orc_idx_from_pair( ispin, p->bc, p->n_cells, p->n_cell_atoms, pj );
+    out[2]               = orc_idx_from_pair( ispin, p->bc, p->n_cells, p->n_cell_atoms, pk );
+    out[3]               = orc_idx_from_pair( ispin, p->bc, p->n_cells, p->n_cell_atoms, pl );
+    return out[0] >= 0 && out[1] >= 0 && out[2] >= 0 && out[3] >= 0;
+}
+
+/* Gradient_Quadruplet, Hamiltonian_Heisenberg.cpp:1073-1117 (serial scatter over quadruplets, then
+   cells; the cell loop runs a outer, c inner as da, db, dc there) */
+static void grad_quadruplet( orc_sys * s, const double * sp, double * g )
+{
+    const orc_params * p = &s->p;
+    for( int q = 0; q < p->n_quad; ++q )
+        for( int da = 0; da < p->n_cells[0]; ++da )
+            for( int db = 0; db < p->n_cells[1]; ++db )
+                for( int dc = 0; dc < p->n_cells[2]; ++dc )
+                {
+                    int icell = da + p->n_cells[0] * ( db + p->n_cells[1] * dc );
+                    int t[4];
+                    if( !quad_spins( s, q, icell, t ) )
+                        continue;
+                    const double *si = sp + 3 * t[0], *sj = sp + 3 * t[1], *sk = sp + 3 * t[2], *sl = sp + 3 * t[3];
+                    double Q = p->quad_Q[q], kl = v3_dot( sk, sl ), ij = v3_dot( si, sj );
+                    for( int c = 0; c < 3; ++c )
+                        g[3 * t[0] + c] -= Q * sj[c] * kl;
+                    for( int c = 0; c < 3; ++c )
+                        g[3 * t[1] + c] -= Q * si[c] * kl;
+                    for( int c = 0; c < 3; ++c )
+                        g[3 * t[2] + c] -= Q * ij * sl[c];
+                    for( int c = 0; c < 3; ++c )
+                        g[3 * t[3] + c] -= Q * ij * sk[c];
+                }
+}
+
+void orc_energy_quadruplet( orc_sys * s, const double * sp, double * E )
+{
+    const orc_params * p = &s->p;
+    for( int q = 0; q < p->n_quad; ++q )
+        for( int da = 0; da < p->n_cells[0]; ++da )
+            for( int db = 0; db < p->n_cells[1]; ++db )
+                for( int dc = 0; dc < p->n_cells[2]; ++dc )
+                {
+                    int icell = da + p->n_cells[0] * ( db + p->n_cells[1] * dc );
+                    int t[4];
+                    if( !quad_spins( s, q, icell, t ) )
+                        continue;
+                    double e = 0.25 * p->quad_Q[q] * v3_dot( sp + 3 * t[0], sp + 3 * t[1] )
+                               * v3_dot( sp + 3 * t[2], sp + 3 * t[3] );
+                    for( int r = 0; r < 4; ++r )
+                        E[t[r]] -= e;
+                }
+}
+
 static void grad_ddi( orc_sys * s, const double * sp, double * g )
 {
     if( s->p.ddi_method == ORC_DDI_FFT )
         gradient_ddi_fft( s, sp, g );
     else if( s->p.ddi_method == ORC_DDI_CUTOFF && s->p.ddi_radius < 0 )
         gradient_ddi_direct( s, sp, g );
+    else if( s->p.ddi_method == ORC_DDI_CUTOFF )
+        gradient_ddi_cutoff( s, sp, g );
 }
 
 void orc_gradient_term( orc_sys * s, int term, const double * spins, double * grad )
@@ -957,6 +1253,7 @@ void orc_gradient_term( orc_sys * s, int term, const double * spins, double * gr
         case 4: grad_dmi( s, spins, grad ); break;
         case 5: grad_ddi( s, spins, grad ); break;
         case 6: gradient_ddi_direct( s, spins, grad ); break;
+        case 7: grad_quadruplet( s, spins, grad ); break;
         default: break;
     }
 }
@@ -982,6 +1279,8 @@ void orc_gradient( orc_sys * s, const double * spins, double * grad )
         grad_dmi( s, spins, grad );
     if( s->p.ddi_method != ORC_DDI_NONE )
         grad_ddi( s, spins, grad );
+    if( s->p.n_quad > 0 )
+        grad_quadruplet( s, spins, grad );
 }
 
 /* Deterministic parallel sum: fixed blocks of ORC_SUM_BLOCK terms summed in parallel, the
@@ -1065,6 +1364,21 @@ void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, 
             ext[q] = s->p.field_normal[q] * B;
         grad_zeeman( s, grad );
         E += det_sum( s->nos, term_zeeman, &( struct sum_ctx ){ s, spins, grad, ext } );
+    }
+    /* quadruplets: the reference tests idx_quadruplet > 0 (:754), so they count only when another
+       contribution precedes them in the list of Update_Energy_Contributions (:200-260) */
+    const int others = has_zeeman( s ) || s->p.n_aniso > 0 || s->site_K || s->p.n_cubic > 0 || s->p.n_exch > 0
+                       || s->p.n_dmi > 0 || s->p.ddi_method != ORC_DDI_NONE;
+    if( s->p.n_quad > 0 && others )
+    {
+        grad_quadruplet( s, spins, grad );
+        double * eq = (double *)calloc( (size_t)s->nos, sizeof( double ) );
+        orc_energy_quadruplet( s, spins, eq );
+        double sum = 0;
+        for( int i = 0; i < s->nos; ++i )
+            sum += eq[i];
+        free( eq );
+        E += sum;
     }
     *energy = E;
 }

@@ -23,6 +23,7 @@ extern "C" {
 
 #define ORC_MAX_BASIS 8
 #define ORC_MAX_PAIRS 256
+#define ORC_MAX_QUADS 64
 
 /* Physical constants, core/include/utility/Constants.hpp:18-40 */
 #define ORC_MU_B 0.057883817555
@@ -85,6 +86,12 @@ typedef struct orc_params
     int ddi_n_periodic_images[3];
     int ddi_pb_zero_padding;
     double ddi_radius;
+
+    /* quadruplets (Hamiltonian_Heisenberg.hpp, Dataparser.cpp:521-680): basis atoms i, j, k, l and
+       the translations d_j, d_k, d_l from i's cell, magnitude Q */
+    int n_quad;
+    int quad[ORC_MAX_QUADS][13]; /* i, j, k, l, d_j[3], d_k[3], d_l[3] */
+    double quad_Q[ORC_MAX_QUADS];
 } orc_params;
 
 /* ---------------- geometry / neighbour tables ---------------- */
@@ -124,12 +131,24 @@ void orc_ddi_padded( const orc_sys * s, int out[3] );
    out layout: [b_inter][comp 6][c][b][a] (a fastest). */
 void orc_ddi_dipole_matrices( const orc_sys * s, double * out );
 
+/* the cutoff method's pair list (Neighbours::Get_Pairs_in_Radius, Neighbours.cpp:158-246) with
+   distances and unit vectors (DDI_from_Pair, :288-307); returns the number of pairs */
+int orc_ddi_cutoff_pairs( const orc_sys * s, int * pairs, double * magnitudes, double * normals, int max_pairs );
+
+/* E_Quadruplet per spin, Hamiltonian_Heisenberg.cpp:513-557 (added into energy, nos values) */
+void orc_energy_quadruplet( orc_sys * s, const double * spins, double * energy );
+
+/* Hamiltonian_Heisenberg::Hessian, Hamiltonian_Heisenberg.cpp:1119-1245: dense 3N x 3N,
+   row-major, index 3*ispin + component */
+void orc_hessian( orc_sys * s, const double * spins, double * H );
+
 /* Gradient_and_Energy, Hamiltonian_Heisenberg.cpp:704-766 */
 void orc_gradient_and_energy( orc_sys * s, const double * spins, double * grad, double * energy );
 /* Gradient (Zeeman first), Hamiltonian_Heisenberg.cpp:670-702 */
 void orc_gradient( orc_sys * s, const double * spins, double * grad );
-/* Single contributions (gradient added into grad): 0 zeeman 1 aniso 2 cubic 3 exchange 4 dmi 5 ddi(fft/cutoff)
-   6 ddi-direct (Gradient_DDI_Direct, :1016-1071) */
+/* Single contributions (gradient added into grad): 0 zeeman 1 aniso 2 cubic 3 exchange 4 dmi
+   5 ddi (the configured method: fft, cutoff with radius >= 0, direct for a negative radius)
+   6 ddi-direct (Gradient_DDI_Direct, :1016-1071) 7 quadruplets (Gradient_Quadruplet, :1073-1117) */
 void orc_gradient_term( orc_sys * s, int term, const double * spins, double * grad );
 
 /* ---------------- LLG ---------------- */

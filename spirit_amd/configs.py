@@ -7,7 +7,8 @@ n_basis_cells, lattice_constant, boundary_conditions, mu_s,
 external_field_magnitude/normal, anisotropy_magnitude/normal,
 cubic_anisotropy_magnitude, hamiltonian, n_shells_exchange, jij, n_shells_dmi,
 dij, dm_chirality, pairs (heisenberg_pairs rows i j da db dc Dx Dy Dz J),
-ddi_method, ddi_n_periodic_images, ddi_pb_zero_padding, ddi_radius, llg_*.
+ddi_method, ddi_n_periodic_images, ddi_pb_zero_padding, ddi_radius, quadruplets (rows
+i j da_j db_j dc_j k da_k db_k dc_k l da_l db_l dc_l Q), llg_*.
 """
 
 
@@ -64,6 +65,12 @@ def config_text(cfg):
     lines.append("ddi_n_periodic_images %s" % _vec(cfg.get("ddi_n_periodic_images", [4, 4, 4])))
     lines.append("ddi_pb_zero_padding %d" % int(cfg.get("ddi_pb_zero_padding", 1)))
     lines.append("ddi_radius %r" % float(cfg.get("ddi_radius", 0.0)))
+    if cfg.get("quadruplets"):
+        quads = cfg["quadruplets"]
+        lines.append("n_interaction_quadruplets %d" % len(quads))
+        lines.append("i j da_j db_j dc_j k da_k db_k dc_k l da_l db_l dc_l Q")
+        for q in quads:
+            lines.append(" ".join(repr(float(x)) if isinstance(x, float) else str(x) for x in q))
     for k in ("llg_seed", "llg_dt", "llg_damping", "llg_temperature", "llg_force_convergence", "llg_n_iterations",
               "llg_n_iterations_log", "llg_n_iterations_amortize", "gneb_spring_constant", "gneb_force_convergence",
               "gneb_n_iterations", "gneb_n_iterations_log", "gneb_moving_endpoints", "gneb_translating_endpoints",

@@ -2,6 +2,7 @@
 // and the HIP-only extensions.  Reference: core/src/Spirit/{Hamiltonian,System,Geometry,
 // Parameters_LLG,Parameters_GNEB,Quantities,Constants,Version,Log}.cpp.
 #include "api_common.hpp"
+#include "hessian.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -155,8 +156,9 @@ try
     h.ddi_images          = { n_periodic_images[0], n_periodic_images[1], n_periodic_images[2] };
     h.ddi_radius          = cutoff_radius;
     h.ddi_pb_zero_padding = pb_zero_padding;
-    if( ddi_method != DDI_NONE && ddi_method != DDI_FFT )
-        log_message( 3, "Hamiltonian_Set_DDI: only the FFT method is evaluated by the HIP engine" );
+    if( ddi_method == DDI_FMM )
+        log_message( 2, "Hamiltonian_Set_DDI: the FMM method has no dipolar field (as Gradient_DDI, "
+                        "Hamiltonian_Heisenberg.cpp:866-878, evaluates none for it)" );
     h.update_interactions( *image->geometry );
     notify_image_changed( state, idx_image, false, true, false );
 }
@@ -1483,6 +1485,40 @@ catch( ... )
     return 1;
 }
 
+extern "C" int Spirit_HIP_Hamiltonian_Get_Hessian( State * state, double * hessian, int idx_image,
+                                                   int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    if( !hessian )
+        throw api_error( 2, "Spirit_HIP_Hamiltonian_Get_Hessian: no output array" );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    dense_hessian( *image->geometry, *image->hamiltonian, hessian );
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 1;
+}
+
+extern "C" int Spirit_HIP_Hamiltonian_Hessian_Vector_Product( State * state, const double * v, double * out,
+                                                              int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    if( !v || !out )
+        throw api_error( 2, "Spirit_HIP_Hamiltonian_Hessian_Vector_Product: no input or output array" );
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    hessian_vector_product( *image->geometry, *image->hamiltonian, v, out );
+    return 0;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 1;
+}
+
 extern "C" double Spirit_HIP_Get_Last_Step_Time( State * state, int idx_image, int idx_chain ) noexcept
 try
 {
@@ -1551,8 +1587,19 @@ try
         return 0;
     const DdiDev & d = plan->dev();
     char msg[256];
-    std::snprintf( msg, sizeof( msg ), "padded %dx%dx%d; spectrum %s; stored %dx%dx%d; pass C kernel %d tile %d (%.3f ms)",
-                   d.Pa, d.Pb, d.Pc, plan->spectrum_format(), d.KB, d.Ha, d.KC, d.c_kind, d.c_tk, plan->pass_c_ms() );
+    if( plan->real_space() )
+    {
+        const DdiRsDev & r = plan->rs_dev();
+        if( r.method == DDI_RS_DIRECT )
+            std::snprintf( msg, sizeof( msg ), "direct sum over %d spins; periodic images %d %d %d", r.nos, r.img[0],
+                           r.img[1], r.img[2] );
+        else
+            std::snprintf( msg, sizeof( msg ), "cutoff: %d pairs", r.n_pairs );
+    }
+    else
+        std::snprintf( msg, sizeof( msg ),
+                       "padded %dx%dx%d; spectrum %s; stored %dx%dx%d; pass C kernel %d tile %d (%.3f ms)", d.Pa, d.Pb,
+                       d.Pc, plan->spectrum_format(), d.KB, d.Ha, d.KC, d.c_kind, d.c_tk, plan->pass_c_ms() );
     if( buffer && length > 0 )
     {
         std::strncpy( buffer, msg, (size_t)length - 1 );

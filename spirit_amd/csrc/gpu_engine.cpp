@@ -126,7 +126,7 @@ std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h )
 
 size_t DdiPlan::bytes() const
 {
-    return X.size() + Dh.size();
+    return X.size() + Dh.size() + b_pos.size() + b_pairs.size();
 }
 
 const char * DdiPlan::spectrum_format() const
@@ -142,8 +142,77 @@ const char * DdiPlan::spectrum_format() const
     return "complex";
 }
 
+// Cutoff pairs (from Hamiltonian::update_interactions) or the positions of the direct sum.
+void DdiPlan::build_real_space( const Geometry & g, const Hamiltonian & h )
+{
+    if( g.nb > MAX_BASIS )
+        throw std::runtime_error( "DDI: too many basis atoms" );
+    rs        = DdiRsDev{};
+    rs.method = h.ddi_radius < 0 ? DDI_RS_DIRECT : DDI_RS_CUTOFF;
+    rs.nos    = g.nos;
+    rs.nb     = g.nb;
+    rs.Na     = g.n_cells[0];
+    rs.Nb     = g.n_cells[1];
+    rs.Nc     = g.n_cells[2];
+    for( int k = 0; k < 3; ++k )
+    {
+        rs.bc[k]  = h.bc[k];
+        rs.img[k] = h.bc[k] ? h.ddi_images[k] : 0;
+        for( int c = 0; c < 3; ++c )
+            rs.bv[k][c] = g.bravais[k][c];
+    }
+    rs.mult = mu_0 * mu_B * mu_B / ( 4 * Pi * 1e-30 );
+    rs.lc   = g.lattice_constant;
+    for( int b = 0; b < g.nb; ++b )
+        rs.mu[b] = g.mu_basis[b];
+    rs.div_nb = make_fast_div( g.nb );
+    rs.div_Na = make_fast_div( g.n_cells[0] );
+    rs.div_Nb = make_fast_div( g.n_cells[1] );
+    if( rs.method == DDI_RS_DIRECT )
+    {
+        std::vector<double> soa( 3 * (size_t)g.nos );
+        aos_to_soa( g.positions.data(), soa.data(), g.nos );
+        b_pos.resize( sizeof( double ) * soa.size() );
+        HIPCHECK( hipMemcpy( b_pos.as<void>(), soa.data(), sizeof( double ) * soa.size(), hipMemcpyHostToDevice ) );
+        rs.pos = b_pos.as<double>();
+    }
+    else
+    {
+        std::vector<DdiPairDev> pv;
+        for( size_t k = 0; k < h.ddi_pairs.size(); ++k )
+        {
+            // Gradient_DDI_Cutoff skips pairs of zero length (Hamiltonian_Heisenberg.cpp:887)
+            if( !( h.ddi_dist[k] > 0.0 ) )
+                continue;
+            const Pair & p = h.ddi_pairs[k];
+            DdiPairDev e{};
+            e.i  = p.i;
+            e.j  = p.j;
+            e.da = p.t[0];
+            e.db = p.t[1];
+            e.dc = p.t[2];
+            e.c  = rs.mult / std::pow( h.ddi_dist[k], 3.0 );
+            for( int c = 0; c < 3; ++c )
+                e.n[c] = h.ddi_n[k][c];
+            pv.push_back( e );
+        }
+        rs.n_pairs = (int)pv.size();
+        b_pairs.resize( sizeof( DdiPairDev ) * std::max<size_t>( 1, pv.size() ) );
+        if( !pv.empty() )
+            HIPCHECK( hipMemcpy( b_pairs.as<void>(), pv.data(), sizeof( DdiPairDev ) * pv.size(),
+                                 hipMemcpyHostToDevice ) );
+        rs.pairs = b_pairs.as<DdiPairDev>();
+    }
+}
+
 void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
 {
+    if( h.ddi_method == DDI_CUTOFF )
+    {
+        build_real_space( g, h );
+        return;
+    }
+    rs     = DdiRsDev{};
     auto P = ddi_padded_dims( g, h );
     if( g.nb > MAX_BASIS )
         throw std::runtime_error( "DDI: too many basis atoms" );
@@ -326,13 +395,22 @@ void DdiPlan::choose_pass_c( hipStream_t st )
 void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0,
                      hipEvent_t evc1 )
 {
+    if( rs.method )
+    {
+        if( evc0 )
+            HIPCHECK( hipEventRecord( evc0, st ) );
+        ddi_rs_launch_field( rs, spins_soa, h_soa, st );
+        if( evc1 )
+            HIPCHECK( hipEventRecord( evc1, st ) );
+        return;
+    }
     ddi_launch_field( d, spins_soa, h_soa, nos, st, evc0, evc1 );
 }
 
 // ---------------------------------------------------------------------------
 // Hamiltonian upload
 // ---------------------------------------------------------------------------
-static FastDiv make_fast_div( int d )
+FastDiv make_fast_div( int d )
 {
     if( d < 1 )
         throw std::runtime_error( "lattice dimension must be positive" );
@@ -461,7 +539,7 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
         for( int c = 0; c < 3; ++c )
             H.site_n[c] = h.site_n[c];
     }
-    H.has_ddi = h.ddi_method == DDI_FFT ? 1 : 0;
+    H.has_ddi = ( h.ddi_method == DDI_FFT || h.ddi_method == DDI_CUTOFF ) ? 1 : 0;
     return pv;
 }
 
@@ -528,7 +606,7 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
         n_blocks += I.n_blocks;
         off += sp.geometry->nos;
         imgs_host.push_back( I );
-        if( sp.hamiltonian->ddi_method == DDI_FFT )
+        if( sp.hamiltonian->ddi_method == DDI_FFT || sp.hamiltonian->ddi_method == DDI_CUTOFF )
         {
             ddi.emplace_back( new DdiPlan() );
             ddi.back()->build( *sp.geometry, *sp.hamiltonian, stream );
@@ -536,9 +614,7 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
         }
         else
         {
-            if( sp.hamiltonian->ddi_method != DDI_NONE )
-                throw std::runtime_error( "DDI method " + std::to_string( sp.hamiltonian->ddi_method )
-                                          + " is not available in the HIP engine (use FFT or none)" );
+            // FMM: Gradient_DDI (Hamiltonian_Heisenberg.cpp:866-878) evaluates no field for it
             ddi.emplace_back( nullptr );
         }
     }
@@ -728,7 +804,7 @@ void LLGRunner::refresh_hamiltonian()
         const uint64_t old_ddi = hams[k]->ddi_version;
         hams[k]->upload( *specs[k].geometry, *specs[k].hamiltonian );
         imgs_host[k].ham = hams[k]->dev;
-        if( specs[k].hamiltonian->ddi_method == DDI_FFT )
+        if( specs[k].hamiltonian->ddi_method == DDI_FFT || specs[k].hamiltonian->ddi_method == DDI_CUTOFF )
         {
             if( !ddi[k] || old_ddi != specs[k].hamiltonian->ddi_version )
             {

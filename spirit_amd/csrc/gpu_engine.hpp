@@ -2,6 +2,7 @@
 #pragma once
 
 #include "ddi.hpp"
+#include "ddi_rs.hpp"
 #include "gneb.hpp"
 #include "host_core.hpp"
 #include "llg.hpp"
@@ -20,6 +21,8 @@ void aos_to_soa( const double * aos, double * soa, long long n );
 void soa_to_aos( const double * soa, double * aos, long long n );
 
 void hip_check( hipError_t e, const char * what );
+// division by an invariant d without hardware division (device_types.hpp FastDiv)
+FastDiv make_fast_div( int d );
 #define HIPCHECK( x ) ::spirit_hip::hip_check( ( x ), #x )
 
 // Owning device allocation.
@@ -84,9 +87,22 @@ public:
     {
         return c_ms;
     }
+    // the real-space methods (ddi_method cutoff: pairs within the radius, or the direct sum
+    // for a negative radius) instead of the FFT convolution
+    bool real_space() const
+    {
+        return rs.method != 0;
+    }
+    const DdiRsDev & rs_dev() const
+    {
+        return rs;
+    }
 
 private:
+    void build_real_space( const Geometry & g, const Hamiltonian & h );
     void choose_pass_c( hipStream_t st );
+    DdiRsDev rs{};
+    DeviceBuffer b_pos, b_pairs;
     DdiDev d{};
     DhatSymmetry sym{};
     double c_ms = 0;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <fstream>
 #include <random>
@@ -68,10 +69,11 @@ void Geometry::generate()
                     }
                     mu_s[ispin] = mu_basis[ia];
                 }
+    // Geometry::calculateBounds (Geometry.cpp:729-744): the bounds start at the origin
     for( int c = 0; c < 3; ++c )
     {
-        bounds_min[c] = DBL_MAX;
-        bounds_max[c] = -DBL_MAX;
+        bounds_min[c] = 0;
+        bounds_max[c] = 0;
     }
     for( int i = 0; i < nos; ++i )
         for( int c = 0; c < 3; ++c )
@@ -82,18 +84,28 @@ void Geometry::generate()
     for( int c = 0; c < 3; ++c )
     {
         center[c]          = 0.5 * ( bounds_min[c] + bounds_max[c] );
-        cell_bounds_min[c] = DBL_MAX;
-        cell_bounds_max[c] = -DBL_MAX;
+        cell_bounds_min[c] = 0;
+        cell_bounds_max[c] = 0;
     }
-    for( int ia = 0; ia < nb; ++ia )
-    {
-        double p[3];
-        basis_position( ia, p );
-        for( int c = 0; c < 3; ++c )
+    // Geometry::calculateUnitCellBounds (Geometry.cpp:745-769): half the extent of the basis
+    // atoms shifted by +-a_k (and the origin)
+    for( int k = 0; k < 3; ++k )
+        for( int ia = 0; ia < nb; ++ia )
         {
-            cell_bounds_min[c] = std::min( cell_bounds_min[c], p[c] );
-            cell_bounds_max[c] = std::max( cell_bounds_max[c], p[c] );
+            double p[3];
+            basis_position( ia, p );
+            for( int c = 0; c < 3; ++c )
+            {
+                const double t  = lattice_constant * bravais[k][c];
+                const double n1 = p[c] + t, n2 = p[c] - t;
+                cell_bounds_min[c] = std::min( cell_bounds_min[c], std::min( n1, n2 ) );
+                cell_bounds_max[c] = std::max( cell_bounds_max[c], std::max( n1, n2 ) );
+            }
         }
+    for( int c = 0; c < 3; ++c )
+    {
+        cell_bounds_min[c] *= 0.5;
+        cell_bounds_max[c] *= 0.5;
     }
     dimensionality = 0;
     for( int c = 0; c < 3; ++c )
@@ -269,6 +281,62 @@ Vec3 dmi_normal_from_pair( const Geometry & g, const Pair & p, int chirality )
 }
 
 // Hamiltonian_Heisenberg::Update_Interactions, Hamiltonian_Heisenberg.cpp:101-198
+void pairs_in_radius( const Geometry & g, double radius, std::vector<Pair> & pairs, std::vector<double> & dist,
+                      std::vector<Vec3> & normal )
+{
+    pairs.clear();
+    dist.clear();
+    normal.clear();
+    if( std::abs( radius ) < 1e-6 )
+    {
+        log_message( 0, "Generating pairs within a radius of less than 1e-06 is not supported" );
+        return;
+    }
+    // translation vectors and how many of them can reach the radius (from the lattice extent)
+    Vec3 tv[3];
+    int span[3] = { 0, 0, 0 };
+    for( int k = 0; k < 3; ++k )
+    {
+        for( int c = 0; c < 3; ++c )
+            tv[k][c] = g.lattice_constant * g.bravais[k][c];
+        const double extent = g.bounds_max[k] - g.bounds_min[k];
+        if( radius > 0 )
+        {
+            if( extent > 0 )
+                span[k] = std::min( g.n_cells[k] - 1, (int)( 1.1 * radius * g.n_cells[k] / extent ) );
+        }
+        else
+            span[k] = g.n_cells[k] - 1;
+        if( std::sqrt( tv[k][0] * tv[k][0] + tv[k][1] * tv[k][1] + tv[k][2] * tv[k][2] ) == 0.0 )
+            span[k] = 0;
+    }
+    const double eps = std::numeric_limits<double>::epsilon();
+    for( int ia = 0; ia < g.nb; ++ia )
+    {
+        double ri[3];
+        g.basis_position( ia, ri );
+        for( int ta = -span[0]; ta <= span[0]; ++ta )
+            for( int tb = -span[1]; tb <= span[1]; ++tb )
+                for( int tc = -span[2]; tc <= span[2]; ++tc )
+                    for( int ja = 0; ja < g.nb; ++ja )
+                    {
+                        double rj[3];
+                        g.basis_position( ja, rj );
+                        for( int c = 0; c < 3; ++c )
+                            rj[c] = ( ( rj[c] + ta * tv[0][c] ) + tb * tv[1][c] ) + tc * tv[2][c];
+                        const double d0 = ri[0] - rj[0], d1 = ri[1] - rj[1], d2 = ri[2] - rj[2];
+                        const double d  = std::sqrt( d0 * d0 + d1 * d1 + d2 * d2 );
+                        if( !( d < radius && d > eps ) )
+                            continue;
+                        pairs.push_back( Pair{ ia, ja, { ta, tb, tc } } );
+                        const double r0 = rj[0] - ri[0], r1 = rj[1] - ri[1], r2 = rj[2] - ri[2];
+                        const double m  = std::sqrt( r0 * r0 + r1 * r1 + r2 * r2 );
+                        dist.push_back( m );
+                        normal.push_back( m > 0 ? Vec3{ r0 / m, r1 / m, r2 / m } : Vec3{ 0, 0, 0 } );
+                    }
+    }
+}
+
 void Hamiltonian::update_interactions( const Geometry & g )
 {
     exch_pairs.clear();
@@ -323,6 +391,11 @@ void Hamiltonian::update_interactions( const Geometry & g )
             dmi_shell.push_back( (int)k );
         }
     }
+    ddi_pairs.clear();
+    ddi_dist.clear();
+    ddi_n.clear();
+    if( ddi_method == DDI_CUTOFF && ddi_radius >= 0 )
+        pairs_in_radius( g, ddi_radius, ddi_pairs, ddi_dist, ddi_n );
     ddi_bc = bc;
     ++version;
     ++ddi_version;

@@ -63,6 +63,10 @@ std::vector<double> shell_radii( const Geometry & g, int n_shells );
 void neighbours_in_shells( const Geometry & g, int n_shells, bool redundant, std::vector<Pair> & pairs,
                            std::vector<int> & shells );
 Vec3 dmi_normal_from_pair( const Geometry & g, const Pair & p, int chirality );
+// Neighbours::Get_Pairs_in_Radius (Neighbours.cpp:158-246) with DDI_from_Pair (:288-307): every
+// pair closer than `radius` (none for |radius| < 1e-6), its distance and unit vector r_ij / |r_ij|
+void pairs_in_radius( const Geometry & g, double radius, std::vector<Pair> & pairs, std::vector<double> & dist,
+                      std::vector<Vec3> & normal );
 
 enum DdiMethod
 {
@@ -111,6 +115,11 @@ struct Hamiltonian
     std::array<int, 3> ddi_images{ 4, 4, 4 };
     bool ddi_pb_zero_padding = true;
     double ddi_radius        = 0;
+    // the cutoff method's pairs (ddi_method == DDI_CUTOFF with ddi_radius >= 0), from
+    // update_interactions (Hamiltonian_Heisenberg.cpp:178-190)
+    std::vector<Pair> ddi_pairs;
+    std::vector<double> ddi_dist;
+    std::vector<Vec3> ddi_n;
 
     // boundary conditions as seen by the last update_interactions (the DDI padding is
     // prepared there, Hamiltonian_Heisenberg.cpp:1501-1520, so later BC changes keep it)

@@ -93,6 +93,29 @@ def gradient_and_energy(p_state, nos, idx_image=-1, idx_chain=-1):
     return g, e.value
 
 
+_Get_Hessian = _fn("Spirit_HIP_Hamiltonian_Get_Hessian", [_vp, _PD, _i, _i], _i)
+_Hessian_VP = _fn("Spirit_HIP_Hamiltonian_Hessian_Vector_Product", [_vp, _PD, _PD, _i, _i], _i)
+
+
+def hessian(p_state, nos, idx_image=-1, idx_chain=-1):
+    """Dense embedding Hessian (3 nos x 3 nos) of the image's Hamiltonian, computed on the GPU
+    (Hamiltonian_Heisenberg::Hessian)."""
+    H = np.zeros((3 * nos, 3 * nos))
+    if _Get_Hessian(ctypes.c_void_p(p_state), H.ctypes.data_as(_PD), _i(idx_image), _i(idx_chain)) != 0:
+        raise RuntimeError("Spirit_HIP_Hamiltonian_Get_Hessian failed")
+    return H
+
+
+def hessian_vector_product(p_state, v, idx_image=-1, idx_chain=-1):
+    """H v for the embedding Hessian, without forming it; v of shape (nos, 3)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    if _Hessian_VP(ctypes.c_void_p(p_state), v.ctypes.data_as(_PD), out.ctypes.data_as(_PD), _i(idx_image),
+                   _i(idx_chain)) != 0:
+        raise RuntimeError("Spirit_HIP_Hamiltonian_Hessian_Vector_Product failed")
+    return out
+
+
 _Energy_per_Spin = _fn("Spirit_HIP_Get_Energy_Contributions_per_Spin", [_vp, _PD, _i, _i], _i)
 
 

@@ -88,6 +88,18 @@ GRAD_CASES = {
                                     ddi_n_periodic_images=[2, 1, 0], ddi_pb_zero_padding=0),
     "ddi_odd_sizes": dict(bravais_lattice="sc", n_basis_cells=[7, 5, 3], boundary_conditions=[0, 0, 0], mu_s=[2.0],
                           ddi_method="fft", ddi_n_periodic_images=[0, 0, 0]),
+    # real-space methods (ddi_method cutoff): pairs within the radius wrapping periodic axes,
+    # and the direct sum (negative radius) with periodic images
+    "ddi_cutoff_periodic": dict(bravais_lattice="sc", n_basis_cells=[9, 8, 3], boundary_conditions=[1, 1, 0],
+                                mu_s=[2.0], external_field_magnitude=0.5, n_shells_exchange=1, jij=[2.0],
+                                n_shells_dmi=1, dij=[0.5], dm_chirality=1, ddi_method="cutoff", ddi_radius=2.5),
+    "ddi_cutoff_two_basis_open": dict(GOLD["physics_ddi"]["cfg"], boundary_conditions=[0, 0, 0], ddi_method="cutoff",
+                                      ddi_radius=3.1),
+    "ddi_direct_periodic_images": dict(bravais_lattice="sc", n_basis_cells=[6, 5, 2], boundary_conditions=[1, 1, 0],
+                                       mu_s=[2.0], n_shells_exchange=1, jij=[1.0], ddi_method="cutoff",
+                                       ddi_radius=-1.0, ddi_n_periodic_images=[2, 1, 3]),
+    "ddi_direct_hex_open": dict(bravais_lattice="hex2d", n_basis_cells=[17, 13, 1], boundary_conditions=[0, 0, 0],
+                                mu_s=[1.5], anisotropy_magnitude=0.2, ddi_method="cutoff", ddi_radius=-1.0),
 }
 
 
@@ -199,6 +211,22 @@ def test_llg_ddi_trajectory(engine, oracle, solver):
     n = 60
     s, _, _ = _run_llg(e, SOLVERS[solver], n)
     s_ref, _, _, _ = S.llg(s0, SOLVERS[solver], n, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
+    assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
+
+
+@pytest.mark.parametrize("radius", [2.2, -1.0])
+def test_llg_ddi_real_space_trajectory(engine, oracle, radius):
+    """Heun with the real-space dipolar field (cutoff pairs within 2.2, and the direct sum) on
+    a 10x8x3 open film: spins after 40 steps against the oracle."""
+    cfg = dict(configs.scaled(configs.FILM_128, [10, 8, 3], llg_temperature=0.0), ddi_method="cutoff",
+               ddi_radius=radius)
+    print_case("llg ddi real space", radius)
+    e = engine(cfg)
+    S = oracle.System(cfg)
+    s0 = oracle.random_spins(S.nos, 15)
+    e.set_spins(s0)
+    s, _, _ = _run_llg(e, SOLVERS["heun"], 40)
+    s_ref, _, _, _ = S.llg(s0, SOLVERS["heun"], 40, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
     assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
 
 

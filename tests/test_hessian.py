@@ -132,3 +132,40 @@ def test_hessian_bad_file_is_reported_not_raised(make_state, tmp_path):
     p = make_state(SC)
     hamiltonian.write_hessian(p, str(tmp_path / "no_such_dir" / "h.txt"))
     assert not os.path.exists(tmp_path / "no_such_dir")
+
+
+def _dipole_jacobian(pos, mu):
+    """Jacobian of the point-dipole gradient: block (i, j) = -mu0 muB^2 mu_i mu_j / (4 pi |r|^3 1e-30)
+    (3 r^ r^T - I) for i != j (positions in Angstrom), zero on the diagonal blocks."""
+    pref = 2.0133545e-28 * 0.057883817555 ** 2 / (4 * np.pi * 1e-30)
+    n = len(pos)
+    J = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for j in range(n):
+            if i != j:
+                r = pos[j] - pos[i]
+                d = np.linalg.norm(r)
+                u = r / d
+                J[3 * i:3 * i + 3, 3 * j:3 * j + 3] = -pref * mu[i] * mu[j] / d ** 3 * (3 * np.outer(u, u) - np.eye(3))
+    return J
+
+
+@pytest.mark.parametrize("name,cfg", [("sc_mixed_bc", SC), ("skewed_basis", SKEW)])
+def test_oracle_dense_hessian(oracle, name, cfg):
+    """The oracle's dense Hessian (Hamiltonian_Heisenberg::Hessian): exchange and DMI Jacobians
+    plus the anisotropy row sums on the diagonal."""
+    S = oracle.System(cfg)
+    np.testing.assert_allclose(S.hessian(), expected_hessian(S), rtol=1e-14, atol=1e-14)
+
+
+def test_oracle_dense_hessian_dipolar_block(oracle):
+    """With a dipolar method set, the dense Hessian adds the point-dipole Jacobian over the
+    positions without periodic images (Hamiltonian_Heisenberg.cpp:1210-1245), whatever the
+    method and the boundary conditions."""
+    for method, radius in (("fft", 0.0), ("cutoff", 2.0)):
+        cfg = dict(SKEW, ddi_method=method, ddi_radius=radius, ddi_n_periodic_images=[1, 1, 1])
+        S = oracle.System(cfg)
+        base = oracle.System(dict(cfg, ddi_method="none"))
+        mu = np.array([2.17] * S.nos)
+        J = _dipole_jacobian(S.positions(), mu)
+        np.testing.assert_allclose(S.hessian() - base.hessian(), J, rtol=0, atol=1e-12 * np.abs(J).max())

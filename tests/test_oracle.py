@@ -431,10 +431,11 @@ MU_B_MEV_PER_T = 0.057883817555
 K_B_MEV_PER_K = 0.08617330350
 
 
-def _dipole_energy_closed_form(pos, spins, mu):
+def _dipole_energy_closed_form(pos, spins, mu, radius=None):
     """E = -sum_{i<j} mu0 muB^2 mu_i mu_j / (4 pi |r|^3) [3 (s_i.r^)(s_j.r^) - s_i.s_j], r in
     Angstrom (1e-10 m), and the gradient dE/ds_i = -sum_j mu0 muB^2 mu_i mu_j / (4 pi |r|^3)
-    [3 r^ (r^.s_j) - s_j]: the classical energy of point dipoles mu_i muB s_i."""
+    [3 r^ (r^.s_j) - s_j]: the classical energy of point dipoles mu_i muB s_i (with `radius`,
+    only the pairs closer than it)."""
     pref = MU_0_T2M3_PER_MEV * MU_B_MEV_PER_T ** 2 / (4 * math.pi * 1e-30)
     n = len(pos)
     E = 0.0
@@ -445,6 +446,8 @@ def _dipole_energy_closed_form(pos, spins, mu):
                 continue
             r = pos[j] - pos[i]
             d = np.linalg.norm(r)
+            if radius is not None and d >= radius:
+                continue
             u = r / d
             c = pref * mu[i] * mu[j] / d ** 3
             g[i] -= c * (3 * u * np.dot(u, spins[j]) - spins[j])
@@ -583,3 +586,70 @@ def test_thermal_amplitude_langevin(oracle, temperature):
     mean, se = paramagnet_mz(run, S.nos, cfg["llg_dt"], 1500, 300, 40)
     want = langevin(2.0 * MU_B_MEV_PER_T * 20.0 / (K_B_MEV_PER_K * temperature))
     assert abs(mean - want) < max(5 * se, 0.01), (mean, want, se)
+
+
+def test_ddi_cutoff_two_moments_closed_form(oracle):
+    """The cutoff method (Gradient_DDI_Cutoff, Hamiltonian_Heisenberg.cpp:880-918) on two moments
+    3 Angstrom apart: the same closed forms as the convolution; with a radius below the
+    separation there is no field."""
+    c = MU_0_T2M3_PER_MEV * MU_B_MEV_PER_T ** 2 * 4.0 / (4 * math.pi * 27.0 * 1e-30)
+    x, y, z = np.eye(3)
+    for radius, scale in ((4.0, 1.0), (2.9, 0.0)):
+        cfg = dict(bravais_lattice="sc", n_basis_cells=[2, 1, 1], boundary_conditions=[0, 0, 0], mu_s=[2.0],
+                   lattice_constant=3.0, ddi_method="cutoff", ddi_radius=radius)
+        S = oracle.System(cfg)
+        for s, want in (((x, x), -2 * c), ((z, z), c), ((x, y), 0.0), ((y, -y), -c)):
+            spins = np.array(s, dtype=float)
+            g = S.gradient_term(5, spins)
+            E = 0.5 * float(np.sum(g * spins))
+            assert abs(E - scale * want) <= 1e-12 * c, (radius, s, E, want)
+
+
+def test_ddi_cutoff_pairs_count(oracle):
+    """Pairs within the radius (Neighbours::Get_Pairs_in_Radius): 6 + 12 sites of a simple cubic
+    lattice closer than 1.5, 6 + 12 + 8 closer than 1.8, each once with its unit vector."""
+    import ctypes
+    for radius, want in ((1.5, 18), (1.8, 26), (0.5, 0)):
+        cfg = dict(bravais_lattice="sc", n_basis_cells=[5, 5, 5], boundary_conditions=[0, 0, 0], mu_s=[1.0],
+                   ddi_method="cutoff", ddi_radius=radius)
+        S = oracle.System(cfg)
+        L = oracle.lib()
+        L.orc_ddi_cutoff_pairs.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                           ctypes.c_int]
+        L.orc_ddi_cutoff_pairs.restype = ctypes.c_int
+        n = L.orc_ddi_cutoff_pairs(S.h, None, None, None, 0)
+        assert n == want
+        if n:
+            pr = np.zeros((n, 5), dtype=np.int32)
+            mag = np.zeros(n)
+            nrm = np.zeros((n, 3))
+            L.orc_ddi_cutoff_pairs(S.h, pr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   mag.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   nrm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+            t = pr[:, 2:].astype(float)
+            assert len({tuple(r) for r in pr}) == n
+            np.testing.assert_allclose(mag, np.linalg.norm(t, axis=1), rtol=1e-15)
+            np.testing.assert_allclose(nrm * mag[:, None], t, atol=1e-15)
+            assert (np.linalg.norm(t, axis=1) < radius).all()
+
+
+@pytest.mark.parametrize("radius", [1.2, 3.0, 100.0])
+def test_ddi_cutoff_lattice_closed_form(oracle, radius):
+    """A 4x3x3 open block with a two-atom basis: the cutoff gradient and energy equal the
+    brute-force point-dipole sum over the pairs closer than the radius; a radius beyond the
+    block reproduces the direct sum (Gradient_DDI_Direct)."""
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[4, 3, 3], boundary_conditions=[0, 0, 0], mu_s=[2.0, 1.5],
+               basis=[[0.0, 0.0, 0.0], [0.5, 0.5, 0.25]], lattice_constant=2.5, ddi_method="cutoff",
+               ddi_radius=radius * 2.5)
+    S = oracle.System(cfg)
+    spins = oracle.random_spins(S.nos, 7)
+    pos = S.positions()
+    mu = np.array([2.0, 1.5] * (S.nos // 2))
+    E_ref, g_ref = _dipole_energy_closed_form(pos, spins, mu, radius * 2.5)
+    g = S.gradient_term(5, spins)
+    E = 0.5 * float(np.sum(g * spins))
+    assert abs(E - E_ref) <= 1e-11 * abs(E_ref)
+    np.testing.assert_allclose(g, g_ref, rtol=0, atol=1e-11 * np.abs(g_ref).max())
+    if radius > 10:
+        np.testing.assert_allclose(g, S.gradient_term(6, spins), rtol=0, atol=1e-12 * np.abs(g_ref).max())
