@@ -280,7 +280,8 @@ def params_from_config(cfg, site_K=None):
     for row in cfg.get("quadruplets", []):
         if float(row[13]) == 0:
             continue
-        p.quad[nq][:] = [int(x) for x in row[:13]]
+        r = [int(x) for x in row[:13]]  # i j da_j db_j dc_j k da_k db_k dc_k l da_l db_l dc_l
+        p.quad[nq][:] = [r[0], r[1], r[5], r[9]] + r[2:5] + r[6:9] + r[10:13]
         p.quad_Q[nq] = float(row[13])
         nq += 1
     p.n_quad = nq

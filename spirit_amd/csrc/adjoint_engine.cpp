@@ -20,6 +20,8 @@ AdjointRunner::AdjointRunner( const Geometry & g, const Hamiltonian & h, const L
         throw std::runtime_error( "adjoint: Depondt or Heun only" );
     if( llg.direct_minimization )
         throw std::runtime_error( "adjoint: LLG dynamics only (direct minimisation is not differentiated)" );
+    if( !h.quads.empty() )
+        throw std::runtime_error( "adjoint: quadruplet interactions are not differentiated" );
     if( n_steps < 1 )
         throw std::runtime_error( "adjoint: at least one step" );
     if( ckpt < 1 || ckpt > n_steps )

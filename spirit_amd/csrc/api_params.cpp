@@ -505,6 +505,8 @@ std::vector<std::pair<std::string, int>> spirit_hip::energy_contribution_list( c
         c.push_back( { "DMI", 4 } );
     if( h.ddi_method != DDI_NONE )
         c.push_back( { "DDI", 5 } );
+    if( !h.quads.empty() )
+        c.push_back( { "Quadruplets", 6 } );
     return c;
 }
 
@@ -521,7 +523,7 @@ double spirit_hip::update_image_energy( Image & image, bool store_field, std::ve
         for( size_t k = 0; k < g.size(); ++k )
             image.eff_field[k] = -1 * g[k];
     const size_t nos = (size_t)image.nos();
-    std::vector<double> terms( 6 * nos );
+    std::vector<double> terms( N_ENERGY_TERMS_HOST * nos );
     r.energy_terms( 0, terms.data() );
     image.E_array.clear();
     for( auto & c : energy_contribution_list( image ) )

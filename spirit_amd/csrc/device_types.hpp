@@ -55,6 +55,15 @@ struct PairDev
     double n[3];
 };
 
+// A quadruplet (Vectormath_Defines.hpp:82-86) with its magnitude Q.
+struct QuadDev
+{
+    int i, j, k, l;
+    int dj[3], dk[3], dl[3];
+    int pad;
+    double Q;
+};
+
 // Everything the fused local-field kernel needs for one image.
 struct HamDev
 {
@@ -81,6 +90,9 @@ struct HamDev
     const double * site_K;   // per-site uniaxial K_i (nullable)
     const PairDev * exch;    // n_exch redundant exchange pairs
     const PairDev * dmi;     // n_dmi redundant DMI pairs
+    int n_quad;              // quadruplets
+    int quad_in_step;        // Gradient_and_Energy adds them (Hamiltonian_Heisenberg.cpp:754)
+    const QuadDev * quad;
 };
 
 // LLG parameters of one image (Method_LLG.cpp:132-226, 66-110).

@@ -436,6 +436,31 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
         HIPCHECK( hipMemcpy( pairs.as<void>(), pv.data(), sizeof( PairDev ) * pv.size(), hipMemcpyHostToDevice ) );
     H.exch = pairs.as<PairDev>();
     H.dmi  = pairs.as<PairDev>() + H.n_exch;
+    if( !h.quads.empty() )
+    {
+        std::vector<QuadDev> qv;
+        for( size_t k = 0; k < h.quads.size(); ++k )
+        {
+            const Quadruplet & q = h.quads[k];
+            if( q.i < 0 || q.i >= g.nb || q.j < 0 || q.j >= g.nb || q.k < 0 || q.k >= g.nb || q.l < 0 || q.l >= g.nb )
+                throw std::runtime_error( "quadruplet with a basis index outside the cell" );
+            QuadDev d{};
+            d.i = q.i, d.j = q.j, d.k = q.k, d.l = q.l;
+            for( int c = 0; c < 3; ++c )
+            {
+                d.dj[c] = q.dj[c];
+                d.dk[c] = q.dk[c];
+                d.dl[c] = q.dl[c];
+            }
+            d.Q = h.quad_Q[k];
+            qv.push_back( d );
+        }
+        quads.resize( sizeof( QuadDev ) * qv.size() );
+        HIPCHECK( hipMemcpy( quads.as<void>(), qv.data(), sizeof( QuadDev ) * qv.size(), hipMemcpyHostToDevice ) );
+        H.n_quad       = (int)qv.size();
+        H.quad_in_step = h.quadruplets_in_gradient_and_energy() ? 1 : 0;
+        H.quad         = quads.as<QuadDev>();
+    }
     if( H.has_site_K )
     {
         site_K.resize( sizeof( double ) * g.nos );
@@ -1169,13 +1194,13 @@ void LLGRunner::energy_terms( int img, double * out )
     StepArgs A = STEP_ARGS();
     if( any_ddi )
         ddi_field( A.s );
-    DeviceBuffer e( sizeof( double ) * 6 * (size_t)n_total );
+    DeviceBuffer e( sizeof( double ) * N_ENERGY_TERMS_HOST * (size_t)n_total );
     llg_launch_energy_terms( A, A.s, e.as<double>(), n_blocks, stream );
     HIPCHECK( hipGetLastError() );
     HIPCHECK( hipStreamSynchronize( stream ) );
     const ImageDev & I = imgs_host[img];
     const int nos      = specs[img].geometry->nos;
-    HIPCHECK( hipMemcpy( out, e.as<double>() + 6 * I.off, sizeof( double ) * 6 * (size_t)nos,
+    HIPCHECK( hipMemcpy( out, e.as<double>() + N_ENERGY_TERMS_HOST * I.off, sizeof( double ) * N_ENERGY_TERMS_HOST * (size_t)nos,
                          hipMemcpyDeviceToHost ) );
 }
 

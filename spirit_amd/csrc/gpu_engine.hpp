@@ -123,7 +123,7 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
 struct HamUpload
 {
     HamDev dev{};
-    DeviceBuffer pairs;
+    DeviceBuffer pairs, quads;
     DeviceBuffer site_K;
     uint64_t version     = ~0ull;
     uint64_t ddi_version = ~0ull;
@@ -183,8 +183,8 @@ public:
     void download_effective_field( int img, double * aos );
     // gradient (AoS) and energy of the current device spins of image img
     double gradient( int img, double * grad_aos );
-    // per-spin energy contributions of image img's current spins, out[t * nos + i] for the six
-    // terms Zeeman, anisotropy, cubic anisotropy, exchange, DMI, DDI
+    // per-spin energy contributions of image img's current spins, out[t * nos + i] for the
+    // N_ENERGY_TERMS_HOST terms Zeeman, anisotropy, cubic anisotropy, exchange, DMI, DDI, quadruplets
     void energy_terms( int img, double * out );
     // injected N(0,1) noise for the next `n_steps` steps, AoS per spin over all images
     void set_injected_noise( const double * noise, long n_steps );

@@ -401,6 +401,13 @@ void Hamiltonian::update_interactions( const Geometry & g )
     ++ddi_version;
 }
 
+bool Hamiltonian::quadruplets_in_gradient_and_energy() const
+{
+    return !quads.empty()
+           && ( has_zeeman() || !aniso_idx.empty() || !site_K.empty() || !cubic_idx.empty() || !exch_pairs.empty()
+                || !dmi_pairs.empty() || ddi_method != DDI_NONE );
+}
+
 bool Hamiltonian::has_zeeman() const
 {
     return std::fabs( field_magnitude ) > 1e-60;
@@ -584,6 +591,61 @@ static void normalize_vec( Vec3 & v )
     }
 }
 
+// Quadruplets_from_File (core/src/io/Dataparser.cpp:521-680): after the count (or from the top of
+// a separate file), a header line names the columns (i j da_j db_j dc_j k da_k db_k dc_k l da_l
+// db_l dc_l Q, any order, case-insensitive), then one quadruplet per line; values of absent
+// columns carry over from the previous line, and rows with Q = 0 are dropped.
+static void read_quadruplets( ConfigFile & f, long n, Hamiltonian & h )
+{
+    f.next_line();
+    std::vector<std::string> cols;
+    std::string w;
+    while( f.stream() >> w )
+    {
+        std::transform( w.begin(), w.end(), w.begin(), ::tolower );
+        cols.push_back( w );
+    }
+    const char * names[14] = { "i",    "j",    "da_j", "db_j", "dc_j", "k",    "da_k",
+                               "db_k", "dc_k", "l",    "da_l", "db_l", "dc_l", "q" };
+    int iv[13]             = { 0 };
+    double Q               = 0;
+    for( long r = 0; r < n && f.next_line(); ++r )
+    {
+        for( auto & c : cols )
+        {
+            int which = -1;
+            for( int k = 0; k < 14; ++k )
+                if( c == names[k] )
+                    which = k;
+            if( which == 13 )
+                f.stream() >> Q;
+            else if( which >= 0 )
+                f.stream() >> iv[which];
+            else
+            {
+                std::string dump;
+                f.stream() >> dump;
+            }
+        }
+        if( Q != 0 )
+        {
+            Quadruplet q{};
+            q.i = iv[0];
+            q.j = iv[1];
+            q.k = iv[5];
+            q.l = iv[9];
+            for( int c = 0; c < 3; ++c )
+            {
+                q.dj[c] = iv[2 + c];
+                q.dk[c] = iv[6 + c];
+                q.dl[c] = iv[10 + c];
+            }
+            h.quads.push_back( q );
+            h.quad_Q.push_back( Q );
+        }
+    }
+}
+
 std::shared_ptr<Hamiltonian> hamiltonian_from_config( const std::string & path, const Geometry & g )
 {
     auto h = std::make_shared<Hamiltonian>();
@@ -716,6 +778,33 @@ std::shared_ptr<Hamiltonian> hamiltonian_from_config( const std::string & path, 
                     for( int s = 0; s < n_shells_dmi; ++s )
                         f.stream() >> dij[s];
                 f.read( h->dmi_chirality, "dm_chirality" );
+            }
+            if( f.find( "n_interaction_quadruplets" ) )
+            {
+                long n = 0;
+                f.stream() >> n;
+                read_quadruplets( f, n, *h );
+            }
+            else
+            {
+                std::string qfile;
+                if( f.read( qfile, "interaction_quadruplets_file" ) && !qfile.empty() )
+                {
+                    ConfigFile qf( qfile );
+                    if( !qf.ok() )
+                        log_message( 0, "Unable to read interaction quadruplets from \"" + qfile + "\"" );
+                    else if( qf.find( "n_interaction_quadruplets" ) )
+                    {
+                        long n = 0;
+                        qf.stream() >> n;
+                        read_quadruplets( qf, n, *h );
+                    }
+                    else
+                    {
+                        qf.rewind();
+                        read_quadruplets( qf, 100000000L, *h );
+                    }
+                }
             }
             f.read( ddi, "ddi_method" );
             f.read3( h->ddi_images, "ddi_n_periodic_images" );

@@ -58,6 +58,14 @@ struct Pair
     int t[3];
 };
 
+// Four-spin interaction <ij><kl> (Vectormath_Defines.hpp:82-86): basis atoms i, j, k, l and
+// the translations of j, k and l from i's cell.
+struct Quadruplet
+{
+    int i, j, k, l;
+    int dj[3], dk[3], dl[3];
+};
+
 // Neighbours.cpp:15-156, 249-286
 std::vector<double> shell_radii( const Geometry & g, int n_shells );
 void neighbours_in_shells( const Geometry & g, int n_shells, bool redundant, std::vector<Pair> & pairs,
@@ -111,6 +119,10 @@ struct Hamiltonian
     std::vector<Vec3> dmi_n;
     std::vector<int> dmi_shell;
 
+    // quadruplets with their magnitudes Q (n_interaction_quadruplets / interaction_quadruplets_file)
+    std::vector<Quadruplet> quads;
+    std::vector<double> quad_Q;
+
     int ddi_method = DDI_NONE;
     std::array<int, 3> ddi_images{ 4, 4, 4 };
     bool ddi_pb_zero_padding = true;
@@ -131,6 +143,10 @@ struct Hamiltonian
 
     void update_interactions( const Geometry & g );
     bool has_zeeman() const;
+    // Gradient_and_Energy adds the quadruplets only when idx_quadruplet > 0
+    // (Hamiltonian_Heisenberg.cpp:754), i.e. when another contribution precedes them in the
+    // list of Update_Energy_Contributions (:200-260)
+    bool quadruplets_in_gradient_and_energy() const;
 };
 
 struct LLGParameters
@@ -220,6 +236,11 @@ public:
     bool find( const std::string & keyword );
     // next non-comment line
     bool next_line();
+    // the next next_line() reads the first line again
+    void rewind()
+    {
+        cursor = 0;
+    }
     std::istringstream & stream()
     {
         return iss;

@@ -42,6 +42,9 @@ void llg_launch_step_advance( uint64_t * step, int by, hipStream_t st );
 void llg_launch_gradient( const StepArgs & A, const double * x, double * grad, int n_blocks, hipStream_t st );
 // per-spin energy contributions (Zeeman, anisotropy, cubic, exchange, DMI, DDI), SoA [6][nos] per
 // image at 6*off; the DDI field of x must be in A.h
+// per-spin energy contributions written by llg_launch_energy_terms: Zeeman, anisotropy, cubic
+// anisotropy, exchange, DMI, DDI, quadruplets ([term][spin] per image)
+constexpr int N_ENERGY_TERMS_HOST = 7;
 void llg_launch_energy_terms( const StepArgs & A, const double * x, double * out, int n_blocks, hipStream_t st );
 void llg_launch_observe( const StepArgs & A, int solver, int n_blocks, hipStream_t st );
 // out = F - (F.s) s per spin (one image's SoA fields)

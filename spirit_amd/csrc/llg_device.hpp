@@ -203,6 +203,82 @@ __device__ __forceinline__ void gather_exchange_neighbours( const HamDev & H, co
         }
 }
 
+// The quadruplet part of spin i's gradient, gathered: Gradient_Quadruplet
+// (Hamiltonian_Heisenberg.cpp:1073-1117) scatters, for every quadruplet q and every cell, the terms
+//   i: Q s_j (s_k.s_l),  j: Q s_i (s_k.s_l),  k: Q (s_i.s_j) s_l,  l: Q (s_i.s_j) s_k
+// (subtracted from the gradient).  Spin i collects the terms of every (q, role) whose basis atom
+// is its own, from the cell the quadruplet is anchored at (its own cell shifted back by the
+// role's translation, wrapped along periodic axes); the quadruplet counts only if all four of
+// its spins exist from that anchor.  Returns the sum t; the spin's energy share is -t.s/4
+// (E_Quadruplet, :513-557).
+__device__ __forceinline__ void quadruplet_gradient( const HamDev & H, const double * __restrict__ x, int nos, int ib,
+                                                     int a, int b, int c, double * t )
+{
+    t[0] = t[1] = t[2] = 0;
+    const int N[3] = { H.Na, H.Nb, H.Nc };
+    for( int k = 0; k < H.n_quad; ++k )
+    {
+        const QuadDev & q = H.quad[k];
+        const int basis[4] = { q.i, q.j, q.k, q.l };
+        for( int r = 0; r < 4; ++r )
+        {
+            if( basis[r] != ib )
+                continue;
+            const int * d = r == 1 ? q.dj : r == 2 ? q.dk : q.dl;
+            int o[3]      = { a, b, c };
+            bool ok       = true;
+            if( r > 0 )
+            {
+                const int p[3] = { a, b, c };
+                for( int u = 0; u < 3; ++u )
+                {
+                    int v = p[u] - d[u];
+                    if( H.bc[u] )
+                    {
+                        if( v < 0 )
+                            v += N[u];
+                        if( v >= N[u] )
+                            v -= N[u];
+                    }
+                    else if( v < 0 || v >= N[u] )
+                        ok = false;
+                    o[u] = v;
+                }
+            }
+            if( !ok )
+                continue;
+            const PairDev pj = { q.i, q.j, q.dj[0], q.dj[1], q.dj[2], 0, 0, 0, 0, 0, { 0, 0, 0 } };
+            const PairDev pk = { q.i, q.k, q.dk[0], q.dk[1], q.dk[2], 0, 0, 0, 0, 0, { 0, 0, 0 } };
+            const PairDev pl = { q.i, q.l, q.dl[0], q.dl[1], q.dl[2], 0, 0, 0, 0, 0, { 0, 0, 0 } };
+            const int si_ = q.i + H.nb * ( o[0] + H.Na * ( o[1] + H.Nb * o[2] ) );
+            const int sj_ = pair_neighbour( H, o[0], o[1], o[2], pj );
+            const int sk_ = pair_neighbour( H, o[0], o[1], o[2], pk );
+            const int sl_ = pair_neighbour( H, o[0], o[1], o[2], pl );
+            if( sj_ < 0 || sk_ < 0 || sl_ < 0 )
+                continue;
+            double vi[3], vj[3], vk[3], vl[3];
+            load3( x, nos, si_, vi );
+            load3( x, nos, sj_, vj );
+            load3( x, nos, sk_, vk );
+            load3( x, nos, sl_, vl );
+            if( r < 2 )
+            {
+                const double kl = dot3( vk, vl );
+                const double * o2 = r == 0 ? vj : vi;
+                for( int u = 0; u < 3; ++u )
+                    t[u] += q.Q * o2[u] * kl;
+            }
+            else
+            {
+                const double ij   = dot3( vi, vj );
+                const double * o2 = r == 2 ? vl : vk;
+                for( int u = 0; u < 3; ++u )
+                    t[u] += q.Q * ij * o2[u];
+            }
+        }
+    }
+}
+
 // Gradient_and_Energy for one spin: g (all terms) and its energy share e
 // (1/2 g.s over aniso+exchange+DMI+DDI, plus the cubic and Zeeman energies).
 __device__ __forceinline__ void local_gradient( const HamDev & H, const double * __restrict__ x, int nos, int i,
@@ -332,12 +408,22 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
         double ext[3] = { H.Bn[0] * H.B, H.Bn[1] * H.B, H.Bn[2] * H.B };
         e += -mu * dot3( ext, s );
     }
+    // quadruplets last, :753-765
+    if( H.quad_in_step )
+    {
+        double t[3];
+        quadruplet_gradient( H, x, nos, ib, a, b, c, t );
+        g[0] -= t[0];
+        g[1] -= t[1];
+        g[2] -= t[2];
+        e += -0.25 * dot3( t, s );
+    }
 }
 
 // Energy contributions of spin i (Hamiltonian_Heisenberg::E_Zeeman, E_Anisotropy,
 // E_Cubic_Anisotropy, E_Exchange, E_DMI, E_DDI_FFT, :306-535), in the order
 // ENERGY_TERMS lists; the pair terms count half of each pair at each of its spins.
-constexpr int N_ENERGY_TERMS = 6; // Zeeman, Anisotropy, Cubic anisotropy, Exchange, DMI, DDI
+constexpr int N_ENERGY_TERMS = 7; // Zeeman, Anisotropy, Cubic anisotropy, Exchange, DMI, DDI, Quadruplets
 __device__ __forceinline__ void local_energy_terms( const HamDev & H, const double * __restrict__ x, int nos, int i,
                                                     const double * __restrict__ hddi, double * E )
 {
@@ -397,6 +483,13 @@ __device__ __forceinline__ void local_energy_terms( const HamDev & H, const doub
         // the DDI gradient is -h (h = mu_i (D * mu s)_i / M): E_i = 1/2 s . (-h)
         const double g[3] = { -hddi[i], -hddi[nos + i], -hddi[2 * nos + i] };
         E[5] += 0.5 * dot3( s, g );
+    }
+    // E_Quadruplet, :513-557: each of the four spins takes -Q (s_i.s_j)(s_k.s_l) / 4
+    if( H.n_quad > 0 )
+    {
+        double t[3];
+        quadruplet_gradient( H, x, nos, ib, a, b, c, t );
+        E[6] += -0.25 * dot3( t, s );
     }
 }
 

@@ -356,6 +356,8 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_energy_terms( StepArgs A, con
     }
 }
 
+static_assert( N_ENERGY_TERMS == N_ENERGY_TERMS_HOST, "energy term count" );
+
 void llg_launch_energy_terms( const StepArgs & A, const double * x, double * out, int n_blocks, hipStream_t st )
 {
     hipLaunchKernelGGL( k_energy_terms, dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A, x, out );

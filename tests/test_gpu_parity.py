@@ -100,6 +100,13 @@ GRAD_CASES = {
                                        ddi_radius=-1.0, ddi_n_periodic_images=[2, 1, 3]),
     "ddi_direct_hex_open": dict(bravais_lattice="hex2d", n_basis_cells=[17, 13, 1], boundary_conditions=[0, 0, 0],
                                 mu_s=[1.5], anisotropy_magnitude=0.2, ddi_method="cutoff", ddi_radius=-1.0),
+    # four-spin interactions (n_interaction_quadruplets) on a two-atom basis, periodic and open axes
+    "quadruplets_two_basis": dict(bravais_lattice="sc", n_basis_cells=[6, 5, 4], boundary_conditions=[1, 0, 1],
+                                  mu_s=[1.0, 2.0], basis=[[0, 0, 0], [0.5, 0.5, 0.5]], n_shells_exchange=1,
+                                  jij=[2.0], external_field_magnitude=0.3,
+                                  quadruplets=[[0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 3.0],
+                                               [0, 1, 0, 0, 0, 1, 1, 0, 0, 0, -1, 0, 0, -1.25],
+                                               [1, 1, 0, 0, -1, 0, 2, 0, 0, 1, 0, -1, 1, 0.5]]),
 }
 
 
@@ -228,6 +235,29 @@ def test_llg_ddi_real_space_trajectory(engine, oracle, radius):
     s, _, _ = _run_llg(e, SOLVERS["heun"], 40)
     s_ref, _, _, _ = S.llg(s0, SOLVERS["heun"], 40, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
     assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
+
+
+def test_llg_quadruplets_trajectory(engine, oracle):
+    """Depondt with four-spin interactions, exchange and a field: spins after 100 steps, and a
+    lattice with quadruplets only, whose Gradient_and_Energy leaves them out
+    (Hamiltonian_Heisenberg.cpp:754), so the spins only precess about nothing: they stay put."""
+    cfg = dict(GRAD_CASES["quadruplets_two_basis"], llg_dt=1e-3, llg_damping=0.3, llg_temperature=0.0)
+    print_case("llg quadruplets")
+    e = engine(cfg)
+    S = oracle.System(cfg)
+    s0 = oracle.random_spins(S.nos, 8)
+    e.set_spins(s0)
+    s, _, _ = _run_llg(e, SOLVERS["depondt"], 100)
+    s_ref, _, _, _ = S.llg(s0, SOLVERS["depondt"], 100, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
+    assert np.abs(s - s_ref).max() <= 1e-9, np.abs(s - s_ref).max()
+    assert np.abs(s - s0).max() > 1e-3
+    only = dict(bravais_lattice="sc", n_basis_cells=[5, 4, 3], boundary_conditions=[1, 1, 1], mu_s=[1.0],
+                external_field_magnitude=0.0, quadruplets=[cfg["quadruplets"][0]], llg_dt=1e-3, llg_damping=0.3)
+    e2 = engine(only)
+    s1 = oracle.random_spins(e2.nos, 9)
+    e2.set_spins(s1)
+    s, _, _ = _run_llg(e2, SOLVERS["depondt"], 20)
+    np.testing.assert_array_equal(s, s1)
 
 
 def test_llg_ddi_trajectory_row_tiles(engine, oracle):
@@ -404,10 +434,11 @@ def test_ddi_spectrum_format(engine, case, expect):
 
 
 ENERGY_TERMS = {"Zeeman": (0, 1.0), "Anisotropy": (1, 0.5), "Cubic anisotropy": (2, 0.25), "Exchange": (3, 0.5),
-                "DMI": (4, 0.5), "DDI": (5, 0.5)}
+                "DMI": (4, 0.5), "DDI": (5, 0.5), "Quadruplets": (7, 0.25)}
 
 
-@pytest.mark.parametrize("case", ["square_neel_periodic", "film_bloch_ddi_open", "hex_neel_periodic"])
+@pytest.mark.parametrize("case", ["square_neel_periodic", "film_bloch_ddi_open", "hex_neel_periodic",
+                                  "quadruplets_two_basis", "ddi_cutoff_periodic"])
 def test_energy_contributions_per_spin(engine, oracle, case):
     """Energy_Contributions_per_Spin (Hamiltonian_Heisenberg.cpp:262-535) against the oracle's
     per-term gradients: a term homogeneous of degree n in s has E_i = (1/n) s_i . g_i (Zeeman

@@ -653,3 +653,88 @@ def test_ddi_cutoff_lattice_closed_form(oracle, radius):
     np.testing.assert_allclose(g, g_ref, rtol=0, atol=1e-11 * np.abs(g_ref).max())
     if radius > 10:
         np.testing.assert_allclose(g, S.gradient_term(6, spins), rtol=0, atol=1e-12 * np.abs(g_ref).max())
+
+
+QUADS = [[0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 3.0],      # <i, i+a> <i+b, i+c>  (Input.md's example)
+         [0, 1, 0, 0, 0, 1, 1, 0, 0, 0, -1, 0, 0, -1.25],   # two basis atoms, negative translation
+         [1, 0, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0.0]]      # Q = 0: dropped
+
+
+def _quad_energy(cfg, spins):
+    """E = -sum over quadruplets and anchor cells of Q (s_i.s_j)(s_k.s_l) (core/docs/Input.md), the
+    partners found by index arithmetic with the boundary conditions."""
+    nc = cfg["n_basis_cells"]
+    nb = len(cfg.get("basis", [[0, 0, 0]]))
+    bc = cfg["boundary_conditions"]
+    E = 0.0
+    for row in cfg["quadruplets"]:
+        i, j, dj, k, dk, l, dl, Q = row[0], row[1], row[2:5], row[5], row[6:9], row[9], row[10:13], row[13]
+        for c in range(nc[2]):
+            for b in range(nc[1]):
+                for a in range(nc[0]):
+                    idx = []
+                    for atom, d in ((i, (0, 0, 0)), (j, dj), (k, dk), (l, dl)):
+                        t = [a + d[0], b + d[1], c + d[2]]
+                        ok = True
+                        for u in range(3):
+                            if bc[u]:
+                                t[u] %= nc[u]
+                            elif not 0 <= t[u] < nc[u]:
+                                ok = False
+                        idx.append(atom + nb * (t[0] + nc[0] * (t[1] + nc[1] * t[2])) if ok else None)
+                    if None in idx:
+                        continue
+                    si, sj, sk, sl = (spins[x] for x in idx)
+                    E -= Q * np.dot(si, sj) * np.dot(sk, sl)
+    return E
+
+
+@pytest.mark.parametrize("bc", [[1, 1, 0], [0, 1, 1]])
+def test_quadruplet_energy_and_gradient_closed_form(oracle, bc):
+    """Quadruplets (E_Quadruplet / Gradient_Quadruplet, Hamiltonian_Heisenberg.cpp:513-557, 1073-1117):
+    the per-spin energies add up to E = -sum Q (s_i.s_j)(s_k.s_l) and the gradient is its
+    derivative (central differences of the independent numpy energy)."""
+    import ctypes
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[4, 3, 3], boundary_conditions=bc, mu_s=[1.0, 2.0],
+               basis=[[0, 0, 0], [0.5, 0.5, 0.5]], quadruplets=QUADS)
+    S = oracle.System(cfg)
+    assert S.params.n_quad == 2
+    s = oracle.random_spins(S.nos, 21)
+    E_ref = _quad_energy(cfg, s)
+    e = np.zeros(S.nos)
+    L = oracle.lib()
+    L.orc_energy_quadruplet.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+    L.orc_energy_quadruplet(S.h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert abs(e.sum() - E_ref) <= 1e-12 * abs(E_ref)
+    g = S.gradient_term(7, s)
+    # homogeneous of degree 4: E = s.g / 4
+    assert abs(0.25 * float(np.sum(g * s)) - E_ref) <= 1e-12 * abs(E_ref)
+    h = 1e-5
+    for idx in (0, 7, 20, S.nos - 1):
+        for q in range(3):
+            sp, sm = s.copy(), s.copy()
+            sp[idx, q] += h
+            sm[idx, q] -= h
+            fd = (_quad_energy(cfg, sp) - _quad_energy(cfg, sm)) / (2 * h)
+            assert abs(fd - g[idx, q]) <= 1e-7 * max(1.0, abs(g[idx, q]))
+
+
+def test_quadruplet_guard_in_gradient_and_energy(oracle):
+    """Gradient_and_Energy adds quadruplets only when idx_quadruplet > 0 (:754): alone they give no
+    gradient or energy there, while Gradient (:699-701, idx >= 0) includes them; with an exchange
+    term before them both do."""
+    base = dict(bravais_lattice="sc", n_basis_cells=[4, 4, 2], boundary_conditions=[1, 1, 1], mu_s=[1.0],
+                quadruplets=[QUADS[0]])
+    S = oracle.System(base)
+    s = oracle.random_spins(S.nos, 3)
+    g, E = S.gradient_and_energy(s)
+    assert np.abs(g).max() == 0 and E == 0
+    gq = S.gradient_term(7, s)
+    np.testing.assert_array_equal(S.gradient(s), gq)
+    S2 = oracle.System(dict(base, n_shells_exchange=1, jij=[1.5]))
+    g2, E2 = S2.gradient_and_energy(s)
+    gx = S2.gradient_term(3, s)
+    np.testing.assert_allclose(g2, gx + gq, rtol=0, atol=1e-13)
+    assert abs(E2 - (0.5 * np.sum(gx * s) + _quad_energy(base, s))) <= 1e-12 * abs(E2)
