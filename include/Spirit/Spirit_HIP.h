@@ -44,6 +44,24 @@ SPIRIT_API int Spirit_HIP_Gradient_and_Energy( State * state, double * gradient,
                                                int idx_image SPIRIT_DEFAULT( -1 ),
                                                int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * gradient and energy as Spirit_HIP_Gradient_and_Energy, of spins the caller already holds
+ * on the current device.  spins and gradient are device arrays of 3*nos doubles (AoS, as
+ * System_Get_Spin_Directions), energy a device array of one double; stream is a hipStream_t
+ * (NULL: the default stream).  The call allocates nothing: its tables, partial sums and
+ * structure-of-arrays copies live in a fixed device array of the library (about 57 MiB per
+ * GPU, held until the process ends), shared by every call on that GPU.  Limits: at most
+ * SPIRIT_HIP_NOALLOC_MAX_SPINS spins, 128 exchange + DMI pairs, 512 quadruplets and no
+ * dipolar interactions (ddi_method none); beyond them it logs an error and returns 1 without
+ * touching the outputs.  Calls through this entry point hold a per-device lock until their
+ * stream has finished, so they never overlap on one GPU.  Returns 0 on success. */
+#define SPIRIT_HIP_NOALLOC_MAX_SPINS 1048576
+SPIRIT_API int Spirit_HIP_Gradient_and_Energy_NoAlloc( State * state, const double * spins, double * gradient,
+                                                      double * energy, void * stream,
+                                                      int idx_image SPIRIT_DEFAULT( -1 ),
+                                                      int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+
 /* The dense 3N x 3N embedding Hessian of the image's Hamiltonian (row-major, index
  * 3*ispin + component), evaluated on the GPU: the matrix Hamiltonian_Heisenberg::Hessian
  * fills (Hamiltonian_Heisenberg.cpp:1119-1245; EMA, MMF and HTST read it), including its

@@ -474,6 +474,16 @@ void HamUpload::upload( const Geometry & g, const Hamiltonian & h )
 
 std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H )
 {
+    std::vector<PairDev> pv( h.exch_pairs.size() + h.dmi_pairs.size() );
+    fill_ham_dev( g, h, H, pv.data(), (int)pv.size() );
+    return pv;
+}
+
+int fill_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H, PairDev * pv, int capacity )
+{
+    const size_t n_pairs = h.exch_pairs.size() + h.dmi_pairs.size();
+    if( n_pairs > (size_t)capacity )
+        throw std::runtime_error( "pair table larger than its buffer (" + std::to_string( capacity ) + " pairs)" );
     if( g.nb > MAX_BASIS )
         throw std::runtime_error( "too many basis atoms (max " + std::to_string( MAX_BASIS ) + ")" );
     if( (int)h.aniso_idx.size() > MAX_BASIS || (int)h.cubic_idx.size() > MAX_BASIS )
@@ -509,7 +519,7 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
         H.cubic_idx[k] = h.cubic_idx[k];
         H.cubic_K[k]   = h.cubic_K[k];
     }
-    std::vector<PairDev> pv;
+    size_t np = 0;
     for( size_t k = 0; k < h.exch_pairs.size(); ++k )
     {
         const Pair & p = h.exch_pairs[k];
@@ -517,7 +527,7 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
         q.i = p.i, q.j = p.j, q.da = p.t[0], q.db = p.t[1], q.dc = p.t[2];
         q.J   = h.exch_J[k];
         q.pad = k < h.exch_shell.size() ? h.exch_shell[k] : 0;
-        pv.push_back( q );
+        pv[np++] = q;
     }
     for( size_t k = 0; k < h.dmi_pairs.size(); ++k )
     {
@@ -528,10 +538,11 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
         q.pad = k < h.dmi_shell.size() ? h.dmi_shell[k] : 0;
         for( int c = 0; c < 3; ++c )
             q.n[c] = h.dmi_n[k][c];
-        pv.push_back( q );
+        pv[np++] = q;
     }
-    for( auto & q : pv )
+    for( size_t k = 0; k < np; ++k )
     {
+        PairDev & q = pv[k];
         if( q.i < 0 || q.i >= g.nb || q.j < 0 || q.j >= g.nb )
             throw std::runtime_error( "pair references a basis atom that does not exist" );
         const long long d = ( q.j - q.i ) + (long long)g.nb * ( q.da + (long long)g.n_cells[0]
@@ -565,7 +576,7 @@ std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, H
             H.site_n[c] = h.site_n[c];
     }
     H.has_ddi = ( h.ddi_method == DDI_FFT || h.ddi_method == DDI_CUTOFF ) ? 1 : 0;
-    return pv;
+    return (int)np;
 }
 
 // ---------------------------------------------------------------------------

@@ -118,6 +118,9 @@ std::array<int, 3> ddi_padded_dims( const Geometry & g, const Hamiltonian & h );
 // per-site K_i, from h.site_K); returns the pair table (n_exch exchange pairs, then n_dmi DMI
 // pairs) that H.exch / H.dmi must point to on the device.
 std::vector<PairDev> build_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H );
+// The same, with the pair table written to pv (at most capacity entries; more throws) instead
+// of a new vector; returns the number of pairs.
+int fill_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H, PairDev * pv, int capacity );
 
 // Builds the POD image description a kernel reads (pair tables are uploaded to `pairs`).
 struct HamUpload

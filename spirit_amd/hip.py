@@ -93,6 +93,19 @@ def gradient_and_energy(p_state, nos, idx_image=-1, idx_chain=-1):
     return g, e.value
 
 
+_Gradient_and_Energy_NoAlloc = _fn("Spirit_HIP_Gradient_and_Energy_NoAlloc", [_vp, _vp, _vp, _vp, _vp, _i, _i], _i)
+NOALLOC_MAX_SPINS = 1048576  # SPIRIT_HIP_NOALLOC_MAX_SPINS
+
+
+def gradient_and_energy_noalloc(p_state, spins_ptr, gradient_ptr, energy_ptr, stream=None, idx_image=-1,
+                                idx_chain=-1):
+    """Spirit_HIP_Gradient_and_Energy_NoAlloc on caller-owned device arrays (raw addresses: spins and
+    gradient 3*nos doubles AoS, energy one double) on `stream` (a hipStream_t address or None).
+    Returns the entry point's status (0 on success)."""
+    return _Gradient_and_Energy_NoAlloc(ctypes.c_void_p(p_state), _vp(spins_ptr), _vp(gradient_ptr), _vp(energy_ptr),
+                                        _vp(stream), _i(idx_image), _i(idx_chain))
+
+
 _Get_Hessian = _fn("Spirit_HIP_Hamiltonian_Get_Hessian", [_vp, _PD, _i, _i], _i)
 _Hessian_VP = _fn("Spirit_HIP_Hamiltonian_Hessian_Vector_Product", [_vp, _PD, _PD, _i, _i], _i)
 
