@@ -64,6 +64,23 @@ def alg_bytes_per_spin_step(dims, solver, ddi):
     return local + n_fe * (per_fe + 24.0), C
 
 
+def design_bytes_per_spin_step(dims, solver, pass_c_launch_bytes):
+    """HBM bytes per spin-step this engine's dataflow has to move (not the algorithmic
+    figure, which prices the kernel spectrum as 6 dense complex components): the local part
+    of the solver, the effective field the post-iteration hook writes on every step (24 B),
+    and per force evaluation the DDI passes with pass C at its stored-spectrum bytes plus the
+    stage's read of the field."""
+    local = {2: 168.0, 3: 168.0, 1: 120.0, 4: 408.0}[solver] + 24.0
+    n_fe = 4 if solver == 4 else 2
+    Na, Nb, Nc = dims
+    Pb = 2 * Nb if Nb > 1 else 1
+    Ha = (2 * Na if Na > 1 else 1) // 2 + 1
+    N = Na * Nb * Nc
+    H_NN, H_PN = Ha * Nb * Nc, Ha * Pb * Nc
+    per_fe = (2 * (24 * N + 48 * H_NN) + 2 * (48 * H_NN + 48 * H_PN) + pass_c_launch_bytes) / N
+    return local + n_fe * (per_fe + 24.0)
+
+
 def ddi_bytes_per_fe(dims):
     """Algorithmic bytes per spin of one DDI convolution (2A + 2B + C, without the stage's read
     of the field): pack + R2C along a, C2C along b, the c pass with the kernel multiply, and
@@ -249,6 +266,15 @@ def run_adjoint(n_steps, ckpt, plain_ms_per_step):
             "loss": loss, "dL_dJ": float(g["J"][0]), "dL_dD": float(g["D"][0]), "dL_dB": float(g["B"][0])}
 
 
+# Algorithmic bytes of one GNEB VP iteration per spin (FP64, each array of the reference's
+# dataflow read once and written once per pass; neighbouring images are the same arrays):
+# gradient and distance (read s, write the effective field: 48), tangents (read s and the
+# field, write tau and the gradient force: 96), total force and VP velocity (read tau, the
+# gradient force, v and the previous force, write F_total and v: 144), VP update and hook
+# (read F_total and s, write v, s and the projected force: 120).
+GNEB_VP_BYTES_PER_SPIN_ITERATION = 408.0
+
+
 def run_gneb(n_iter, noi):
     """GNEB on the 128x128 hexagonal monolayer: noi images from a Neel skyrmion (radius 8) to
     the ferromagnet by homogeneous rotation, climbing image set automatically after a first
@@ -278,9 +304,14 @@ def run_gneb(n_iter, noi):
     E = np.array(chain.get_energy(p))
     nos = 128 * 128
     state.delete(p)
+    value = nos * noi * info.total_iterations / t
+    ach = GNEB_VP_BYTES_PER_SPIN_ITERATION * value / 1e9
     return {"workload": "hex128_gneb%d_vp_climbing" % noi, "images": noi, "spins_per_image": nos,
             "iterations": info.total_iterations, "seconds": t, "iterations_per_s": info.total_iterations / t,
-            "value": nos * noi * info.total_iterations / t, "unit": "spin-iterations/s",
+            "value": value, "unit": "spin-iterations/s",
+            "roofline": {"bound": "hbm", "alg_bytes_per_spin_iteration": GNEB_VP_BYTES_PER_SPIN_ITERATION,
+                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                         "timed": "the whole Simulation_GNEB_Start call (set-up, copies, run)"},
             "energy_first_last_max_meV": [float(E[0]), float(E[-1]), float(E.max())],
             "max_torque": float(info.max_torque)}
 
@@ -500,6 +531,9 @@ def main():
     if g["conv_n"] > 0:
         t = g["conv_ms"] / g["conv_n"] * 1e-3
         c_bytes, c_kernel = pass_c_bytes(wl["dims"], g["ddi_info"])
+        d_bytes = design_bytes_per_spin_step(wl["dims"], wl["solver"], c_bytes)
+        out["config"]["design_bytes_per_spin_step"] = d_bytes
+        out["config"]["hbm_frac_of_step_design_bytes"] = d_bytes * value / (HBM_PEAK_GBS * 1e9)
         ach = c_bytes / t / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS,

@@ -76,7 +76,7 @@ struct HamDev
     int has_site_K;
     int dmi_same;            // DMI pair k has exchange pair k's (i, j, translation) for every k < n_dmi
     int halo[3];             // largest |translation| of an exchange pair along a, b, c
-    int pad1;
+    int row_tile_log2;       // stage kernels sweep tiles of 2^row_tile_log2 rows along b through the c planes
     FastDiv div_nb, div_Na, div_Nb; // index -> (basis atom, a, b, c) without hardware division
     double B;                // external_field_magnitude * mu_B (Hamiltonian_Heisenberg.cpp:35)
     double Bn[3];            // external_field_normal

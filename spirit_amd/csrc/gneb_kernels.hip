@@ -18,6 +18,15 @@ namespace spirit_hip
 {
 
 // Sum of NV per-thread values over the block; thread 0 writes them to out[slot[k]].
+// A block's partial sums are read by the last block of the same launch, which may run on
+// another XCD (the XCDs' L2 caches are not coherent with each other): they are stored with
+// device-scope stores, which write through to memory, so the arrival in pass_tail needs no
+// device-wide cache write-back.
+__device__ __forceinline__ void partial_store( double * p, double v )
+{
+    __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
+
 template<int NV>
 __device__ __forceinline__ void block_sums( double ( &v )[NV], double * out, const int ( &slot )[NV] )
 {
@@ -39,7 +48,7 @@ __device__ __forceinline__ void block_sums( double ( &v )[NV], double * out, con
             double s = sh[k][0];
             for( int j = 1; j < GNEB_THREADS / 64; ++j )
                 s += sh[k][j];
-            out[slot[k]] = s;
+            partial_store( out + slot[k], s );
         }
 }
 
@@ -57,7 +66,7 @@ __device__ __forceinline__ void block_max_store( double v, double * out )
         double m = sh[0];
         for( int j = 1; j < GNEB_THREADS / 64; ++j )
             m = fmax( m, sh[j] );
-        *out = m;
+        partial_store( out, m );
     }
 }
 
@@ -587,24 +596,42 @@ __device__ void chain_tail( const GnebArgs & A )
 
     __shared__ double red[3][256];
     const int n_red_images = ( PHASE == GPH_TRANSLATE ) ? 1 : noi;
-    for( int img = w; img < n_red_images; img += nw )
+    // Each wave sums IB images per round, lane b adding the partials of blocks b, b+64, ...
+    // of each image in block order, then a shuffle tree: the loads of all IB images are in
+    // flight together (the partials come from memory, the whole chain's in one round).
+    constexpr int IB = 4;
+    for( int img0 = w * IB; img0 < n_red_images; img0 += nw * IB )
     {
-        double s[3] = { 0, 0, 0 };
+        double s[IB][3];
+#pragma unroll
+        for( int u = 0; u < IB; ++u )
+            s[u][0] = s[u][1] = s[u][2] = 0;
         for( int b = lane; b < A.bpi; b += 64 )
         {
-            const double * p = A.part + (long long)( img * A.bpi + b ) * GNEB_NPART;
-            for( int k = 0; k < NS; ++k )
-                s[k] = MAXRED ? fmax( s[k], p[slots[k]] ) : s[k] + p[slots[k]];
-        }
-        for( int k = 0; k < NS; ++k )
-            for( int off = 32; off > 0; off >>= 1 )
+#pragma unroll
+            for( int u = 0; u < IB; ++u )
             {
-                const double o = __shfl_down( s[k], off, 64 );
-                s[k]           = MAXRED ? fmax( s[k], o ) : s[k] + o;
+                const int img    = min( img0 + u, n_red_images - 1 );
+                const double * p = A.part + (long long)( img * A.bpi + b ) * GNEB_NPART;
+#pragma unroll
+                for( int k = 0; k < NS; ++k )
+                    s[u][k] = MAXRED ? fmax( s[u][k], p[slots[k]] ) : s[u][k] + p[slots[k]];
             }
-        if( lane == 0 )
+        }
+#pragma unroll
+        for( int u = 0; u < IB; ++u )
+        {
+#pragma unroll
             for( int k = 0; k < NS; ++k )
-                red[k][img] = s[k];
+                for( int off = 32; off > 0; off >>= 1 )
+                {
+                    const double o = __shfl_down( s[u][k], off, 64 );
+                    s[u][k]        = MAXRED ? fmax( s[u][k], o ) : s[u][k] + o;
+                }
+            if( lane == 0 && img0 + u < n_red_images )
+                for( int k = 0; k < NS; ++k )
+                    red[k][img0 + u] = s[u][k];
+        }
     }
     __syncthreads();
     chain_phase<PHASE>( A, staged ? ch_lds : A.ch, red );
@@ -614,23 +641,28 @@ __device__ void chain_tail( const GnebArgs & A )
             A.ch[e] = ch_lds[e];
 }
 
-// Arrival of a block at the end of a pass: its partials are made visible device-wide, the
-// arrival counter is advanced, and the block that arrives last (all partials written) runs
-// the chain phase, then rearms the counter for the next pass.  Every thread of the block
-// calls it, after the block's partial slots were stored by thread 0.
+// Arrival of a block at the end of a pass: thread 0 waits until its device-scope partial
+// stores are performed, then advances the arrival counter; the block that arrives last (every
+// partial in memory) drops its XCD's stale cache lines, runs the chain phase and rearms the
+// counter for the next pass.  A device-wide release fence in every block instead would write
+// back each XCD's whole L2 once per block.  Every thread of the block calls it, after the
+// block's partial slots were stored by thread 0 (partial_store).
 template<int PHASE>
 __device__ __forceinline__ void pass_tail( const GnebArgs & A )
 {
     __shared__ int last;
     if( threadIdx.x == 0 )
     {
-        __threadfence();
-        last = atomicAdd( A.count, 1 ) == (int)gridDim.x - 1 ? 1 : 0;
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup" );
+        __builtin_amdgcn_s_waitcnt( 0 ); // vmcnt = expcnt = lgkmcnt = 0: the partial stores are complete
+        last = __hip_atomic_fetch_add( A.count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == (int)gridDim.x - 1
+                   ? 1
+                   : 0;
     }
     __syncthreads();
     if( !last )
         return;
-    __threadfence();
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
     chain_tail<PHASE>( A );
     if( threadIdx.x == 0 )
         atomicExch( A.count, 0 );

@@ -557,6 +557,18 @@ int fill_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H, PairDev
         H.halo[1] = std::max( H.halo[1], std::abs( pv[k].db ) );
         H.halo[2] = std::max( H.halo[2], std::abs( pv[k].dc ) );
     }
+    // stage-kernel row tile: 16 rows by default (256x256x128 Depondt+DDI: 0.308 / 0.300 /
+    // 0.287 / 0.286 ms per stage at 4 / 8 / 16 / 32 rows); SPIRIT_HIP_STAGE_ROW_TILE (a power
+    // of two, 0 for lattice order) overrides it for launch-shape sweeps
+    H.row_tile_log2 = 4;
+    if( const char * rt = std::getenv( "SPIRIT_HIP_STAGE_ROW_TILE" ) )
+    {
+        const int v = std::atoi( rt );
+        int l       = 0;
+        while( ( 1 << l ) < v && l < 12 )
+            ++l;
+        H.row_tile_log2 = ( v > 0 && ( 1 << l ) == v ) ? l : 0;
+    }
     H.dmi_same = H.n_dmi <= H.n_exch ? 1 : 0;
     for( int k = 0; k < H.n_dmi && H.dmi_same; ++k )
     {

@@ -569,25 +569,26 @@ __device__ __forceinline__ int xcd_contiguous_block()
 }
 
 // Order in which the blocks of one image visit the lattice in the stage kernels: rows of
-// cells along a (all basis atoms, whole blocks per row) in tiles of STAGE_ROW_TILE
+// cells along a (all basis atoms, whole blocks per row) in tiles of 2^row_tile_log2
 // consecutive b, each tile swept through every c plane, so the rows b +- 1 and c +- 1 a
 // block reads were fetched into the same L2 by blocks that ran just before it instead of
 // a whole 2D plane earlier.  A bijection on the image's blocks; the identity where the
 // rows do not hold whole blocks, Nb is not a multiple of the tile or the lattice is 2D.
-constexpr int STAGE_ROW_TILE = 8;
 __device__ __forceinline__ int stage_block_order( const HamDev & H, int q )
 {
-    const int R = H.nb * H.Na;
-    if( R % LLG_THREADS != 0 || H.Nb % STAGE_ROW_TILE != 0 || H.Nc == 1 )
+    const int R    = H.nb * H.Na;
+    const int TL   = H.row_tile_log2;
+    const int TILE = 1 << TL;
+    if( TL <= 0 || R % LLG_THREADS != 0 || ( H.Nb & ( TILE - 1 ) ) != 0 || H.Nc == 1 )
         return q;
     const int per_row = R / LLG_THREADS;
     const int j       = q % per_row;
     int t             = q / per_row;
-    const int bi      = t % STAGE_ROW_TILE;
-    t /= STAGE_ROW_TILE;
+    const int bi      = t & ( TILE - 1 );
+    t >>= TL;
     const int c  = t % H.Nc;
     const int bt = t / H.Nc;
-    return ( bt * STAGE_ROW_TILE + bi + H.Nb * c ) * per_row + j;
+    return ( ( bt << TL ) + bi + H.Nb * c ) * per_row + j;
 }
 
 // block-wide reduction of (sum e, max t) into partials[blk]
