@@ -444,7 +444,7 @@ __device__ __forceinline__ void tl_force( const TlArgs & T, int i, const double 
     }
     double th[3];
     thermal_xi( T.L, mu, T.step, i, th );
-    virtual_force( T.L, SOLVER_DEPONDT, mu, xi_, F, th, fv );
+    virtual_force( T.L, SOLVER_DEPONDT, 1.0 / mu, xi_, F, th, fv );
     const double k = vf_scale( T.L, mu ), a = T.L.damping;
     double c1[3], c2[3], c3[3];
     cross3( dxi, F, c1 );

@@ -81,6 +81,7 @@ struct HamDev
     double B;                // external_field_magnitude * mu_B (Hamiltonian_Heisenberg.cpp:35)
     double Bn[3];            // external_field_normal
     double mu[MAX_BASIS];    // mu_s per basis atom
+    double inv_mu[MAX_BASIS]; // 1 / mu_s
     int aniso_idx[MAX_BASIS];
     double aniso_K[MAX_BASIS];
     double aniso_n[MAX_BASIS][3];
@@ -105,6 +106,7 @@ struct LLGDev
     int thermal; // temperature > 0
     uint64_t seed;
     double force_convergence; // the image stops once its max torque is below this
+    double vf_dtg;            // dt gamma / mu_B / (1 + damping^2), the dynamics' virtual-force scale
 };
 
 // One image of a batched launch: where its fields live and how it is parameterised.

@@ -14,6 +14,8 @@ namespace spirit_hip
 
 constexpr int GNEB_THREADS = 256;
 constexpr int GNEB_NPART   = 8;  // partial-sum slots per block
+constexpr int GNEB_ISUM    = 4;  // per-image sum slots
+constexpr int GNEB_COUNT_STRIDE = 32; // arrival counters 128 bytes apart
 constexpr int GNEB_NCH     = 32; // scalar slots per image
 constexpr int GNEB_CH_LDS  = 64; // chains up to this many images keep their scalars in LDS in the chain phases
 
@@ -95,7 +97,9 @@ struct GnebArgs
     const int * types;           // image types (0 normal, 1 climbing, 2 falling, 3 stationary)
     int * stop;                  // 0 running, 1 converged, 2 two images coincide
     long long * iter;            // iterations done
-    int * count;                 // blocks of the running pass that have stored their partials (0 between passes)
+    int * count;                 // arrival counters of the running pass (all 0 between passes): [0] images whose
+                                 // sums are stored, [GNEB_COUNT_STRIDE*(1+img)] blocks of image img done
+    double * isum;               // [noi][GNEB_ISUM] per-image sums of the running pass's partials
     GnebParamsDev P;
 };
 

@@ -50,8 +50,10 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
     b_types.resize( sizeof( int ) * noi );
     b_stop.resize( sizeof( int ) );
     b_iter.resize( sizeof( long long ) );
-    b_count.resize( sizeof( int ) );
-    HIPCHECK( hipMemsetAsync( b_count.as<void>(), 0, sizeof( int ), st ) );
+    b_count.resize( sizeof( int ) * GNEB_COUNT_STRIDE * ( 1 + (size_t)noi ) );
+    HIPCHECK( hipMemsetAsync( b_count.as<void>(), 0, b_count.size(), st ) );
+    b_isum.resize( sizeof( double ) * GNEB_ISUM * (size_t)noi );
+    HIPCHECK( hipMemsetAsync( b_isum.as<void>(), 0, b_isum.size(), st ) );
     HIPCHECK( hipMemsetAsync( b_stop.as<void>(), 0, sizeof( int ), st ) );
     HIPCHECK( hipMemsetAsync( b_iter.as<void>(), 0, sizeof( long long ), st ) );
 
@@ -75,6 +77,7 @@ GNEBRunner::GNEBRunner( const std::vector<ImageSpec> & images, const GNEBParamet
     A.stop     = b_stop.as<int>();
     A.iter     = b_iter.as<long long>();
     A.count    = b_count.as<int>();
+    A.isum     = b_isum.as<double>();
     set_types( types );
     set_parameters( par );
     if( min_is_solver( solver ) && solver != SOLVER_VP )

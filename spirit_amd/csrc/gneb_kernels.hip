@@ -560,14 +560,68 @@ __device__ void hermite_point( const double * ch, int noi, int n_interp, int k, 
 template<int PHASE>
 __device__ __forceinline__ void chain_phase( const GnebArgs & A, double * ch, const double ( *red )[256] );
 
-// The chain step that follows a per-spin pass, run by the last block of that pass to finish
-// (every thread of the block calls it): per-image sums of the pass's partial slots (one wave
-// per image at a time, fixed order), then the chain logic of the phase.  The chain scalars
-// are staged in LDS (chains of up to GNEB_CH_LDS images): the sequential parts of the logic
-// then pay LDS latency instead of L2 latency per dependent read.
+// Partial-sum slots a chain phase consumes and how they combine (sums, or maxima for the hook).
 template<int PHASE>
-__device__ void chain_tail( const GnebArgs & A )
+struct PhaseSlots
 {
+    static constexpr bool MAXRED = PHASE == GPH_HOOK;
+    static constexpr int NS      = ( PHASE == GPH_DIST || PHASE == GPH_TRANSLATE || PHASE == GPH_VP ) ? 2
+                                   : ( PHASE == GPH_TANGENT || PHASE == GPH_SHRINK1 )            ? 3
+                                                                                                 : 1;
+    __device__ static int slot( int k )
+    {
+        if( PHASE == GPH_DIST )
+            return k == 0 ? GP_E : GP_DIST;
+        if( PHASE == GPH_TANGENT )
+            return k == 0 ? GP_TT : k == 1 ? GP_FT : GP_ET;
+        if( PHASE == GPH_TRANSLATE || PHASE == GPH_VP )
+            return k == 0 ? GP_A : GP_B;
+        if( PHASE == GPH_SHRINK1 )
+            return k == 0 ? GP_A : k == 1 ? GP_B : GP_C;
+        if( PHASE == GPH_HOOK )
+            return GP_C;
+        return GP_A;
+    }
+};
+
+// The sums of one image's block partials (wave 0 of the image's last block): lane b adds the
+// partials of blocks b, b+64, ... in block order, then a shuffle tree; stored to isum[img]
+// with device-scope stores for the chain's last image.
+template<int PHASE>
+__device__ __forceinline__ void image_sums( const GnebArgs & A, int img )
+{
+    using PS       = PhaseSlots<PHASE>;
+    const int lane = threadIdx.x;
+    if( threadIdx.x >= 64 )
+        return;
+    double s[3] = { 0, 0, 0 };
+    for( int b = lane; b < A.bpi; b += 64 )
+    {
+        const double * p = A.part + (long long)( img * A.bpi + b ) * GNEB_NPART;
+#pragma unroll
+        for( int k = 0; k < PS::NS; ++k )
+            s[k] = PS::MAXRED ? fmax( s[k], p[PS::slot( k )] ) : s[k] + p[PS::slot( k )];
+    }
+#pragma unroll
+    for( int k = 0; k < PS::NS; ++k )
+        for( int off = 32; off > 0; off >>= 1 )
+        {
+            const double o = __shfl_down( s[k], off, 64 );
+            s[k]           = PS::MAXRED ? fmax( s[k], o ) : s[k] + o;
+        }
+    if( lane == 0 )
+        for( int k = 0; k < PS::NS; ++k )
+            partial_store( A.isum + img * GNEB_ISUM + k, s[k] );
+}
+
+// The chain step that follows a per-spin pass, run by the block that completes the last
+// image (every thread of the block calls it): the images' sums, then the chain logic of the
+// phase.  The chain scalars are staged in LDS (chains of up to GNEB_CH_LDS images): the
+// sequential parts of the logic then pay LDS latency instead of L2 latency per dependent read.
+template<int PHASE>
+__device__ void chain_tail( const GnebArgs & A, int n_red_images )
+{
+    using PS = PhaseSlots<PHASE>;
     extern __shared__ double ch_lds[];
     const int noi       = A.noi;
     const bool staged   = noi <= GNEB_CH_LDS;
@@ -575,64 +629,11 @@ __device__ void chain_tail( const GnebArgs & A )
     if( staged )
         for( int e = threadIdx.x; e < n_scalars; e += blockDim.x )
             ch_lds[e] = A.ch[e];
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
-    constexpr bool MAXRED = PHASE == GPH_HOOK;
-    constexpr int NS      = ( PHASE == GPH_DIST || PHASE == GPH_TRANSLATE || PHASE == GPH_VP ) ? 2
-                            : ( PHASE == GPH_TANGENT || PHASE == GPH_SHRINK1 )            ? 3
-                                                                                          : 1;
-    int slots[3] = { 0, 0, 0 };
-    if( PHASE == GPH_DIST )
-        slots[0] = GP_E, slots[1] = GP_DIST;
-    else if( PHASE == GPH_TANGENT )
-        slots[0] = GP_TT, slots[1] = GP_FT, slots[2] = GP_ET;
-    else if( PHASE == GPH_TRANSLATE || PHASE == GPH_VP )
-        slots[0] = GP_A, slots[1] = GP_B;
-    else if( PHASE == GPH_SHRINK1 )
-        slots[0] = GP_A, slots[1] = GP_B, slots[2] = GP_C;
-    else if( PHASE == GPH_SHRINK2 || PHASE == GPH_SHRINK3 || PHASE == GPH_SHRINK4 )
-        slots[0] = GP_A;
-    else if( PHASE == GPH_HOOK )
-        slots[0] = GP_C;
-
     __shared__ double red[3][256];
-    const int n_red_images = ( PHASE == GPH_TRANSLATE ) ? 1 : noi;
-    // Each wave sums IB images per round, lane b adding the partials of blocks b, b+64, ...
-    // of each image in block order, then a shuffle tree: the loads of all IB images are in
-    // flight together (the partials come from memory, the whole chain's in one round).
-    constexpr int IB = 4;
-    for( int img0 = w * IB; img0 < n_red_images; img0 += nw * IB )
-    {
-        double s[IB][3];
+    for( int img = threadIdx.x; img < n_red_images; img += blockDim.x )
 #pragma unroll
-        for( int u = 0; u < IB; ++u )
-            s[u][0] = s[u][1] = s[u][2] = 0;
-        for( int b = lane; b < A.bpi; b += 64 )
-        {
-#pragma unroll
-            for( int u = 0; u < IB; ++u )
-            {
-                const int img    = min( img0 + u, n_red_images - 1 );
-                const double * p = A.part + (long long)( img * A.bpi + b ) * GNEB_NPART;
-#pragma unroll
-                for( int k = 0; k < NS; ++k )
-                    s[u][k] = MAXRED ? fmax( s[u][k], p[slots[k]] ) : s[u][k] + p[slots[k]];
-            }
-        }
-#pragma unroll
-        for( int u = 0; u < IB; ++u )
-        {
-#pragma unroll
-            for( int k = 0; k < NS; ++k )
-                for( int off = 32; off > 0; off >>= 1 )
-                {
-                    const double o = __shfl_down( s[u][k], off, 64 );
-                    s[u][k]        = MAXRED ? fmax( s[u][k], o ) : s[u][k] + o;
-                }
-            if( lane == 0 && img0 + u < n_red_images )
-                for( int k = 0; k < NS; ++k )
-                    red[k][img0 + u] = s[u][k];
-        }
-    }
+        for( int k = 0; k < PS::NS; ++k )
+            red[k][img] = A.isum[img * GNEB_ISUM + k];
     __syncthreads();
     chain_phase<PHASE>( A, staged ? ch_lds : A.ch, red );
     __syncthreads();
@@ -641,31 +642,48 @@ __device__ void chain_tail( const GnebArgs & A )
             A.ch[e] = ch_lds[e];
 }
 
-// Arrival of a block at the end of a pass: thread 0 waits until its device-scope partial
-// stores are performed, then advances the arrival counter; the block that arrives last (every
-// partial in memory) drops its XCD's stale cache lines, runs the chain phase and rearms the
-// counter for the next pass.  A device-wide release fence in every block instead would write
-// back each XCD's whole L2 once per block.  Every thread of the block calls it, after the
-// block's partial slots were stored by thread 0 (partial_store).
+// Arrival of a block at the end of a pass, in two levels so that no counter takes more than
+// one image's worth of atomics: thread 0 waits until its device-scope partial stores are
+// performed and counts the block in its image; the image's last block drops its XCD's stale
+// cache lines, sums the image's partials and counts the image in the chain; the chain's last
+// image runs the chain phase.  Counters are rearmed by the blocks that completed them.  A
+// device-wide release fence in every block instead would write back each XCD's whole L2 once
+// per block.  Every thread of the block calls it, after the block's partial slots were stored
+// by thread 0 (partial_store).
 template<int PHASE>
 __device__ __forceinline__ void pass_tail( const GnebArgs & A )
 {
     __shared__ int last;
+    const int img      = blockIdx.x / A.bpi;
+    const int n_images = gridDim.x / A.bpi; // the translation pass runs over the blocks of image 0 only
+    int * img_count    = A.count + GNEB_COUNT_STRIDE * ( 1 + img );
     if( threadIdx.x == 0 )
     {
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup" );
         __builtin_amdgcn_s_waitcnt( 0 ); // vmcnt = expcnt = lgkmcnt = 0: the partial stores are complete
-        last = __hip_atomic_fetch_add( A.count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == (int)gridDim.x - 1
-                   ? 1
-                   : 0;
+        last = __hip_atomic_fetch_add( img_count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == A.bpi - 1 ? 1 : 0;
     }
     __syncthreads();
     if( !last )
         return;
     __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
-    chain_tail<PHASE>( A );
+    image_sums<PHASE>( A, img );
+    __syncthreads();
     if( threadIdx.x == 0 )
-        atomicExch( A.count, 0 );
+    {
+        __hip_atomic_store( img_count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "workgroup" );
+        __builtin_amdgcn_s_waitcnt( 0 ); // the image sums (stored by lane 0 of wave 0, this thread) are complete
+        last = __hip_atomic_fetch_add( A.count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT ) == n_images - 1 ? 1
+                                                                                                                 : 0;
+    }
+    __syncthreads();
+    if( !last )
+        return;
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
+    chain_tail<PHASE>( A, n_images );
+    if( threadIdx.x == 0 )
+        __hip_atomic_store( A.count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
 }
 
 // The chain logic of one phase, run by the whole block: per-image work on one thread per

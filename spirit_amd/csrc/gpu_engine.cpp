@@ -504,7 +504,10 @@ int fill_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H, PairDev
     for( int c = 0; c < 3; ++c )
         H.Bn[c] = h.field_normal[c];
     for( int b = 0; b < g.nb; ++b )
-        H.mu[b] = g.mu_basis[b];
+    {
+        H.mu[b]     = g.mu_basis[b];
+        H.inv_mu[b] = 1.0 / g.mu_basis[b];
+    }
     H.n_aniso = (int)h.aniso_idx.size();
     for( int k = 0; k < H.n_aniso; ++k )
     {
@@ -623,6 +626,7 @@ static LLGDev make_llg_dev( const LLGParameters & p )
     L.thermal             = p.temperature > 0 ? 1 : 0;
     L.seed                = p.seed;
     L.force_convergence   = p.force_convergence;
+    L.vf_dtg              = p.dt * gamma_e / mu_B / ( 1 + p.damping * p.damping );
     return L;
 }
 

@@ -409,7 +409,7 @@ private:
     int solver = 0, noi = 0, nos = 0, bpi = 0;
     GnebArgs A{};
     DeviceBuffer b_eff, b_Fg, b_tau, b_Ft, b_ftr, b_fv, b_fv0, b_v, b_fprev, b_pred, b_tmp, b_part, b_ch, b_types,
-        b_stop, b_iter, b_count;
+        b_stop, b_iter, b_count, b_isum;
     // VP_OSO, LBFGS_OSO, LBFGS_Atlas on the chain's forces F_total
     void setup_minimiser();
     DeviceBuffer b_mv, b_mprev, b_mgrad, b_mgpr, b_msd, b_mq, b_ma3, b_mpart, b_msc;

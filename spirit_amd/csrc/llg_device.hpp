@@ -41,6 +41,59 @@ __device__ __forceinline__ void normalize3( double * a )
         a[2] /= n;
     }
 }
+// a / |a| with one division (a product by the reciprocal norm); zero vectors stay zero
+__device__ __forceinline__ void normalize3_rcp( double * a )
+{
+    double z = dot3( a, a );
+    if( z > 0 )
+    {
+        const double r = 1.0 / sqrt( z );
+        a[0] *= r;
+        a[1] *= r;
+        a[2] *= r;
+    }
+}
+// sin and cos of an angle in [0, 0.25] from their Taylor series (through x^13 and x^14:
+// the first omitted terms are below 1e-20 relative), else from sincos.
+__device__ __forceinline__ void sincos_small( double t, double * sn, double * cs )
+{
+    if( t <= 0.25 )
+    {
+        const double x = t * t;
+        double ps      = 1.0 / 6227020800.0;
+        ps             = ps * x - 1.0 / 39916800.0;
+        ps             = ps * x + 1.0 / 362880.0;
+        ps             = ps * x - 1.0 / 5040.0;
+        ps             = ps * x + 1.0 / 120.0;
+        ps             = ps * x - 1.0 / 6.0;
+        *sn            = t + t * ( ps * x );
+        double pc      = -1.0 / 87178291200.0;
+        pc             = pc * x + 1.0 / 479001600.0;
+        pc             = pc * x - 1.0 / 3628800.0;
+        pc             = pc * x + 1.0 / 40320.0;
+        pc             = pc * x - 1.0 / 720.0;
+        pc             = pc * x + 1.0 / 24.0;
+        pc             = pc * x - 0.5;
+        *cs            = 1.0 + pc * x;
+    }
+    else
+        sincos( t, sn, cs );
+}
+// Rodrigues rotation (as rotate3) for the solver stages: the angle of one time step is
+// small, so its sine and cosine come from sincos_small
+__device__ __forceinline__ void rotate3_step( const double * v, const double * axis, double angle, double * out )
+{
+    double sn, c;
+    sincos_small( angle, &sn, &c );
+    double cr[3];
+    cross3( axis, v, cr );
+    double d = dot3( axis, v );
+    double o[3];
+#pragma unroll
+    for( int q = 0; q < 3; ++q )
+        o[q] = v[q] * c + cr[q] * sn + axis[q] * d * ( 1 - c );
+    out[0] = o[0], out[1] = o[1], out[2] = o[2];
+}
 // Rodrigues rotation, Vectormath.cpp:474-479
 __device__ __forceinline__ void rotate3( const double * v, const double * axis, double angle, double * out )
 {
@@ -494,7 +547,8 @@ __device__ __forceinline__ void local_energy_terms( const HamDev & H, const doub
 }
 
 // Method_LLG::Calculate_Force_Virtual (no STT), Method_LLG.cpp:132-226
-__device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, double mu, const double * s,
+// inv_mu = 1 / mu_s of the spin (the dynamics' division by mu_s, as a product)
+__device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, double inv_mu, const double * s,
                                                const double * F, const double * xi, double * fv )
 {
     const double a = P.damping;
@@ -510,7 +564,7 @@ __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, dou
             fv[q] = dtg * c[q];
         return;
     }
-    const double dtg = P.dt * gamma_e / mu_B / ( 1 + a * a );
+    const double dtg = P.vf_dtg; // dt gamma / mu_B / (1 + a^2), Method_LLG.cpp:149
     double c[3], v[3];
 #pragma unroll
     for( int q = 0; q < 3; ++q )
@@ -521,7 +575,7 @@ __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, dou
         v[q] += dtg * a * c[q];
 #pragma unroll
     for( int q = 0; q < 3; ++q )
-        v[q] /= mu;
+        v[q] *= inv_mu;
     if( P.thermal )
     {
 #pragma unroll

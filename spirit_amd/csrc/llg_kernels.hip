@@ -71,7 +71,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
             thermal_field( I->llg, mu, A.step_dev ? *A.step_dev + A.step : A.step, i, A.noise ? A.noise + base : nullptr,
                            xi );
         double fv[3];
-        virtual_force( I->llg, SOLVER, mu, si, F, xi, fv );
+        virtual_force( I->llg, SOLVER, H.inv_mu[basis_of( H, i )], si, F, xi, fv );
         if( A.tape_F )
         {
             store3( A.tape_F + base, nos, i, F );
@@ -96,9 +96,9 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                 store3( A.aux + base, nos, i, fv );
                 double axis[3] = { 1 * fv[0], 1 * fv[1], 1 * fv[2] };
                 double angle   = sqrt( dot3( fv, fv ) );
-                normalize3( axis );
+                normalize3_rcp( axis );
                 double pr[3];
-                rotate3( si, axis, angle, pr );
+                rotate3_step( si, axis, angle, pr );
                 store3( A.sp + base, nos, i, pr );
             }
             else
@@ -111,8 +111,8 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                 for( int q = 0; q < 3; ++q )
                     t[q] += 0.5 * fv[q];
                 double angle = sqrt( dot3( t, t ) );
-                normalize3( t );
-                rotate3( s0, t, angle, snew );
+                normalize3_rcp( t );
+                rotate3_step( s0, t, angle, snew );
                 store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
             }
         }
@@ -129,7 +129,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     pr[q] += 1 * t[q];
-                normalize3( pr );
+                normalize3_rcp( pr );
                 store3( A.sp + base, nos, i, pr );
             }
             else
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     t[q] += 0.5 * t1[q];
-                normalize3( t );
+                normalize3_rcp( t );
                 snew[0] = t[0], snew[1] = t[1], snew[2] = t[2];
                 store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
             }
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     pr[q] += 0.5 * k[q];
-                normalize3( pr );
+                normalize3_rcp( pr );
                 store3( A.sp + base, nos, i, pr );
             }
             else if( STAGE == 1 || STAGE == 2 )
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     pr[q] += cp * k[q];
-                normalize3( pr );
+                normalize3_rcp( pr );
                 store3( ( STAGE == 1 ? A.sp2 : A.sp ) + base, nos, i, pr );
             }
             else
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     acc[q] += 1.0 / 6.0 * k[q];
-                normalize3( acc );
+                normalize3_rcp( acc );
                 snew[0] = acc[0], snew[1] = acc[1], snew[2] = acc[2];
                 store3( ( A.s_out ? A.s_out : A.s ) + base, nos, i, snew );
                 if( A.obs )
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solv
         LLGDev P           = I->llg;
         P.thermal          = 0;
         double fv[3];
-        virtual_force( P, solver, H.mu[basis_of( H, i )], si, F, xi, fv );
+        virtual_force( P, solver, H.inv_mu[basis_of( H, i )], si, F, xi, fv );
         double d = dot3( fv, si ), pr[3];
 #pragma unroll
         for( int q = 0; q < 3; ++q )
@@ -380,10 +380,27 @@ __global__ void __launch_bounds__( FIN_THREADS )
         return;
     const ImageDev & I = imgs[img];
     double e = 0, t = 0;
-    for( int k = threadIdx.x; k < I.n_blocks; k += blockDim.x )
+    // thread k adds the partials k, k + blockDim.x, ... in that order; eight of them are
+    // loaded before the first is added, so their latencies overlap
+    constexpr int U = 8;
+    const int nb    = I.n_blocks;
+    for( int k0 = threadIdx.x; k0 < nb; k0 += U * blockDim.x )
     {
-        e += part_E[I.block_begin + k];
-        t = fmax( t, part_T[I.block_begin + k] );
+        double pe[U], pt[U];
+#pragma unroll
+        for( int u = 0; u < U; ++u )
+        {
+            const int k = k0 + u * blockDim.x;
+            pe[u]       = k < nb ? part_E[I.block_begin + k] : 0.0;
+            pt[u]       = k < nb ? part_T[I.block_begin + k] : 0.0;
+        }
+#pragma unroll
+        for( int u = 0; u < U; ++u )
+            if( k0 + u * blockDim.x < nb )
+            {
+                e += pe[u];
+                t = fmax( t, pt[u] );
+            }
     }
     sE[threadIdx.x] = e;
     sT[threadIdx.x] = t;
