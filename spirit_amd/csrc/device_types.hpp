@@ -14,6 +14,7 @@ namespace spirit_hip
 constexpr int MAX_BASIS      = 8;
 constexpr int MAX_PAIRS      = 64;
 constexpr int MAX_FFT_STAGES = 16;
+constexpr int GATHER_PAIRS   = 6; // pair tables up to this size are gathered into registers per spin
 
 // Physical constants, core/include/utility/Constants.hpp:18-40
 constexpr double mu_B  = 0.057883817555;
@@ -94,6 +95,10 @@ struct HamDev
     int n_quad;              // quadruplets
     int quad_in_step;        // Gradient_and_Energy adds them (Hamiltonian_Heisenberg.cpp:754)
     const QuadDev * quad;
+    // the first GATHER_PAIRS exchange and DMI pairs, held in the record itself: the gathered
+    // field evaluation reads them with the record (scalar loads), not through a pointer
+    PairDev exch_g[GATHER_PAIRS];
+    PairDev dmi_g[GATHER_PAIRS];
 };
 
 // LLG parameters of one image (Method_LLG.cpp:132-226, 66-110).

@@ -572,6 +572,10 @@ int fill_ham_dev( const Geometry & g, const Hamiltonian & h, HamDev & H, PairDev
             ++l;
         H.row_tile_log2 = ( v > 0 && ( 1 << l ) == v ) ? l : 0;
     }
+    for( int k = 0; k < GATHER_PAIRS && k < H.n_exch; ++k )
+        H.exch_g[k] = pv[k];
+    for( int k = 0; k < GATHER_PAIRS && k < (int)h.dmi_pairs.size(); ++k )
+        H.dmi_g[k] = pv[H.n_exch + k];
     H.dmi_same = H.n_dmi <= H.n_exch ? 1 : 0;
     for( int k = 0; k < H.n_dmi && H.dmi_same; ++k )
     {
@@ -689,6 +693,7 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
     }
     b_pE.resize( sizeof( double ) * n_blocks );
     b_pT.resize( sizeof( double ) * n_blocks );
+    alloc_thermal_field();
     b_stop.resize( sizeof( int ) );
     b_stopstep.resize( sizeof( long long ) * specs.size() );
     b_img_stop.resize( sizeof( int ) * specs.size() );
@@ -846,6 +851,20 @@ void LLGRunner::set_llg( int img, const LLGParameters & p )
     specs[img].llg      = p;
     imgs_host[img].llg  = make_llg_dev( p );
     upload_images();
+    alloc_thermal_field();
+}
+
+// the step's stored thermal field, once any image runs at T > 0
+void LLGRunner::alloc_thermal_field()
+{
+    bool thermal = false;
+    for( auto & I : imgs_host )
+        thermal = thermal || I.llg.thermal;
+    if( thermal && b_xi.size() == 0 )
+    {
+        release_step_graph();
+        b_xi.resize( sizeof( double ) * 3 * (size_t)n_total );
+    }
 }
 
 void LLGRunner::refresh_hamiltonian()
@@ -946,7 +965,7 @@ void LLGRunner::ddi_field( const double * x, bool prof_events )
 static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuffer & s, DeviceBuffer & sp,
                               DeviceBuffer & sp2, DeviceBuffer & aux, DeviceBuffer & aux2, DeviceBuffer & h,
                               DeviceBuffer & eff, DeviceBuffer & efft, DeviceBuffer & pE, DeviceBuffer & pT,
-                              DeviceBuffer & stop, DeviceBuffer & img_stop )
+                              DeviceBuffer & stop, DeviceBuffer & img_stop, DeviceBuffer & xi )
 {
     StepArgs A{};
     A.imgs    = imgs.as<ImageDev>();
@@ -963,11 +982,13 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
     A.part_T  = pT.as<double>();
     A.stop     = stop.as<int>();
     A.img_stop = img_stop.as<int>();
+    A.xi       = xi.as<double>();
     return A;
 }
 
 #define STEP_ARGS()                                                                                                    \
-    base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop )
+    base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop, \
+                  b_xi )
 
 void LLGRunner::launch_step( const StepArgs & A )
 {

@@ -287,7 +287,8 @@ private:
     std::vector<std::unique_ptr<HamUpload>> hams;
     std::vector<std::unique_ptr<DdiPlan>> ddi;
     DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
-        b_noise, b_stopstep, b_img_stop, b_nstop, b_step;
+        b_noise, b_stopstep, b_img_stop, b_nstop, b_step, b_xi;
+    void alloc_thermal_field();
     std::vector<double> obs_host;
     std::vector<int> img_stopped; // host view of the per-image stop flags
     std::vector<long> steps_last; // steps each image advanced in the last run

@@ -23,6 +23,8 @@ struct StepArgs
     double * eff;          // observed steps: the first evaluation's force F1 (LLG stages) or the
                            // projected effective field (direct-minimisation kernels)
     const double * noise;  // injected N(0,1) thermal noise of this step, AoS per spin (nullable)
+    double * xi;           // the step's thermal field: drawn and stored by the first stage, read by
+                           // the later ones (nullable: every stage draws it)
     double * part_E;       // per-block energy partial sums
     double * part_T;       // per-block max squared torque
     int * stop;            // set once every image has stopped; every stage returns early

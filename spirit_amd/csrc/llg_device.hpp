@@ -232,7 +232,7 @@ __device__ __forceinline__ int pair_neighbour( const HamDev & H, int a, int b, i
 }
 
 // neighbours a spin gathers into registers before using them (larger tables loop)
-constexpr int LG_MAX_NB = 6;
+constexpr int LG_MAX_NB = GATHER_PAIRS;
 
 // The exchange neighbours of spin i (tables of up to LG_MAX_NB pairs), all loads issued
 // before any is used.  A pair that does not start at this basis atom or leaves an open
@@ -248,7 +248,7 @@ __device__ __forceinline__ void gather_exchange_neighbours( const HamDev & H, co
     for( int k = 0; k < LG_MAX_NB; ++k )
         if( k < H.n_exch )
         {
-            const PairDev & p = H.exch[k];
+            const PairDev & p = H.exch_g[k];
             const int j = p.i != ib ? -1 : interior ? i + p.delta : pair_neighbour( H, a, b, c, p );
             load3( x, nos, j >= 0 ? j : i, se[k] );
             if( j < 0 )
@@ -381,7 +381,7 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
         for( int k = 0; k < LG_MAX_NB; ++k )
             if( k < H.n_exch )
             {
-                const double J = H.exch[k].J;
+                const double J = H.exch_g[k].J;
 #pragma unroll
                 for( int q = 0; q < 3; ++q )
                     g[q] -= J * se[k][q];
@@ -408,7 +408,7 @@ __device__ __forceinline__ void local_gradient( const HamDev & H, const double *
         for( int k = 0; k < LG_MAX_NB; ++k )
             if( k < H.n_dmi )
             {
-                const PairDev & p = H.dmi[k];
+                const PairDev & p = H.dmi_g[k];
                 double cr[3];
                 cross3( se[k], p.n, cr );
 #pragma unroll

@@ -68,8 +68,18 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
         const double F[3] = { -1 * g[0], -1 * g[1], -1 * g[2] };
         double xi[3]      = { 0, 0, 0 };
         if( I->llg.thermal && !I->llg.direct_minimization )
-            thermal_field( I->llg, mu, A.step_dev ? *A.step_dev + A.step : A.step, i, A.noise ? A.noise + base : nullptr,
-                           xi );
+        {
+            // drawn once per step and shared by the stages (Solver_Depondt.hpp:32)
+            if( STAGE > 0 && A.xi )
+                load3( A.xi + base, nos, i, xi );
+            else
+            {
+                thermal_field( I->llg, mu, A.step_dev ? *A.step_dev + A.step : A.step, i,
+                               A.noise ? A.noise + base : nullptr, xi );
+                if( STAGE == 0 && A.xi )
+                    store3( A.xi + base, nos, i, xi );
+            }
+        }
         double fv[3];
         virtual_force( I->llg, SOLVER, H.inv_mu[basis_of( H, i )], si, F, xi, fv );
         if( A.tape_F )

@@ -262,7 +262,7 @@ def test_llg_quadruplets_trajectory(engine, oracle):
 
 def test_llg_ddi_trajectory_row_tiles(engine, oracle):
     """A 256x16x4 film with per-site K_i: rows of 256 cells fill whole blocks, so the stage
-    kernels visit the lattice in 8-row tiles swept through c, and most sites take their
+    kernels visit the lattice in 16-row tiles swept through c, and most sites take their
     neighbours as fixed index offsets; Depondt + DDI, 30 steps."""
     cfg = configs.scaled(configs.CUBE_256, [256, 16, 4])
     print_case("llg ddi", "256x16x4 row tiles")
