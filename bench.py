@@ -143,6 +143,34 @@ def run_gpu(args, wl):
                 norm_err=norm_err, setup_s=setup_s, ddi_info=info)
 
 
+def copy_rates(sizes_mib=(256, 1024, 4096), reps=20):
+    """Device bandwidth of plain streaming kernels (torch copy, sum, fill; FP64) with HIP
+    events: the practical ceiling the DDI and stage kernels are compared with."""
+    import torch
+    dev = torch.device("cuda")
+    out = {}
+    for mib in sizes_mib:
+        n = mib * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        res = {}
+        for name, fn, moved in (("copy", lambda: b.copy_(a), 2), ("read", lambda: a.sum(), 1),
+                                ("write", lambda: b.fill_(1.5), 1)):
+            fn()
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            res[name + "_GBps"] = moved * n * 8 / ms / 1e6
+        out["%d_MiB" % mib] = res
+        del a, b
+    return out
+
+
 def copy_calibration(mib):
     """Device-to-device copies of `mib` MiB: under a counter profiler, the copy kernel's
     FETCH_SIZE / WRITE_SIZE per dispatch against its known size give the counters' scale."""
@@ -497,7 +525,12 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device three times (a known-size copy kernel "
                          "for scaling hardware byte counters)")
+    ap.add_argument("--copy-rates", action="store_true",
+                    help="print the device's copy / read / write bandwidth (one JSON line) and exit")
     args = ap.parse_args()
+    if args.copy_rates:
+        print(json.dumps(copy_rates()), flush=True)
+        return
     legs = set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,

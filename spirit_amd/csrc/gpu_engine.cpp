@@ -715,7 +715,10 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
     upload_spins();
     if( min_is_solver( solver ) )
         setup_minimiser();
+    else
+        tune_stage_tiles();
 }
+
 
 // Buffers and scalar state of a direct-minimisation run, zero as after the solver's
 // Initialize (Solver_VP.hpp:2-18, Solver_VP_OSO.hpp:2-19, Solver_LBFGS_OSO.hpp:12-31,
@@ -875,6 +878,8 @@ void LLGRunner::refresh_hamiltonian()
         const uint64_t old_ddi = hams[k]->ddi_version;
         hams[k]->upload( *specs[k].geometry, *specs[k].hamiltonian );
         imgs_host[k].ham = hams[k]->dev;
+        if( tuned_row_tile >= 0 )
+            imgs_host[k].ham.row_tile_log2 = tuned_row_tile;
         if( specs[k].hamiltonian->ddi_method == DDI_FFT || specs[k].hamiltonian->ddi_method == DDI_CUTOFF )
         {
             if( !ddi[k] || old_ddi != specs[k].hamiltonian->ddi_version )
@@ -989,6 +994,67 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
 #define STEP_ARGS()                                                                                                    \
     base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop, \
                   b_xi )
+
+// The stage kernels' row tile (rows along b swept through the c planes) by timing: the first
+// solver stage at tiles of 8, 16 and 32 rows, where every image's rows hold whole blocks and
+// Nb is a multiple of the tile, three launches each after a warm-up; the fastest is kept for
+// the run (lattices below 2^20 spins keep the default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the
+// tile instead.
+void LLGRunner::tune_stage_tiles()
+{
+    if( std::getenv( "SPIRIT_HIP_STAGE_ROW_TILE" ) || n_total < ( 1LL << 20 ) )
+        return;
+    std::vector<int> cand;
+    for( int tl : { 3, 4, 5 } )
+    {
+        bool ok = true;
+        for( auto & I : imgs_host )
+        {
+            const HamDev & H = I.ham;
+            ok = ok && ( H.nb * H.Na ) % LLG_THREADS == 0 && H.Nb % ( 1 << tl ) == 0 && H.Nc > 1;
+        }
+        if( ok )
+            cand.push_back( tl );
+    }
+    if( cand.size() < 2 )
+        return;
+    StepArgs A = STEP_ARGS();
+    A.obs      = 0;
+    hipEvent_t e0, e1;
+    HIPCHECK( hipEventCreate( &e0 ) );
+    HIPCHECK( hipEventCreate( &e1 ) );
+    double best = 1e30;
+    int pick    = cand[0];
+    for( int tl : cand )
+    {
+        for( auto & I : imgs_host )
+            I.ham.row_tile_log2 = tl;
+        upload_images();
+        llg_launch_stage( solver, 0, A, n_blocks, stream );
+        double t = 1e30;
+        for( int rep = 0; rep < 3; ++rep )
+        {
+            HIPCHECK( hipEventRecord( e0, stream ) );
+            llg_launch_stage( solver, 0, A, n_blocks, stream );
+            HIPCHECK( hipEventRecord( e1, stream ) );
+            HIPCHECK( hipEventSynchronize( e1 ) );
+            float ms = 0;
+            HIPCHECK( hipEventElapsedTime( &ms, e0, e1 ) );
+            t = std::min( t, (double)ms );
+        }
+        if( t < best )
+        {
+            best = t;
+            pick = tl;
+        }
+    }
+    (void)hipEventDestroy( e0 );
+    (void)hipEventDestroy( e1 );
+    tuned_row_tile = pick;
+    for( auto & I : imgs_host )
+        I.ham.row_tile_log2 = pick;
+    upload_images();
+}
 
 void LLGRunner::launch_step( const StepArgs & A )
 {

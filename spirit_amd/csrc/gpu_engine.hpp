@@ -289,6 +289,8 @@ private:
     DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
         b_noise, b_stopstep, b_img_stop, b_nstop, b_step, b_xi;
     void alloc_thermal_field();
+    void tune_stage_tiles();
+    int tuned_row_tile = -1; // row_tile_log2 picked by tune_stage_tiles (-1: the default)
     std::vector<double> obs_host;
     std::vector<int> img_stopped; // host view of the per-image stop flags
     std::vector<long> steps_last; // steps each image advanced in the last run
