@@ -24,11 +24,11 @@ SPIRIT_API void Parameters_LLG_Set_Time_Step( State * state, float dt, int idx_i
 SPIRIT_API void Parameters_LLG_Set_Damping( State * state, float damping, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Parameters_LLG.h:104 */
 SPIRIT_API void Parameters_LLG_Set_Non_Adiabatic_Damping( State * state, float beta, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
-/* Parameters_LLG.h:113 (spin-transfer torque is stored but not evaluated by the HIP engine) */
+/* Parameters_LLG.h:113 (a normal shorter than 0.9 becomes (0, 0, 1), as Parameters_LLG.cpp:332-336) */
 SPIRIT_API void Parameters_LLG_Set_STT( State * state, bool use_gradient, float magnitude, const float normal[3], int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Parameters_LLG.h:118 */
 SPIRIT_API void Parameters_LLG_Set_Temperature( State * state, float T, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
-/* Parameters_LLG.h:126 (stored; the HIP engine uses a homogeneous temperature) */
+/* Parameters_LLG.h:126 (the direction is stored as given and normalised where the distribution is formed) */
 SPIRIT_API void Parameters_LLG_Set_Temperature_Gradient( State * state, float inclination, const float direction[3], int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Parameters_LLG.h:135 */
 SPIRIT_API const char * Parameters_LLG_Get_Output_Tag( State * state, int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;

@@ -75,6 +75,12 @@ class OrcLLG(ctypes.Structure):
         ("direct_minimization", ctypes.c_int),
         ("seed", ctypes.c_uint64),
         ("noise", ctypes.POINTER(ctypes.c_double)),
+        ("beta", ctypes.c_double),
+        ("stt_use_gradient", ctypes.c_int),
+        ("stt_magnitude", ctypes.c_double),
+        ("stt_normal", ctypes.c_double * 3),
+        ("tgrad_inclination", ctypes.c_double),
+        ("tgrad_direction", ctypes.c_double * 3),
     ]
 
 
@@ -133,6 +139,8 @@ def lib():
         L.orc_hessian.argtypes = [vp, P(d), P(d)]
         L.orc_ddi_cutoff_pairs.argtypes = [vp, P(i), P(d), P(d), i]
         L.orc_ddi_cutoff_pairs.restype = i
+        L.orc_jacobian_direction.argtypes = [vp, P(d), P(d), P(d)]
+        L.orc_temperature_distribution.argtypes = [vp, P(d), d, d, P(d)]
         L.orc_llg_iterate.argtypes = [vp, i, P(OrcLLG), P(d), i, ctypes.c_uint64, P(d), P(d), P(d)]
         L.orc_llg_run.argtypes = [vp, i, P(OrcLLG), P(d), ctypes.c_long, d, P(d), P(d), P(d)]
         L.orc_llg_run.restype = ctypes.c_long
@@ -344,13 +352,37 @@ class System:
         lib().orc_hessian(self.h, _dp(s), _dp(H))
         return H
 
+    def jacobian_direction(self, vf, je):
+        """Vectormath::jacobian of the field vf (nos, 3) applied to the direction je."""
+        v = np.ascontiguousarray(vf, dtype=np.float64).reshape(self.nos, 3)
+        out = np.zeros((self.nos, 3))
+        lib().orc_jacobian_direction(self.h, _dp(v), _dp(np.asarray(je, dtype=np.float64)), _dp(out))
+        return out
+
+    def temperature_distribution(self, direction, T, inclination):
+        """Per-site temperature of a gradient (Vectormath::get_gradient_distribution)."""
+        out = np.zeros(self.nos)
+        lib().orc_temperature_distribution(self.h, _dp(np.asarray(direction, dtype=np.float64)), ctypes.c_double(T),
+                                           ctypes.c_double(inclination), _dp(out))
+        return out
+
     def llg(self, spins, solver, n_steps, dt=1e-3, damping=0.3, temperature=0.0, seed=0, step0=0,
-            direct_minimization=False, noise=None):
+            direct_minimization=False, noise=None, beta=0.0, stt=None, temperature_gradient=None):
+        """stt = (magnitude, normal, use_gradient); temperature_gradient = (inclination, direction)."""
         s = np.array(spins, dtype=np.float64).reshape(self.nos, 3).copy()
         par = OrcLLG()
         par.dt, par.damping, par.temperature = dt, damping, temperature
         par.direct_minimization = 1 if direct_minimization else 0
         par.seed = seed
+        par.beta = beta
+        if stt is not None:
+            mag, normal, grad = stt
+            par.stt_magnitude = mag
+            par.stt_normal[:] = _normalized(normal)
+            par.stt_use_gradient = 1 if grad else 0
+        if temperature_gradient is not None:
+            par.tgrad_inclination = temperature_gradient[0]
+            par.tgrad_direction[:] = [float(x) for x in temperature_gradient[1]]
         keep = None
         if noise is not None:
             keep = np.ascontiguousarray(noise, dtype=np.float64)

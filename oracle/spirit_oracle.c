@@ -1462,13 +1462,149 @@ void orc_sib_transform( const double s[3], const double f[3], double out[3] )
     out[0] = o0, out[1] = o1, out[2] = o2;
 }
 
+/* Geometry::calculateBounds, core/src/data/Geometry.cpp:729-743: the bounds start at 0 */
+static void geometry_bounds( const orc_sys * s, double bmin[3], double bmax[3] )
+{
+    for( int q = 0; q < 3; ++q )
+        bmin[q] = bmax[q] = 0;
+    for( int i = 0; i < s->nos; ++i )
+        for( int q = 0; q < 3; ++q )
+        {
+            double v = s->positions[3 * i + q];
+            if( v < bmin[q] )
+                bmin[q] = v;
+            if( v > bmax[q] )
+                bmax[q] = v;
+        }
+}
+
+/* Vectormath::get_gradient_distribution, Vectormath.cpp:633-652 */
+void orc_temperature_distribution( const orc_sys * s, const double direction[3], double T, double inclination,
+                                   double * out )
+{
+    double d[3] = { direction[0], direction[1], direction[2] };
+    double n    = sqrt( d[0] * d[0] + d[1] * d[1] + d[2] * d[2] );
+    if( n > 0 )
+        for( int q = 0; q < 3; ++q )
+            d[q] /= n; /* :638 */
+    double bmin[3], bmax[3];
+    geometry_bounds( s, bmin, bmax );
+    /* :641 distribution_i = inclination * (d . r_i) */
+    for( int i = 0; i < s->nos; ++i )
+    {
+        const double * r = s->positions + 3 * i;
+        out[i]           = inclination * ( d[0] * r[0] + d[1] * r[1] + d[2] * r[2] );
+    }
+    /* :644-648 the start: T at the smaller of the bounds' projections */
+    double pmin     = bmin[0] * d[0] + bmin[1] * d[1] + bmin[2] * d[2];
+    double pmax     = bmax[0] * d[0] + bmax[1] * d[1] + bmax[2] * d[2];
+    double dist_min = pmin < pmax ? pmin : pmax;
+    double shift    = T - inclination * dist_min;
+    for( int i = 0; i < s->nos; ++i )
+    {
+        double t = out[i] + shift;
+        /* :651 set_range(0, 1e30) */
+        out[i] = t < 0 ? 0 : ( t > 1e30 ? 1e30 : t );
+    }
+}
+
+/* Vectormath::jacobian, Vectormath.cpp:816-898, then s_c_grad = J je (Method_LLG.cpp:188-196) */
+void orc_jacobian_direction( const orc_sys * s, const double * vf, const double je[3], double * out )
+{
+    const orc_params * p = &s->p;
+    const int N = p->n_cell_atoms, n[3] = { p->n_cells[0], p->n_cells[1], p->n_cells[2] };
+    /* :827-838 base matrix of the scaled Bravais vectors (as columns) and its inverse */
+    double B[3][3], Bi[3][3];
+    for( int r = 0; r < 3; ++r )
+        for( int k = 0; k < 3; ++k )
+            B[r][k] = p->lattice_constant * p->bravais[k][r];
+    double det = B[0][0] * ( B[1][1] * B[2][2] - B[1][2] * B[2][1] ) - B[0][1] * ( B[1][0] * B[2][2] - B[1][2] * B[2][0] )
+                 + B[0][2] * ( B[1][0] * B[2][1] - B[1][1] * B[2][0] );
+    Bi[0][0] = ( B[1][1] * B[2][2] - B[1][2] * B[2][1] ) / det;
+    Bi[0][1] = ( B[0][2] * B[2][1] - B[0][1] * B[2][2] ) / det;
+    Bi[0][2] = ( B[0][1] * B[1][2] - B[0][2] * B[1][1] ) / det;
+    Bi[1][0] = ( B[1][2] * B[2][0] - B[1][0] * B[2][2] ) / det;
+    Bi[1][1] = ( B[0][0] * B[2][2] - B[0][2] * B[2][0] ) / det;
+    Bi[1][2] = ( B[0][2] * B[1][0] - B[0][0] * B[1][2] ) / det;
+    Bi[2][0] = ( B[1][0] * B[2][1] - B[1][1] * B[2][0] ) / det;
+    Bi[2][1] = ( B[0][1] * B[2][0] - B[0][0] * B[2][1] ) / det;
+    Bi[2][2] = ( B[0][0] * B[1][1] - B[0][1] * B[1][0] ) / det;
+#pragma omp parallel for collapse( 3 )
+    for( int c = 0; c < n[2]; ++c )
+        for( int b = 0; b < n[1]; ++b )
+            for( int a = 0; a < n[0]; ++a )
+                for( int ib = 0; ib < N; ++ib )
+                {
+                    const int pos[3] = { a, b, c };
+                    const long cur   = ib + (long)N * ( a + (long)n[0] * ( b + (long)n[1] * c ) );
+                    double Mcol[3][3]; /* Mcol[t] = column t of m_matrix */
+                    for( int t = 0; t < 3; ++t )
+                    {
+                        /* linear_idx (Vectormath.cpp:773-814) at +-1 along translation t */
+                        long idx[2];
+                        for( int side = 0; side < 2; ++side )
+                        {
+                            int q[3] = { pos[0], pos[1], pos[2] };
+                            q[t] += side == 0 ? 1 : -1;
+                            if( q[t] < 0 || q[t] >= n[t] )
+                            {
+                                if( p->bc[t] )
+                                    q[t] = ( n[t] + ( q[t] % n[t] ) ) % n[t];
+                                else
+                                {
+                                    idx[side] = -1;
+                                    continue;
+                                }
+                            }
+                            idx[side] = ib + (long)N * ( q[0] + (long)n[0] * ( q[1] + (long)n[1] * q[2] ) );
+                        }
+                        double factor = 0.5; /* central difference */
+                        const double *m0, *m1;
+                        if( idx[0] >= 0 )
+                            m0 = vf + 3 * idx[0];
+                        else
+                        {
+                            m0 = vf + 3 * cur;
+                            factor *= 2;
+                        }
+                        if( idx[1] >= 0 )
+                            m1 = vf + 3 * idx[1];
+                        else
+                        {
+                            m1 = vf + 3 * cur;
+                            factor *= 2;
+                        }
+                        for( int r = 0; r < 3; ++r )
+                            Mcol[t][r] = factor * ( m0[r] - m1[r] );
+                    }
+                    /* :896 J = m_matrix * B^-1, then J je */
+                    for( int r = 0; r < 3; ++r )
+                    {
+                        double v = 0;
+                        for( int k = 0; k < 3; ++k )
+                        {
+                            double Jrk = Mcol[0][r] * Bi[0][k] + Mcol[1][r] * Bi[1][k] + Mcol[2][r] * Bi[2][k];
+                            v += Jrk * je[k];
+                        }
+                        out[3 * cur + r] = v;
+                    }
+                }
+}
+
 /* Method_LLG::Prepare_Thermal_Field, Method_LLG.cpp:66-110 (generator replaced, see header) */
 static void prepare_thermal( orc_sys * s, const orc_llg * par, uint64_t step, int local_step )
 {
-    if( !( par->temperature > 0 ) )
+    if( !( par->temperature > 0 || par->tgrad_inclination != 0 ) )
         return;
     double a   = par->damping;
     double eps = sqrt( 2 * a * par->dt * ORC_GAMMA / ORC_MU_B * ORC_K_B ) / ( 1 + a * a );
+    /* :80-93: with a gradient, the site's temperature from the distribution */
+    double * Ti = NULL;
+    if( par->tgrad_inclination != 0 )
+    {
+        Ti = (double *)malloc( sizeof( double ) * (size_t)s->nos );
+        orc_temperature_distribution( s, par->tgrad_direction, par->temperature, par->tgrad_inclination, Ti );
+    }
 #pragma omp parallel for
     for( int i = 0; i < s->nos; ++i )
     {
@@ -1480,10 +1616,11 @@ static void prepare_thermal( orc_sys * s, const orc_llg * par, uint64_t step, in
         }
         else
             orc_normal3( par->seed, step, (uint64_t)i, z );
-        double f = eps * sqrt( par->temperature / s->mu_s[i] );
+        double f = eps * sqrt( ( Ti ? Ti[i] : par->temperature ) / s->mu_s[i] );
         for( int q = 0; q < 3; ++q )
             s->xi[3 * i + q] = f * z[q];
     }
+    free( Ti );
 }
 
 /* Method_LLG::Calculate_Force, Method_LLG.cpp:113-129 */
@@ -1496,7 +1633,7 @@ static double calc_force( orc_sys * s, const double * spins, double * force )
     return E;
 }
 
-/* Method_LLG::Calculate_Force_Virtual, Method_LLG.cpp:132-226 (no STT) */
+/* Method_LLG::Calculate_Force_Virtual, Method_LLG.cpp:132-226 */
 static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, const double * spins, const double * F,
                                 double * Fv )
 {
@@ -1518,7 +1655,16 @@ static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, co
         return;
     }
     double dtg = par->dt * ORC_GAMMA / ORC_MU_B / ( 1 + a * a );
-    int thermal = par->temperature > 0;
+    int thermal = par->temperature > 0 || par->tgrad_inclination != 0;
+    /* STT (:150-157, 182-212): a_j = b_j = stt_magnitude, the current direction je = s_c_vec */
+    const double a_j = par->stt_magnitude, beta = par->beta;
+    const int stt    = a_j > 0;
+    double * s_c_grad = NULL;
+    if( stt && par->stt_use_gradient )
+    {
+        s_c_grad = (double *)malloc( sizeof( double ) * 3 * (size_t)s->nos );
+        orc_jacobian_direction( s, spins, par->stt_normal, s_c_grad );
+    }
 #pragma omp parallel for
     for( int i = 0; i < s->nos; ++i )
     {
@@ -1530,6 +1676,27 @@ static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, co
             v[q] += dtg * a * c[q];
         for( int q = 0; q < 3; ++q )
             v[q] /= s->mu_s[i];
+        if( stt && par->stt_use_gradient )
+        {
+            /* :197-201: + dtg a_j (alpha - beta) g + dtg a_j (1 + beta alpha) g x s */
+            const double * g = s_c_grad + 3 * i;
+            const double c1 = dtg * a_j * ( a - beta ), c2 = dtg * a_j * ( 1 + beta * a );
+            for( int q = 0; q < 3; ++q )
+                v[q] += c1 * g[q];
+            v3_cross( g, spins + 3 * i, c );
+            for( int q = 0; q < 3; ++q )
+                v[q] += c2 * c[q];
+        }
+        else if( stt )
+        {
+            /* :206-207: - dtg a_j (alpha - beta) s_c - dtg a_j (1 + beta alpha) s_c x s */
+            const double c1 = -dtg * a_j * ( a - beta ), c2 = -dtg * a_j * ( 1 + beta * a );
+            for( int q = 0; q < 3; ++q )
+                v[q] += c1 * par->stt_normal[q];
+            v3_cross( par->stt_normal, spins + 3 * i, c );
+            for( int q = 0; q < 3; ++q )
+                v[q] += c2 * c[q];
+        }
         if( thermal )
         {
             const double * xi = s->xi + 3 * i;
@@ -1542,6 +1709,7 @@ static void calc_force_virtual( orc_sys * s, const orc_llg * par, int solver, co
         for( int q = 0; q < 3; ++q )
             Fv[3 * i + q] = v[q];
     }
+    free( s_c_grad );
 }
 
 /* -(a x b) helper: out = -1 * a.cross(b) (set_c_cross(-1, a, b, out)) */

@@ -160,7 +160,23 @@ typedef struct orc_llg
     int direct_minimization;
     uint64_t seed;             /* Philox key for the thermal field */
     const double * noise;      /* if non-NULL: injected N(0,1) values, [n_steps][nos][3] */
+    /* Parameters_Method_LLG.hpp:19-38 */
+    double beta;               /* non-adiabatic STT parameter */
+    int stt_use_gradient;      /* 1: the spin derivative along the current, 0: monolayer */
+    double stt_magnitude;      /* a_j; STT acts for a_j > 0 */
+    double stt_normal[3];      /* polarisation / current direction (unit) */
+    double tgrad_inclination;  /* K per length unit of the positions */
+    double tgrad_direction[3];
 } orc_llg;
+
+/* Vectormath::jacobian (core/src/engine/Vectormath.cpp:816-898) of the vector field vf (AoS),
+   applied to the direction je as Method_LLG.cpp:188-196 does: out_i = J_i je */
+void orc_jacobian_direction( const orc_sys * s, const double * vf, const double je[3], double * out );
+
+/* Vectormath::get_gradient_distribution (core/src/engine/Vectormath.cpp:633-652) with the
+   cut to [0, 1e30] of Method_LLG.cpp:84-86: the per-site temperature */
+void orc_temperature_distribution( const orc_sys * s, const double direction[3], double T, double inclination,
+                                   double * out );
 
 /* Iterate n_steps LLG steps in place (Method_LLG.cpp:66-301 + Solver_*.hpp).
    step0: global index of the first step (thermal-noise counter).

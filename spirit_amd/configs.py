@@ -8,7 +8,9 @@ external_field_magnitude/normal, anisotropy_magnitude/normal,
 cubic_anisotropy_magnitude, hamiltonian, n_shells_exchange, jij, n_shells_dmi,
 dij, dm_chirality, pairs (heisenberg_pairs rows i j da db dc Dx Dy Dz J),
 ddi_method, ddi_n_periodic_images, ddi_pb_zero_padding, ddi_radius, quadruplets (rows
-i j da_j db_j dc_j k da_k db_k dc_k l da_l db_l dc_l Q), llg_*.
+i j da_j db_j dc_j k da_k db_k dc_k l da_l db_l dc_l Q), llg_* (including llg_beta,
+llg_stt_magnitude / _use_gradient / _polarisation_normal and
+llg_temperature_gradient_inclination / _direction).
 """
 
 
@@ -78,6 +80,15 @@ def config_text(cfg):
               "gneb_n_energy_interpolations"):
         if k in cfg:
             lines.append("%s %s" % (k, repr(cfg[k]) if isinstance(cfg[k], float) else cfg[k]))
+    # after llg_temperature: the reader takes the first line that starts with a keyword
+    for k in ("llg_temperature_gradient_direction", "llg_stt_polarisation_normal"):
+        if k in cfg:
+            lines.append("%s %s" % (k, _vec([float(x) for x in cfg[k]])))
+    for k in ("llg_beta", "llg_stt_magnitude", "llg_temperature_gradient_inclination"):
+        if k in cfg:
+            lines.append("%s %r" % (k, float(cfg[k])))
+    if "llg_stt_use_gradient" in cfg:
+        lines.append("llg_stt_use_gradient %d" % int(bool(cfg["llg_stt_use_gradient"])))
     return "\n".join(lines) + "\n"
 
 

@@ -22,6 +22,10 @@ AdjointRunner::AdjointRunner( const Geometry & g, const Hamiltonian & h, const L
         throw std::runtime_error( "adjoint: LLG dynamics only (direct minimisation is not differentiated)" );
     if( !h.quads.empty() )
         throw std::runtime_error( "adjoint: quadruplet interactions are not differentiated" );
+    if( llg.stt_magnitude > 0 )
+        throw std::runtime_error( "adjoint: spin-transfer torque is not differentiated" );
+    if( llg.temperature_gradient_inclination != 0 )
+        throw std::runtime_error( "adjoint: temperature gradients are not differentiated" );
     if( n_steps < 1 )
         throw std::runtime_error( "adjoint: at least one step" );
     if( ckpt < 1 || ckpt > n_steps )

@@ -390,17 +390,6 @@ static void llg_save_current( MethodLLG & m, long iteration, bool initial, bool 
     llg_save_energy( *m.image, m.idx_image, m.starttime, iteration, m.n_iterations, initial, final );
 }
 
-// LLG dynamics the engine does not evaluate refuse to start rather than run other physics.
-static void check_llg_supported( const LLGParameters & P, int solver )
-{
-    const bool dynamics = !P.direct_minimization && ( solver == SOLVER_SIB || solver == SOLVER_DEPONDT
-                                                      || solver == SOLVER_HEUN || solver == SOLVER_RK4 );
-    if( dynamics && P.stt_magnitude > 0 )
-        throw api_error( 2, "Simulation_LLG_Start: spin-transfer torque is not available in the HIP engine" );
-    if( dynamics && P.temperature_gradient_inclination != 0 )
-        throw api_error( 2, "Simulation_LLG_Start: temperature gradients are not available in the HIP engine" );
-}
-
 // Whether the step files of the output parameters are written at log steps.
 static bool llg_step_files( const LLGParameters & P )
 {
@@ -460,7 +449,6 @@ try
     if( image->iteration_allowed || chain->iteration_allowed )
         throw api_error( 3, "Tried to use Simulation_Start on image " + std::to_string( idx_image ) + " of chain "
                                 + std::to_string( idx_chain ) + ", but there is already a simulation running." );
-    check_llg_supported( *image->llg, solver_type );
     auto m = std::make_shared<MethodLLG>();
     {
         std::lock_guard<std::recursive_mutex> lk( image->mtx );
@@ -586,7 +574,6 @@ try
         for( int j = 0; j < k; ++j )
             if( E[j].image == E[k].image )
                 throw api_error( 3, "Spirit_HIP_Simulation_LLG_Start_Batch: an image appears twice" );
-        check_llg_supported( *E[k].image->llg, solver_type );
     }
     std::vector<ImageSpec> specs;
     long am = 1;

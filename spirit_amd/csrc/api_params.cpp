@@ -16,6 +16,27 @@ using namespace spirit_hip;
     std::shared_ptr<Chain> chain;                                                                                      \
     from_indices( state, idx_image, idx_chain, image, chain )
 
+static Vec3 as_vec3( const float * v )
+{
+    return Vec3{ v[0], v[1], v[2] };
+}
+
+// Parameters_LLG_Set_STT (Parameters_LLG.cpp:324-338): a normal shorter than 0.9 becomes (0, 0, 1)
+static Vec3 stt_normal( const float * v )
+{
+    const double len2 = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+    if( len2 < 0.81 )
+    {
+        log_message( 3, "s_c_vec = {0,0,0} replaced by {0,0,1}" );
+        return Vec3{ 0, 0, 1 };
+    }
+    Vec3 n{ v[0], v[1], v[2] };
+    const double s = std::sqrt( len2 );
+    for( auto & x : n )
+        x /= s;
+    return n;
+}
+
 static Vec3 normalized3( const float * v )
 {
     Vec3 n{ v[0], v[1], v[2] };
@@ -1051,8 +1072,7 @@ extern "C" void Parameters_LLG_Set_STT( State * state, bool use_gradient, float 
                                         int idx_image, int idx_chain ) noexcept
 {
     LLG_SET( p.stt_use_gradient = use_gradient; p.stt_magnitude = magnitude;
-             p.stt_polarisation_normal = normalized3( normal );
-             if( magnitude != 0 ) log_message( 3, "Parameters_LLG_Set_STT: spin-transfer torque is not evaluated by the HIP engine" ) );
+             p.stt_polarisation_normal = stt_normal( normal ) );
 }
 extern "C" void Parameters_LLG_Set_Temperature( State * state, float T, int idx_image, int idx_chain ) noexcept
 {
@@ -1061,9 +1081,9 @@ extern "C" void Parameters_LLG_Set_Temperature( State * state, float T, int idx_
 extern "C" void Parameters_LLG_Set_Temperature_Gradient( State * state, float inclination, const float direction[3],
                                                          int idx_image, int idx_chain ) noexcept
 {
+    // stored as given; the distribution normalises it (Vectormath.cpp:638)
     LLG_SET( p.temperature_gradient_inclination = inclination;
-             p.temperature_gradient_direction   = normalized3( direction );
-             if( inclination != 0 ) log_message( 3, "Parameters_LLG_Set_Temperature_Gradient: the HIP engine uses a homogeneous temperature" ) );
+             p.temperature_gradient_direction   = as_vec3( direction ) );
 }
 
 #define LLG_GET( ret, expr )                                                                                           \

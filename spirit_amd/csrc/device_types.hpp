@@ -101,6 +101,14 @@ struct HamDev
     PairDev dmi_g[GATHER_PAIRS];
 };
 
+// Spin-transfer torque of the dynamics' virtual force (Method_LLG.cpp:182-212).
+enum SttMode
+{
+    STT_OFF       = 0,
+    STT_MONOLAYER = 1, // fixed polarisation p
+    STT_GRADIENT  = 2, // p = (j . grad) s, the spin's derivative along the current direction
+};
+
 // LLG parameters of one image (Method_LLG.cpp:132-226, 66-110).
 struct LLGDev
 {
@@ -108,10 +116,17 @@ struct LLGDev
     double damping;
     double temperature;
     int direct_minimization;
-    int thermal; // temperature > 0
+    int thermal; // temperature > 0 or a temperature gradient
     uint64_t seed;
     double force_convergence; // the image stops once its max torque is below this
     double vf_dtg;            // dt gamma / mu_B / (1 + damping^2), the dynamics' virtual-force scale
+    // STT: the virtual force gains stt_c1 p + stt_c2 (p x s) after the division by mu_s
+    int stt;                  // SttMode
+    double stt_c1, stt_c2;
+    double stt_p[3];          // monolayer polarisation
+    double stt_w[3];          // gradient: B^-1 j, the weights of the differences along the three
+                              // lattice translations (B: columns lattice_constant * bravais_k)
+    const double * temp_site; // per-site temperature of a temperature gradient (nullable: homogeneous)
 };
 
 // One image of a batched launch: where its fields live and how it is parameterised.

@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -325,6 +327,17 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
     log_message( 5, msg );
 }
 
+// Pass C choices made in this process, by plan shape: the candidates do not round alike,
+// so two plans of one shape (two images, a rebuilt plan) take the same kernel and give
+// bit-identical fields, whatever the timing noise of a later measurement.
+static std::mutex pass_c_mutex;
+struct PassCChoice
+{
+    int kind, tk;
+    double ms;
+};
+static std::map<std::vector<int>, PassCChoice> pass_c_chosen;
+
 // Times every pass C kernel that can run the plan on a zero configuration (the run time
 // does not depend on the values) and keeps the fastest.  The environment variable
 // SPIRIT_HIP_DDI_PASS_C = generic | lds | reg | reg4 restricts the choice to that kernel where it
@@ -356,6 +369,21 @@ void DdiPlan::choose_pass_c( hipStream_t st )
     c_ms      = 0;
     if( cand.size() == 1 )
         return;
+    std::vector<int> key = { d.Pa, d.Pb, d.Pc, d.Hs, d.Na, d.Nb, d.Nc, d.nb, d.n_inter, d.dh_real, d.sym_b,
+                             d.sym_c, d.KB, d.KC, d.lds_max };
+    for( auto & c : cand )
+        key.push_back( c.first ), key.push_back( c.second );
+    {
+        std::lock_guard<std::mutex> lk( pass_c_mutex );
+        auto it = pass_c_chosen.find( key );
+        if( it != pass_c_chosen.end() )
+        {
+            d.c_kind = it->second.kind;
+            d.c_tk   = it->second.tk;
+            c_ms     = it->second.ms;
+            return;
+        }
+    }
     const size_t nos = (size_t)d.Na * d.Nb * d.Nc * d.nb;
     DeviceBuffer s0( sizeof( double ) * 3 * nos ), h0( sizeof( double ) * 3 * nos );
     HIPCHECK( hipMemsetAsync( s0.as<void>(), 0, s0.size(), st ) );
@@ -390,6 +418,8 @@ void DdiPlan::choose_pass_c( hipStream_t st )
     d.c_kind = pick.first;
     d.c_tk   = pick.second;
     c_ms     = best;
+    std::lock_guard<std::mutex> lk( pass_c_mutex );
+    pass_c_chosen.emplace( key, PassCChoice{ pick.first, pick.second, best } );
 }
 
 void DdiPlan::field( const double * spins_soa, double * h_soa, int nos, hipStream_t st, hipEvent_t evc0,
@@ -620,18 +650,108 @@ void soa_to_aos( const double * soa, double * aos, long long n )
     }
 }
 
-static LLGDev make_llg_dev( const LLGParameters & p )
+static LLGDev make_llg_dev( const LLGParameters & p, const Geometry & g )
 {
     LLGDev L{};
     L.dt                  = p.dt;
     L.damping             = p.damping;
     L.temperature         = p.temperature;
     L.direct_minimization = p.direct_minimization ? 1 : 0;
-    L.thermal             = p.temperature > 0 ? 1 : 0;
+    // Prepare_Thermal_Field draws a field at T > 0 or with a temperature gradient (Method_LLG.cpp:72)
+    L.thermal             = ( p.temperature > 0 || p.temperature_gradient_inclination != 0 ) ? 1 : 0;
     L.seed                = p.seed;
     L.force_convergence   = p.force_convergence;
     L.vf_dtg              = p.dt * gamma_e / mu_B / ( 1 + p.damping * p.damping );
+    // spin-transfer torque, Method_LLG.cpp:150-157, 182-212 (applied for a_j > 0 only)
+    L.stt = STT_OFF;
+    if( p.stt_magnitude > 0 )
+    {
+        const double dtg = L.vf_dtg, aj = p.stt_magnitude, al = p.damping, be = p.beta;
+        const Vec3 & j   = p.stt_polarisation_normal;
+        if( p.stt_use_gradient )
+        {
+            L.stt    = STT_GRADIENT;
+            L.stt_c1 = dtg * aj * ( al - be );
+            L.stt_c2 = dtg * aj * ( 1 + be * al );
+            // Vectormath::jacobian's J = M B^-1 (Vectormath.cpp:827-838, 896) applied to j is
+            // M (B^-1 j): the differences along the three translations weighted by w = B^-1 j
+            double B[3][3];
+            for( int r = 0; r < 3; ++r )
+                for( int k = 0; k < 3; ++k )
+                    B[r][k] = g.lattice_constant * g.bravais[k][r];
+            const double det = B[0][0] * ( B[1][1] * B[2][2] - B[1][2] * B[2][1] )
+                               - B[0][1] * ( B[1][0] * B[2][2] - B[1][2] * B[2][0] )
+                               + B[0][2] * ( B[1][0] * B[2][1] - B[1][1] * B[2][0] );
+            if( det == 0 )
+                throw std::runtime_error( "STT gradient: the Bravais vectors are linearly dependent" );
+            double Bi[3][3];
+            for( int r = 0; r < 3; ++r )
+                for( int k = 0; k < 3; ++k )
+                {
+                    // adjugate: cofactor of B[k][r]
+                    const int r1 = ( k + 1 ) % 3, r2 = ( k + 2 ) % 3, c1 = ( r + 1 ) % 3, c2 = ( r + 2 ) % 3;
+                    Bi[r][k] = ( B[r1][c1] * B[r2][c2] - B[r1][c2] * B[r2][c1] ) / det;
+                }
+            for( int t = 0; t < 3; ++t )
+                L.stt_w[t] = Bi[t][0] * j[0] + Bi[t][1] * j[1] + Bi[t][2] * j[2];
+        }
+        else
+        {
+            L.stt    = STT_MONOLAYER;
+            L.stt_c1 = -dtg * aj * ( al - be );
+            L.stt_c2 = -dtg * aj * ( 1 + be * al );
+            for( int q = 0; q < 3; ++q )
+                L.stt_p[q] = j[q];
+        }
+    }
+    L.temp_site = nullptr;
     return L;
+}
+
+// Vectormath::get_gradient_distribution (Vectormath.cpp:633-652): T_i = incl (d . r_i) +
+// (T - incl min(d . bounds_min, d . bounds_max)), cut to [0, 1e30]
+static std::vector<double> temperature_distribution( const LLGParameters & p, const Geometry & g )
+{
+    Vec3 d         = p.temperature_gradient_direction;
+    const double n = std::sqrt( d[0] * d[0] + d[1] * d[1] + d[2] * d[2] );
+    if( n > 0 )
+        for( int q = 0; q < 3; ++q )
+            d[q] /= n;
+    const double incl  = p.temperature_gradient_inclination;
+    const double bmin  = g.bounds_min[0] * d[0] + g.bounds_min[1] * d[1] + g.bounds_min[2] * d[2];
+    const double bmax  = g.bounds_max[0] * d[0] + g.bounds_max[1] * d[1] + g.bounds_max[2] * d[2];
+    const double shift = p.temperature - incl * std::min( bmin, bmax );
+    std::vector<double> T( g.nos );
+    for( int i = 0; i < g.nos; ++i )
+    {
+        const double * r = &g.positions[3 * (size_t)i];
+        double t         = incl * ( d[0] * r[0] + d[1] * r[1] + d[2] * r[2] );
+        t += shift;
+        T[i] = std::min( std::max( 0.0, t ), 1e30 );
+    }
+    return T;
+}
+
+void LLGRunner::apply_llg( size_t k )
+{
+    const LLGParameters & p = specs[k].llg;
+    imgs_host[k].llg        = make_llg_dev( p, *specs[k].geometry );
+    if( b_temp.size() < specs.size() )
+        b_temp.resize( specs.size() );
+    if( p.temperature_gradient_inclination != 0 )
+    {
+        const std::vector<double> T = temperature_distribution( p, *specs[k].geometry );
+        if( !b_temp[k] )
+            b_temp[k].reset( new DeviceBuffer( sizeof( double ) * T.size() ) );
+        HIPCHECK( hipMemcpy( b_temp[k]->as<void>(), T.data(), sizeof( double ) * T.size(), hipMemcpyHostToDevice ) );
+        imgs_host[k].llg.temp_site = b_temp[k]->as<double>();
+    }
+    int any = 0;
+    for( auto & I : imgs_host )
+        any = any || ( I.llg.stt == STT_GRADIENT && !I.llg.direct_minimization );
+    if( any != stt_gradient_any )
+        release_step_graph(); // the captured stage kernels are the other instantiation
+    stt_gradient_any = any;
 }
 
 LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : specs( images ), solver( solver_ )
@@ -653,7 +773,7 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
         hams.back()->upload( *sp.geometry, *sp.hamiltonian );
         ImageDev I{};
         I.ham         = hams.back()->dev;
-        I.llg         = make_llg_dev( sp.llg );
+        I.llg         = make_llg_dev( sp.llg, *sp.geometry );
         I.off         = off;
         I.block_begin = n_blocks;
         I.n_blocks    = ( sp.geometry->nos + LLG_THREADS - 1 ) / LLG_THREADS;
@@ -675,6 +795,8 @@ LLGRunner::LLGRunner( const std::vector<ImageSpec> & images, int solver_ ) : spe
         }
     }
     n_total         = off;
+    for( size_t k = 0; k < specs.size(); ++k )
+        apply_llg( k );
     const size_t f3 = sizeof( double ) * 3 * (size_t)n_total;
     b_s.resize( f3 );
     b_sp.resize( f3 );
@@ -851,8 +973,8 @@ void LLGRunner::upload_images()
 
 void LLGRunner::set_llg( int img, const LLGParameters & p )
 {
-    specs[img].llg      = p;
-    imgs_host[img].llg  = make_llg_dev( p );
+    specs[img].llg = p;
+    apply_llg( img );
     upload_images();
     alloc_thermal_field();
 }
@@ -991,9 +1113,16 @@ static StepArgs base_args_of( DeviceBuffer & imgs, DeviceBuffer & blk, DeviceBuf
     return A;
 }
 
+static StepArgs with_stt( StepArgs A, int stt_gradient )
+{
+    A.stt_gradient = stt_gradient;
+    return A;
+}
+
 #define STEP_ARGS()                                                                                                    \
-    base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_img_stop, \
-                  b_xi )
+    with_stt( base_args_of( b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop,   \
+                            b_img_stop, b_xi ),                                                                        \
+              stt_gradient_any )
 
 // The stage kernels' row tile (rows along b swept through the c planes) by timing: the first
 // solver stage at tiles of 8, 16 and 32 rows, where every image's rows hold whole blocks and

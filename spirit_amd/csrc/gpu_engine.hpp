@@ -289,6 +289,10 @@ private:
     DeviceBuffer b_imgs, b_blk, b_s, b_sp, b_sp2, b_aux, b_aux2, b_h, b_eff, b_efft, b_pE, b_pT, b_stop, b_obs,
         b_noise, b_stopstep, b_img_stop, b_nstop, b_step, b_xi;
     void alloc_thermal_field();
+    // image k's LLGDev from its parameters, with its per-site temperatures (gradient) uploaded
+    void apply_llg( size_t k );
+    std::vector<std::unique_ptr<DeviceBuffer>> b_temp; // per image: T_i of a temperature gradient
+    int stt_gradient_any = 0;                          // some image's STT uses the spin derivative
     void tune_stage_tiles();
     int tuned_row_tile = -1; // row_tile_log2 picked by tune_stage_tiles (-1: the default)
     std::vector<double> obs_host;

@@ -35,7 +35,7 @@ struct StepArgs
     uint64_t step;         // global step index (thermal-noise counter); with step_dev, the offset from *step_dev
     const uint64_t * step_dev; // base step index in device memory (graph replays); nullptr: step is absolute
     int obs;               // compute observables (energy, torque, effective field)
-    int pad;
+    int stt_gradient;      // some image's STT uses the spin derivative along the current (STT_GRADIENT)
 };
 
 void llg_launch_stage( int solver, int stage, const StepArgs & A, int n_blocks, hipStream_t st );

@@ -546,10 +546,56 @@ __device__ __forceinline__ void local_energy_terms( const HamDev & H, const doub
     }
 }
 
-// Method_LLG::Calculate_Force_Virtual (no STT), Method_LLG.cpp:132-226
-// inv_mu = 1 / mu_s of the spin (the dynamics' division by mu_s, as a product)
+// The derivative of the spin field along the current direction j at spin i:
+// Vectormath::jacobian (Vectormath.cpp:816-898) times j (Method_LLG.cpp:188-196).  Along
+// each lattice translation t the central difference (s(+t) - s(-t)) / 2, one-sided (factor 1)
+// where an open boundary removes a neighbour; J j = sum_t diff_t w_t with w = B^-1 j.
+__device__ __forceinline__ void stt_direction_gradient( const HamDev & H, const LLGDev & P,
+                                                        const double * __restrict__ x, int nos, int i, double * g )
+{
+    const int cell = fast_div( i, H.div_nb );
+    const int ib   = i - cell * H.nb;
+    const int rest = fast_div( cell, H.div_Na );
+    const int a    = cell - rest * H.Na;
+    const int c    = fast_div( rest, H.div_Nb );
+    const int b    = rest - c * H.Nb;
+    const int pos[3] = { a, b, c };
+    const int n[3]   = { H.Na, H.Nb, H.Nc };
+    g[0] = g[1] = g[2] = 0;
+#pragma unroll
+    for( int t = 0; t < 3; ++t )
+    {
+        // neighbour coordinate along t at +1 and -1 (linear_idx, Vectormath.cpp:773-814)
+        int up = pos[t] + 1, dn = pos[t] - 1;
+        bool has_up = up < n[t], has_dn = dn >= 0;
+        if( !has_up && H.bc[t] )
+            up -= n[t], has_up = true;
+        if( !has_dn && H.bc[t] )
+            dn += n[t], has_dn = true;
+        double factor = 0.5;
+        int q_up[3] = { a, b, c }, q_dn[3] = { a, b, c };
+        q_up[t] = up, q_dn[t] = dn;
+        const int j_up = has_up ? ib + H.nb * ( q_up[0] + H.Na * ( q_up[1] + H.Nb * q_up[2] ) ) : i;
+        const int j_dn = has_dn ? ib + H.nb * ( q_dn[0] + H.Na * ( q_dn[1] + H.Nb * q_dn[2] ) ) : i;
+        if( !has_up )
+            factor *= 2;
+        if( !has_dn )
+            factor *= 2;
+        double m0[3], m1[3];
+        load3( x, nos, j_up, m0 );
+        load3( x, nos, j_dn, m1 );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            g[q] += factor * ( m0[q] - m1[q] ) * P.stt_w[t];
+    }
+}
+
+// Method_LLG::Calculate_Force_Virtual, Method_LLG.cpp:132-226
+// inv_mu = 1 / mu_s of the spin (the dynamics' division by mu_s, as a product); sg is the
+// spin's STT direction derivative (stt_direction_gradient) when P.stt == STT_GRADIENT
 __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, double inv_mu, const double * s,
-                                               const double * F, const double * xi, double * fv )
+                                               const double * F, const double * xi, double * fv,
+                                               const double * sg = nullptr )
 {
     const double a = P.damping;
     if( solver == SOLVER_LBFGS_OSO || solver == SOLVER_LBFGS_ATLAS || P.direct_minimization || solver == SOLVER_VP
@@ -576,6 +622,19 @@ __device__ __forceinline__ void virtual_force( const LLGDev & P, int solver, dou
 #pragma unroll
     for( int q = 0; q < 3; ++q )
         v[q] *= inv_mu;
+    if( P.stt != STT_OFF )
+    {
+        // monolayer (:203-208): c1 = -dtg a_j (alpha - beta), c2 = -dtg a_j (1 + beta alpha);
+        // gradient (:184-201): the same with the opposite signs and p = (j . grad) s
+        const double * p = P.stt == STT_GRADIENT ? sg : P.stt_p;
+        cross3( p, s, c );
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            v[q] += P.stt_c1 * p[q];
+#pragma unroll
+        for( int q = 0; q < 3; ++q )
+            v[q] += P.stt_c2 * c[q];
+    }
     if( P.thermal )
     {
 #pragma unroll
@@ -604,7 +663,9 @@ __device__ __forceinline__ void thermal_field( const LLGDev & P, double mu, uint
         normal3( P.seed, step, (uint64_t)i, z );
     const double a   = P.damping;
     const double eps = sqrt( 2 * a * P.dt * gamma_e / mu_B * k_B ) / ( 1 + a * a );
-    const double f   = eps * sqrt( P.temperature / mu );
+    // a temperature gradient replaces T by the site's T_i (Method_LLG.cpp:80-93)
+    const double T   = P.temp_site ? P.temp_site[i] : P.temperature;
+    const double f   = eps * sqrt( T / mu );
 #pragma unroll
     for( int q = 0; q < 3; ++q )
         xi[q] = f * z[q];

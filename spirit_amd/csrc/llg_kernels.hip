@@ -23,7 +23,9 @@ namespace spirit_hip
 // ---------------------------------------------------------------------------
 // the stage kernel
 // ---------------------------------------------------------------------------
-template<int SOLVER, int STAGE>
+// STT_G: some image takes the spin-transfer torque from the spin's derivative along the
+// current (a separate instantiation: its six neighbour loads cost registers the others need)
+template<int SOLVER, int STAGE, bool STT_G>
 __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
 {
     if( *A.stop )
@@ -80,8 +82,11 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_llg_stage( StepArgs A )
                     store3( A.xi + base, nos, i, xi );
             }
         }
+        double sg[3] = { 0, 0, 0 };
+        if( STT_G && I->llg.stt == STT_GRADIENT && !I->llg.direct_minimization )
+            stt_direction_gradient( H, I->llg, x, nos, i, sg );
         double fv[3];
-        virtual_force( I->llg, SOLVER, H.inv_mu[basis_of( H, i )], si, F, xi, fv );
+        virtual_force( I->llg, SOLVER, H.inv_mu[basis_of( H, i )], si, F, xi, fv, sg );
         if( A.tape_F )
         {
             store3( A.tape_F + base, nos, i, F );
@@ -282,8 +287,11 @@ __global__ void __launch_bounds__( LLG_THREADS ) k_observe( StepArgs A, int solv
         const double xi[3] = { 0, 0, 0 };
         LLGDev P           = I->llg;
         P.thermal          = 0;
+        double sg[3]       = { 0, 0, 0 };
+        if( P.stt == STT_GRADIENT && !P.direct_minimization )
+            stt_direction_gradient( H, P, x, nos, i, sg );
         double fv[3];
-        virtual_force( P, solver, H.inv_mu[basis_of( H, i )], si, F, xi, fv );
+        virtual_force( P, solver, H.inv_mu[basis_of( H, i )], si, F, xi, fv, sg );
         double d = dot3( fv, si ), pr[3];
 #pragma unroll
         for( int q = 0; q < 3; ++q )
@@ -445,7 +453,10 @@ __global__ void __launch_bounds__( FIN_THREADS )
 template<int SOLVER, int STAGE>
 static void launch_stage( const StepArgs & A, int n_blocks, hipStream_t st )
 {
-    hipLaunchKernelGGL( ( k_llg_stage<SOLVER, STAGE> ), dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A );
+    if( A.stt_gradient )
+        hipLaunchKernelGGL( ( k_llg_stage<SOLVER, STAGE, true> ), dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A );
+    else
+        hipLaunchKernelGGL( ( k_llg_stage<SOLVER, STAGE, false> ), dim3( n_blocks ), dim3( LLG_THREADS ), 0, st, A );
 }
 
 __global__ void k_step_advance( uint64_t * step, int by )

@@ -50,8 +50,35 @@ def set_damping(p_state, damping, idx_image=-1, idx_chain=-1):
     _Set_Damping(ctypes.c_void_p(p_state), _f(damping), _i(idx_image), _i(idx_chain))
 
 
-def set_temperature(p_state, temperature, idx_image=-1, idx_chain=-1):
+_Set_Temperature_Gradient = _fn("Parameters_LLG_Set_Temperature_Gradient", [_vp, _f, ctypes.POINTER(_F3), _i, _i])
+_Get_Temperature_Gradient = _fn("Parameters_LLG_Get_Temperature_Gradient",
+                                [_vp, ctypes.POINTER(_f), ctypes.POINTER(_F3), _i, _i])
+_Set_STT = _fn("Parameters_LLG_Set_STT", [_vp, _b, _f, ctypes.POINTER(_F3), _i, _i])
+_Get_STT = _fn("Parameters_LLG_Get_STT", [_vp, ctypes.POINTER(_b), ctypes.POINTER(_f), ctypes.POINTER(_F3), _i, _i])
+_Set_Non_Adiabatic_Damping = _fn("Parameters_LLG_Set_Non_Adiabatic_Damping", [_vp, _f, _i, _i])
+_Get_Non_Adiabatic_Damping = _fn("Parameters_LLG_Get_Non_Adiabatic_Damping", [_vp, _i, _i], _f)
+
+
+def set_temperature(p_state, temperature, gradient_inclination=0, gradient_direction=(1.0, 0.0, 0.0), idx_image=-1,
+                    idx_chain=-1):
+    """Homogeneous base temperature [K] and a linear gradient of `gradient_inclination` [K per
+    length unit] along `gradient_direction` (core/python/spirit/parameters/llg.py set_temperature)."""
     _Set_Temperature(ctypes.c_void_p(p_state), _f(temperature), _i(idx_image), _i(idx_chain))
+    d = _F3(*gradient_direction)
+    _Set_Temperature_Gradient(ctypes.c_void_p(p_state), _f(gradient_inclination), ctypes.byref(d), _i(idx_image),
+                              _i(idx_chain))
+
+
+def set_stt(p_state, use_gradient, magnitude, direction, idx_image=-1, idx_chain=-1):
+    """Spin-transfer torque: the spin derivative along `direction` (`use_gradient`) or the
+    monolayer approximation with polarisation `direction`, of strength `magnitude`."""
+    d = _F3(*direction)
+    _Set_STT(ctypes.c_void_p(p_state), _b(use_gradient), _f(magnitude), ctypes.byref(d), _i(idx_image), _i(idx_chain))
+
+
+def set_non_adiabatic_damping(p_state, beta, idx_image=-1, idx_chain=-1):
+    """The non-adiabatic parameter beta of the spin-transfer torque."""
+    _Set_Non_Adiabatic_Damping(ctypes.c_void_p(p_state), _f(beta), _i(idx_image), _i(idx_chain))
 
 
 def set_output_general(p_state, any=True, initial=False, final=False, idx_image=-1, idx_chain=-1):
@@ -81,7 +108,24 @@ def get_damping(p_state, idx_image=-1, idx_chain=-1):
 
 
 def get_temperature(p_state, idx_image=-1, idx_chain=-1):
-    return float(_Get_Temperature(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+    """(base temperature, gradient inclination, gradient direction)."""
+    T = float(_Get_Temperature(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+    incl, d = _f(), _F3()
+    _Get_Temperature_Gradient(ctypes.c_void_p(p_state), ctypes.byref(incl), ctypes.byref(d), _i(idx_image),
+                              _i(idx_chain))
+    return T, float(incl.value), [float(x) for x in d]
+
+
+def get_stt(p_state, idx_image=-1, idx_chain=-1):
+    """(magnitude, direction, use_gradient) of the spin-transfer torque."""
+    g, m, d = _b(), _f(), _F3()
+    _Get_STT(ctypes.c_void_p(p_state), ctypes.byref(g), ctypes.byref(m), ctypes.byref(d), _i(idx_image),
+             _i(idx_chain))
+    return float(m.value), [float(x) for x in d], bool(g.value)
+
+
+def get_non_adiabatic_damping(p_state, idx_image=-1, idx_chain=-1):
+    return float(_Get_Non_Adiabatic_Damping(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
 
 
 _Set_Output_Configuration = _fn("Parameters_LLG_Set_Output_Configuration", [_vp, _b, _b, _i, _i, _i])

@@ -738,3 +738,147 @@ def test_quadruplet_guard_in_gradient_and_energy(oracle):
     gx = S2.gradient_term(3, s)
     np.testing.assert_allclose(g2, gx + gq, rtol=0, atol=1e-13)
     assert abs(E2 - (0.5 * np.sum(gx * s) + _quad_energy(base, s))) <= 1e-12 * abs(E2)
+
+
+# ---------------------------------------------------------------------------
+# Spin-transfer torque and temperature gradients (Method_LLG.cpp:66-110, 182-212)
+# ---------------------------------------------------------------------------
+SKEWED = dict(bravais_vectors=[[1.0, 0.0, 0.0], [0.5, 0.8, 0.0], [0.2, 0.3, 0.9]], n_basis_cells=[4, 3, 3],
+              basis=[[0.0, 0.0, 0.0], [0.3, 0.4, 0.2]], lattice_constant=1.7, boundary_conditions=[0, 0, 0],
+              mu_s=[1.0, 2.0], n_shells_exchange=0, n_shells_dmi=0, ddi_method="none")
+
+
+def test_jacobian_direction_linear_field(oracle):
+    """Vectormath::jacobian (Vectormath.cpp:816-898) of a linear field f(r) = A r + b is A
+    at every site of any lattice: the central differences along the three translations and
+    the one-sided ones at open boundaries are exact for it, and J = M B^-1 undoes the
+    (skewed, scaled) Bravais basis.  A transposed or missing B^-1, a dropped factor 1/2 or a
+    wrong boundary factor fails this."""
+    S = oracle.System(SKEWED)
+    pos = S.positions()
+    A = np.array([[0.3, -1.2, 0.7], [2.0, 0.1, -0.4], [-0.6, 0.9, 1.5]])
+    f = pos @ A.T + np.array([0.2, -0.1, 0.4])
+    for je in ([1.0, 0.0, 0.0], [0.3, -0.5, 0.81], [0.0, 0.0, 1.0]):
+        je = np.array(je) / np.linalg.norm(je)
+        out = S.jacobian_direction(f, je)
+        np.testing.assert_allclose(out, np.tile(A @ je, (S.nos, 1)), rtol=0, atol=1e-12)
+
+
+def test_jacobian_direction_helix(oracle):
+    """On a periodic chain of spacing a the central difference of the helix
+    s_n = (cos k n a, sin k n a, 0) along the chain is sin(k a) / a (-sin, cos, 0)_n, and
+    along the other directions (one cell, periodic) it vanishes."""
+    N, lc, m = 24, 0.7, 5
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[N, 1, 1], boundary_conditions=[1, 1, 1], lattice_constant=lc,
+               mu_s=[2.0], n_shells_exchange=0, n_shells_dmi=0, ddi_method="none")
+    S = oracle.System(cfg)
+    k = 2 * math.pi * m / (N * lc)
+    x = np.arange(N) * lc
+    s = np.stack([np.cos(k * x), np.sin(k * x), 0 * x], axis=1)
+    want = math.sin(k * lc) / lc * np.stack([-np.sin(k * x), np.cos(k * x), 0 * x], axis=1)
+    np.testing.assert_allclose(S.jacobian_direction(s, [1.0, 0.0, 0.0]), want, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(S.jacobian_direction(s, [0.0, 1.0, 0.0]), 0 * want, rtol=0, atol=1e-15)
+
+
+def test_temperature_distribution_closed_form(oracle):
+    """Vectormath::get_gradient_distribution (Vectormath.cpp:633-652): along a chain at
+    x_n = n a the temperature rises linearly from T at the smaller end of the gradient's
+    direction and is cut at 0."""
+    N, lc = 20, 0.5
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[N, 1, 1], boundary_conditions=[0, 0, 0], lattice_constant=lc,
+               mu_s=[2.0], n_shells_exchange=0, n_shells_dmi=0, ddi_method="none")
+    S = oracle.System(cfg)
+    x = np.arange(N) * lc
+    L = (N - 1) * lc
+    np.testing.assert_allclose(S.temperature_distribution([2.0, 0, 0], 10.0, 3.0), 10.0 + 3.0 * x, atol=1e-12)
+    np.testing.assert_allclose(S.temperature_distribution([-1.0, 0, 0], 10.0, 3.0), 10.0 + 3.0 * (L - x),
+                               atol=1e-12)
+    np.testing.assert_allclose(S.temperature_distribution([1.0, 0, 0], 2.0, -1.0), np.maximum(0.0, 2.0 - x),
+                               atol=1e-12)
+    np.testing.assert_allclose(S.temperature_distribution([0, 1.0, 0], 7.0, 5.0), np.full(N, 7.0), atol=1e-12)
+
+
+def _free_spin():
+    return dict(bravais_lattice="sc", n_basis_cells=[1, 1, 1], boundary_conditions=[0, 0, 0], mu_s=[2.0],
+                n_shells_exchange=0, n_shells_dmi=0, ddi_method="none")
+
+
+@pytest.mark.parametrize("beta", [0.1, 0.3])
+def test_stt_monolayer_closed_form(oracle, beta):
+    """A free spin under the monolayer torque (Method_LLG.cpp:203-208) obeys
+    ds/dt = w_p p x s + w_d (p x s) x s with w_p = -gamma a_j (alpha - beta) / (mu_B (1 + alpha^2))
+    and w_d = -gamma a_j (1 + beta alpha) / (mu_B (1 + alpha^2)): its polar angle from p follows
+    tan(theta/2) = tan(theta0/2) exp(w_d t) and its azimuth about p turns at w_p.  Depondt,
+    2000 steps of 1 fs."""
+    S = oracle.System(_free_spin())
+    alpha, aj, dt, n = 0.3, 0.4, 1e-3, 2000
+    th0 = 1.0
+    s0 = np.array([[math.sin(th0), 0.0, math.cos(th0)]])
+    s, _, _, _ = S.llg(s0, 2, n, dt=dt, damping=alpha, beta=beta, stt=(aj, [0, 0, 1], False))
+    g = oracle.GAMMA / MU_B_MEV_PER_T / (1 + alpha * alpha)
+    wd, wp = -g * aj * (1 + beta * alpha), -g * aj * (alpha - beta)
+    t = n * dt
+    th = 2 * math.atan(math.tan(th0 / 2) * math.exp(wd * t))
+    ph = wp * t
+    want = np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+    assert np.abs(s[0] - want).max() < 1e-6, (s[0], want)
+    assert abs(th - th0) > 0.5 and (beta == alpha or abs(ph) > 0.02)
+
+
+def test_stt_gradient_helix_translation(oracle):
+    """With beta = alpha the gradient torque (Method_LLG.cpp:184-201) moves a helix rigidly:
+    every spin turns about the helix axis by -dt gamma a_j / mu_B * sin(k a) / a per step,
+    exactly (the torque axis is the same for every spin and stays so).  With beta != alpha the
+    (alpha - beta) term tilts the spins out of the plane by -c1 sin(k a) / a to first order."""
+    N, lc, m = 32, 1.0, 3
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[N, 1, 1], boundary_conditions=[1, 1, 1], lattice_constant=lc,
+               mu_s=[2.0], n_shells_exchange=0, n_shells_dmi=0, ddi_method="none")
+    S = oracle.System(cfg)
+    k = 2 * math.pi * m / (N * lc)
+    x = np.arange(N) * lc
+    s0 = np.stack([np.cos(k * x), np.sin(k * x), 0 * x], axis=1)
+    alpha, aj, dt, n = 0.2, 0.5, 1e-3, 500
+    s, _, _, _ = S.llg(s0, 2, n, dt=dt, damping=alpha, beta=alpha, stt=(aj, [1, 0, 0], True))
+    dphi = -dt * oracle.GAMMA / MU_B_MEV_PER_T * aj * math.sin(k * lc) / lc
+    ph = k * x + n * dphi
+    want = np.stack([np.cos(ph), np.sin(ph), 0 * x], axis=1)
+    np.testing.assert_allclose(s, want, rtol=0, atol=1e-11)
+    assert abs(n * dphi) > 0.1
+    # beta != alpha: one step tilts s_z by -c1 k' with c1 = dtg a_j (alpha - beta)
+    beta = 0.05
+    s1, _, _, _ = S.llg(s0, 2, 1, dt=dt, damping=alpha, beta=beta, stt=(aj, [1, 0, 0], True))
+    c1 = dt * oracle.GAMMA / MU_B_MEV_PER_T / (1 + alpha * alpha) * aj * (alpha - beta)
+    kp = math.sin(k * lc) / lc
+    np.testing.assert_allclose(s1[:, 2], np.full(N, -c1 * kp), rtol=1e-3)
+
+
+def test_temperature_gradient_langevin(oracle):
+    """Free spins in a field under a temperature gradient (Method_LLG.cpp:80-93) equilibrate
+    site by site to the Langevin magnetisation of their own temperature: T rises from 20 K
+    to 60 K along a; the cold and hot quarters of the film each match the mean of
+    L(mu_s mu_B B / k_B T_i) over their sites (0.34 and 0.16, standard error 0.006).  With the
+    gradient ignored (20 K everywhere) the hot quarter would read 0.40; with twice the
+    inclination, about 0.09."""
+    cfg = dict(PARAMAGNET, n_basis_cells=[64, 16, 1])
+    S = oracle.System(cfg)
+    incl, T0 = 40.0 / 63.0, 20.0
+    Ti = S.temperature_distribution([1.0, 0, 0], T0, incl)
+    state = {"s": np.tile([0.0, 0.0, 1.0], (S.nos, 1)), "step": 0}
+    samples = []
+
+    def run(k):
+        s, _, _, _ = S.llg(state["s"], oracle.SOLVER_HEUN, k, dt=cfg["llg_dt"], damping=0.3, temperature=T0,
+                           seed=cfg["llg_seed"], step0=state["step"], temperature_gradient=(incl, [1.0, 0, 0]))
+        state["s"], state["step"] = s, state["step"] + k
+        return s
+
+    run(1500)
+    for _ in range(40):
+        samples.append(run(300)[:, 2].copy())
+    mz = np.array(samples)  # [sample][site]
+    a_idx = np.arange(S.nos) % 64
+    for sel in (a_idx < 16, a_idx >= 48):
+        series = mz[:, sel].mean(axis=1)
+        mean, se = float(series.mean()), float(series.std(ddof=1) / math.sqrt(len(series)))
+        want = float(np.mean([langevin(2.0 * MU_B_MEV_PER_T * 20.0 / (K_B_MEV_PER_K * t)) for t in Ti[sel]]))
+        assert abs(mean - want) < max(5 * se, 0.01), (mean, want, se)
