@@ -113,6 +113,16 @@ void sync_all_hosts( State * state )
 // Parameters or spins changed on the host while a single-shot method holds device copies.
 void notify_image_changed( State * state, int idx_image, bool spins_changed, bool ham_changed, bool llg_changed )
 {
+    auto mc = state->method_mc.find( idx_image );
+    if( mc != state->method_mc.end() )
+    {
+        // a single-shot Monte Carlo run keeps the host spins current after every N_Shot
+        if( ham_changed )
+            mc->second->runner->refresh_hamiltonian();
+        if( spins_changed )
+            mc->second->runner->upload_spins();
+        return;
+    }
     auto it = state->method_image.find( idx_image );
     if( it == state->method_image.end() || !it->second || !it->second->runner )
         return;
@@ -205,6 +215,7 @@ try
     if( !state )
         return;
     state->method_image.clear();
+    state->method_mc.clear();
     state->method_chain.reset();
     delete state;
 }
@@ -728,8 +739,97 @@ static bool mc_colouring( const Geometry & g, const Hamiltonian & h, int & K, in
     return false;
 }
 
+// Method_MC's set-up (Method_MC.cpp:20-46) on the GPU: the colour classes of the lattice, the
+// image's device record and spins, the starting cone angle and the Metropolis kernels' arguments.
+static std::shared_ptr<MethodMC> mc_setup( State * state, const std::shared_ptr<Image> & image, int idx_image )
+{
+    MCParameters & P = *image->mc;
+    int K = 0, col[3] = { 0, 0, 0 };
+    if( !mc_colouring( *image->geometry, *image->hamiltonian, K, col ) )
+        throw api_error( 2, "Simulation_MC_Start: the lattice's pairs admit no colouring into at most 8 classes "
+                            "(one basis atom is required)" );
+    sync_image_host( state, idx_image );
+    auto m   = std::make_shared<MethodMC>();
+    m->image = image;
+    ImageSpec sp{ image->geometry, image->hamiltonian, *image->llg, image->spins.data() };
+    m->runner.reset( new LLGRunner( { sp }, SOLVER_DEPONDT ) );
+    const int nos      = image->nos();
+    const int n_blocks = ( nos + 255 ) / 256;
+    m->cone.resize( 2 * sizeof( double ) );
+    m->part.resize( sizeof( double ) * K * (size_t)n_blocks );
+    m->step.resize( sizeof( uint64_t ) );
+    m->cone0 = Pi * P.metropolis_cone_angle / 180.0; // Method_MC.cpp:34
+    HIPCHECK( hipMemcpy( m->cone.as<void>(), &m->cone0, sizeof( double ), hipMemcpyHostToDevice ) );
+    HIPCHECK( hipMemset( m->part.as<void>(), 0, m->part.size() ) );
+    HIPCHECK( hipMemset( m->step.as<void>(), 0, sizeof( uint64_t ) ) );
+    McArgs & A    = m->A;
+    A.img         = m->runner->dev_images();
+    A.s           = m->runner->dev_spins();
+    A.nos         = nos;
+    A.K           = K;
+    A.ca          = col[0];
+    A.cb          = col[1];
+    A.cc          = col[2];
+    A.step_cone   = P.metropolis_step_cone ? 1 : 0;
+    A.adaptive    = P.metropolis_cone_adaptive ? 1 : 0;
+    A.seed        = P.seed;
+    A.temperature = P.temperature;
+    A.kT          = k_B * P.temperature;
+    A.target      = P.acceptance_ratio_target;
+    A.cone        = m->cone.as<double>();
+    A.part        = m->part.as<double>();
+    A.n_blocks    = n_blocks;
+    A.step        = m->step.as<uint64_t>();
+    // per-colour sweeps where ca is invertible mod K (one basis atom is required above)
+    for( int x = 1; x < K; ++x )
+        if( ( col[0] * x ) % K == 1 )
+            A.inv_ca = x;
+    A.groups        = ( image->geometry->n_cells[0] + K - 1 ) / K;
+    A.n_blocks_rows = (int)( ( (long)A.groups * image->geometry->n_cells[1] * image->geometry->n_cells[2] + 255 )
+                             / 256 );
+    if( A.n_blocks_rows > n_blocks )
+        A.inv_ca = 0;
+    m->n_iterations = std::max( 1L, P.n_iterations );
+    m->t_start      = std::chrono::steady_clock::now();
+    return m;
+}
+
+// n Metropolis iterations (Method_MC::Iteration, Method_MC.cpp:40-183) on the device
+static void mc_sweeps( MethodMC & m, long n )
+{
+    for( long k = 0; k < n; ++k )
+        mc_launch_iteration( m.A, m.step.as<uint64_t>(), m.runner->stream );
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipStreamSynchronize( m.runner->stream ) );
+    m.iteration += n;
+}
+
+// The spins and, with an adaptive cone, the cone angle back to the host; the image's energy.
+static void mc_download( MethodMC & m )
+{
+    m.runner->download_spins();
+    MCParameters & P = *m.image->mc;
+    double cone_end  = m.cone0;
+    HIPCHECK( hipMemcpy( &cone_end, m.cone.as<void>(), sizeof( double ), hipMemcpyDeviceToHost ) );
+    if( P.metropolis_step_cone && P.metropolis_cone_adaptive )
+        P.metropolis_cone_angle = cone_end * 180.0 / Pi;
+    m.image->E = update_image_energy( *m.image, true );
+}
+
+// Method_MC::Finalize (Method_MC.cpp:203-206) of a single-shot run
+static void mc_finalize( State * state, int idx_image )
+{
+    auto it = state->method_mc.find( idx_image );
+    if( it == state->method_mc.end() )
+        return;
+    it->second->image->iteration_allowed  = false;
+    it->second->image->singleshot_allowed = false;
+    state->method_mc.erase( it );
+}
+
 // Method_MC with the Metropolis algorithm (Method_MC.cpp:40-200, Method.cpp:57-113): n_iterations
-// sweeps on the GPU; the spins, the adapted cone angle and the energy come back at the end.
+// sweeps on the GPU; the spins, the adapted cone angle and the energy come back at the end.  In
+// single-shot mode the method is set up and Simulation_SingleShot / _N_Shot run the sweeps.
 extern "C" void Simulation_MC_Start( State * state, int n_iterations, int n_iterations_log, bool singleshot,
                                      Simulation_Run_Info * info, int idx_image, int idx_chain ) noexcept
 try
@@ -740,80 +840,33 @@ try
     if( image->iteration_allowed || chain->iteration_allowed )
         throw api_error( 3, "Tried to use Simulation_Start on image " + std::to_string( idx_image ) + " of chain "
                                 + std::to_string( idx_chain ) + ", but there is already a simulation running." );
-    if( singleshot )
-        throw api_error( 2, "Simulation_MC_Start: single-shot Monte Carlo is not available in the HIP engine" );
     (void)n_iterations_log;
     MCParameters & P = *image->mc;
     if( n_iterations > 0 )
         P.n_iterations = n_iterations;
-    const long n = std::max( 1L, P.n_iterations );
-    int K = 0, col[3] = { 0, 0, 0 };
-    if( !mc_colouring( *image->geometry, *image->hamiltonian, K, col ) )
-        throw api_error( 2, "Simulation_MC_Start: the lattice's pairs admit no colouring into at most 8 classes "
-                            "(one basis atom is required)" );
-    sync_image_host( state, idx_image );
     std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    auto m = mc_setup( state, image, idx_image );
+    if( singleshot )
+    {
+        state->method_mc[idx_image] = m;
+        image->singleshot_allowed   = true;
+        image->iteration_allowed    = true;
+        return;
+    }
     image->iteration_allowed = true;
-    const auto t_start       = std::chrono::steady_clock::now();
-    long done                = 0;
+    const long n             = m->n_iterations;
     try
     {
-        ImageSpec sp{ image->geometry, image->hamiltonian, *image->llg, image->spins.data() };
-        LLGRunner r( { sp }, SOLVER_DEPONDT );
-        const int nos      = image->nos();
-        const int n_blocks = ( nos + 255 ) / 256;
-        DeviceBuffer cone( 2 * sizeof( double ) ), part( sizeof( double ) * K * (size_t)n_blocks ),
-            step( sizeof( uint64_t ) );
-        const double cone0 = Pi * P.metropolis_cone_angle / 180.0; // Method_MC.cpp:34
-        HIPCHECK( hipMemcpy( cone.as<void>(), &cone0, sizeof( double ), hipMemcpyHostToDevice ) );
-        HIPCHECK( hipMemset( part.as<void>(), 0, part.size() ) );
-        HIPCHECK( hipMemset( step.as<void>(), 0, sizeof( uint64_t ) ) );
-        McArgs A{};
-        A.img         = r.dev_images();
-        A.s           = r.dev_spins();
-        A.nos         = nos;
-        A.K           = K;
-        A.ca          = col[0];
-        A.cb          = col[1];
-        A.cc          = col[2];
-        A.step_cone   = P.metropolis_step_cone ? 1 : 0;
-        A.adaptive    = P.metropolis_cone_adaptive ? 1 : 0;
-        A.seed        = P.seed;
-        A.temperature = P.temperature;
-        A.kT          = k_B * P.temperature;
-        A.target      = P.acceptance_ratio_target;
-        A.cone        = cone.as<double>();
-        A.part        = part.as<double>();
-        A.n_blocks    = n_blocks;
-        A.step        = step.as<uint64_t>();
-        // per-colour sweeps where ca is invertible mod K (one basis atom is required above)
-        for( int x = 1; x < K; ++x )
-            if( ( col[0] * x ) % K == 1 )
-                A.inv_ca = x;
-        A.groups        = ( image->geometry->n_cells[0] + K - 1 ) / K;
-        A.n_blocks_rows = (int)( ( (long)A.groups * image->geometry->n_cells[1] * image->geometry->n_cells[2] + 255 )
-                                 / 256 );
-        if( A.n_blocks_rows > n_blocks )
-            A.inv_ca = 0;
         const long batch = 1000;
-        while( done < n && image->iteration_allowed && !stop_file_present() )
+        while( m->iteration < n && image->iteration_allowed && !stop_file_present() )
         {
-            const long m = std::min( batch, n - done );
-            for( long k = 0; k < m; ++k )
-                mc_launch_iteration( A, step.as<uint64_t>(), r.stream );
-            HIPCHECK( hipGetLastError() );
-            HIPCHECK( hipStreamSynchronize( r.stream ) );
-            done += m;
+            mc_sweeps( *m, std::min( batch, n - m->iteration ) );
             if( P.max_walltime_sec > 0
-                && std::chrono::duration<double>( std::chrono::steady_clock::now() - t_start ).count()
+                && std::chrono::duration<double>( std::chrono::steady_clock::now() - m->t_start ).count()
                        > (double)P.max_walltime_sec )
                 break;
         }
-        r.download_spins();
-        double cone_end = cone0;
-        HIPCHECK( hipMemcpy( &cone_end, cone.as<void>(), sizeof( double ), hipMemcpyDeviceToHost ) );
-        if( P.metropolis_step_cone && P.metropolis_cone_adaptive )
-            P.metropolis_cone_angle = cone_end * 180.0 / Pi;
+        mc_download( *m );
     }
     catch( ... )
     {
@@ -821,18 +874,54 @@ try
         throw;
     }
     image->iteration_allowed = false;
-    image->E                 = update_image_energy( *image, true );
     if( info )
     {
-        double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - t_start ).count();
-        info->total_iterations = (int)done;
+        double ms = std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now() - m->t_start ).count();
+        info->total_iterations = (int)m->iteration;
         info->total_walltime   = (int)ms;
-        info->total_ips        = ms > 0 ? float( done / ms * 1000.0 ) : 0.f;
+        info->total_ips        = ms > 0 ? float( m->iteration / ms * 1000.0 ) : 0.f;
     }
 }
 catch( ... )
 {
     handle_exception( idx_image, idx_chain );
+}
+
+// Simulation_N_Shot of a single-shot Monte Carlo run (Simulation.cpp:454-535): N iterations
+// while the method continues, the spins back after them, Finalize once it stops.
+static void mc_n_shot( State * state, int N, int idx_image )
+{
+    auto m           = state->method_mc[idx_image];
+    Image & image    = *m->image;
+    MCParameters & P = *image.mc;
+    auto walltime    = [&]() {
+        return P.max_walltime_sec > 0
+               && std::chrono::duration<double>( std::chrono::steady_clock::now() - m->t_start ).count()
+                      > (double)P.max_walltime_sec;
+    };
+    auto cont = [&]() { return m->iteration < m->n_iterations && image.iteration_allowed && !stop_file_present(); };
+    try
+    {
+        if( cont() && !walltime() )
+        {
+            std::lock_guard<std::recursive_mutex> lk( image.mtx );
+            // all N iterations, as the reference's loop (the continuation is tested before it)
+            const long n = std::max( 0, N );
+            auto t0      = std::chrono::steady_clock::now();
+            mc_sweeps( *m, n );
+            const double sec = std::chrono::duration<double>( std::chrono::steady_clock::now() - t0 ).count();
+            if( sec > 0 && n > 0 )
+                m->ips = n / sec;
+            mc_download( *m );
+        }
+        if( !cont() || walltime() )
+            mc_finalize( state, idx_image );
+    }
+    catch( ... )
+    {
+        mc_finalize( state, idx_image );
+        throw;
+    }
 }
 
 extern "C" void Simulation_MMF_Start( State * state, int solver_type, int n_iterations, int n_iterations_log,
@@ -870,6 +959,11 @@ try
     if( !( image->iteration_allowed && image->singleshot_allowed ) )
         throw api_error( 3, "Tried to use Simulation_SingleShot on image " + std::to_string( idx_image )
                                 + " but no SingleShot simulation has been started." );
+    if( state->method_mc.count( idx_image ) )
+    {
+        mc_n_shot( state, N, idx_image );
+        return;
+    }
     auto m = state->method_image[idx_image];
     try
     {
@@ -934,9 +1028,14 @@ try
         image->iteration_allowed = false;
         if( image->singleshot_allowed )
         {
-            auto m = state->method_image[idx_image];
-            if( m )
-                llg_finalize( *m );
+            if( state->method_mc.count( idx_image ) )
+                mc_finalize( state, idx_image );
+            else
+            {
+                auto m = state->method_image[idx_image];
+                if( m )
+                    llg_finalize( *m );
+            }
         }
     }
     else if( chain->iteration_allowed )
@@ -1009,6 +1108,9 @@ try
     from_indices( state, idx_image, idx_chain, image, chain );
     if( chain->iteration_allowed && state->method_chain )
         return (float)state->method_chain->ips;
+    auto mc = state->method_mc.find( idx_image );
+    if( mc != state->method_mc.end() )
+        return (float)mc->second->ips;
     auto it = state->method_image.find( idx_image );
     if( it != state->method_image.end() && it->second )
         return (float)it->second->ips;
@@ -1026,6 +1128,9 @@ try
     std::shared_ptr<Image> image;
     std::shared_ptr<Chain> chain;
     from_indices( state, idx_image, idx_chain, image, chain );
+    auto mc = state->method_mc.find( idx_image );
+    if( mc != state->method_mc.end() )
+        return (int)mc->second->iteration;
     if( state->method_chain && ( chain->iteration_allowed || !state->method_image.count( idx_image ) ) )
         return (int)state->method_chain->iteration;
     auto it = state->method_image.find( idx_image );
@@ -1062,6 +1167,11 @@ try
     std::shared_ptr<Image> image;
     std::shared_ptr<Chain> chain;
     from_indices( state, idx_image, idx_chain, image, chain );
+    auto mc = state->method_mc.find( idx_image );
+    if( mc != state->method_mc.end() )
+        return (int)std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now()
+                                                               - mc->second->t_start )
+            .count();
     auto it = state->method_image.find( idx_image );
     if( it != state->method_image.end() && it->second )
         return (int)std::chrono::duration<double, std::milli>( std::chrono::steady_clock::now()
@@ -1103,7 +1213,7 @@ try
     if( chain->iteration_allowed )
         return "GNEB";
     if( image->iteration_allowed )
-        return "LLG";
+        return state->method_mc.count( idx_image ) ? "MC" : "LLG";
     return "None";
 }
 catch( ... )

@@ -5,6 +5,7 @@
 
 #include "gpu_engine.hpp"
 #include "host_core.hpp"
+#include "mc.hpp"
 
 #include <atomic>
 #include <chrono>
@@ -89,6 +90,22 @@ struct MethodLLG
     std::vector<float> history_max_torque, history_energy;
 };
 
+// A running Monte Carlo method (one image), core/src/engine/Method_MC.cpp: the image's device
+// spins and Hamiltonian record (held by an LLG runner), the adaptive cone, the rejection
+// partials and the step counter of the Metropolis kernels.
+struct MethodMC
+{
+    std::shared_ptr<Image> image;
+    std::unique_ptr<LLGRunner> runner;
+    DeviceBuffer cone, part, step;
+    McArgs A{};
+    double cone0       = 0;
+    long iteration     = 0;
+    long n_iterations  = 0;
+    double ips         = 0;
+    std::chrono::steady_clock::time_point t_start;
+};
+
 // A running GNEB method on the chain, core/src/engine/Method_GNEB.cpp
 struct MethodGNEB
 {
@@ -117,6 +134,7 @@ struct State
     int noi              = 0;
     int idx_active_image = 0;
     std::map<int, std::shared_ptr<spirit_hip::MethodLLG>> method_image;
+    std::map<int, std::shared_ptr<spirit_hip::MethodMC>> method_mc; // single-shot Monte Carlo
     std::shared_ptr<spirit_hip::MethodGNEB> method_chain;
     std::string datetime;
     std::string config_file;

@@ -107,3 +107,52 @@ def test_mc_zero_temperature_and_cone(make_state, oracle):
     assert abs(system.get_energy(p) - E_prev) <= 1e-5 * abs(E_prev) + 1e-3
     cone = mc.get_metropolis_cone(p)
     assert cone[0] and cone[2] and abs(cone[1] - 30.0) > 0.5
+
+
+def test_mc_single_shot_equals_full_run(make_state):
+    """Simulation_MC_Start in single-shot mode, then Simulation_N_Shot / _SingleShot (the
+    reference's Simulation.cpp:454-535): 25 + 20 + 15 single iterations give the spins, the
+    adapted cone angle and the iteration count of one 60-iteration run, bit for bit (same
+    kernels, same counter-based random numbers); the method ends by itself after the 60th
+    iteration, and Simulation_Stop ends a single-shot run early."""
+    from spirit_amd import simulation, system
+    from spirit_amd.parameters import mc
+    print_case("mc single shot")
+    cfg = dict(bravais_lattice="sc", n_basis_cells=[40, 30, 2], boundary_conditions=[1, 1, 0], mu_s=[2.0],
+               external_field_magnitude=0.5, external_field_normal=[0, 0, 1], anisotropy_magnitude=0.1,
+               anisotropy_normal=[0, 0, 1], n_shells_exchange=1, jij=[3.0], n_shells_dmi=1, dij=[1.0],
+               dm_chirality=2)
+    runs = []
+    for single in (False, True):
+        p = make_state(cfg)
+        s0 = configs.random_spins(system.get_nos(p), 71)
+        system.get_spin_directions(p)[:] = s0
+        mc.set_temperature(p, 15.0)
+        mc.set_metropolis_cone(p, True, 25.0, True, 0.5)
+        if single:
+            simulation.start(p, simulation.METHOD_MC, n_iterations=60, single_shot=True)
+            assert simulation.running_on_image(p)
+            assert simulation.get_method_name(p) == "MC"
+            simulation.n_shot(p, 25)
+            simulation.n_shot(p, 20)
+            assert simulation.get_iteration(p) == 45
+            mid = np.array(system.get_spin_directions(p))
+            assert np.abs(mid - s0).max() > 1e-3  # the host spins follow the run
+            for _ in range(15):
+                simulation.single_shot(p)
+            assert not simulation.running_on_image(p)  # ended after the 60th iteration
+        else:
+            info = simulation.start(p, simulation.METHOD_MC, n_iterations=60)
+            assert info.total_iterations == 60
+        runs.append((np.array(system.get_spin_directions(p)), mc.get_metropolis_cone(p)))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    # Simulation_Stop ends a single-shot run; the spins stay those of the iterations done
+    p = make_state(cfg)
+    system.get_spin_directions(p)[:] = configs.random_spins(system.get_nos(p), 72)
+    simulation.start(p, simulation.METHOD_MC, n_iterations=100, single_shot=True)
+    simulation.n_shot(p, 10)
+    after10 = np.array(system.get_spin_directions(p))
+    simulation.stop(p)
+    assert not simulation.running_on_image(p)
+    np.testing.assert_array_equal(np.array(system.get_spin_directions(p)), after10)

@@ -264,6 +264,38 @@ def test_reference_reader_and_writer_interoperate(make_state, tmp_path):
 
 
 @pytest.mark.gpu
+def test_gpu_run_files_match_reference_writer(make_state, tmp_path):
+    """After an LLG run on the GPU (Depondt, 50 steps, spins back from the device), every
+    format the engine writes equals, byte for byte, what the reference's OVF library writes
+    for the same segment and spins, and the reference's parser reads the engine's file back
+    to the same values."""
+    from spirit_amd import io, simulation, spiritlib
+    print("CASE ovf bytes after a gpu run", flush=True)
+    spiritlib.load_spirit_library().Spirit_Version_Full.restype = ctypes.c_char_p
+    L = ref_ovf()
+    p = make_state(configs.scaled(configs.SQUARE_64, [9, 7, 2]))
+    set_spins(p, unit(np.random.default_rng(12), 126))
+    simulation.start(p, simulation.METHOD_LLG, 2, n_iterations=50)
+    s1 = spins(p)
+    data = np.ascontiguousarray(s1, dtype=np.float64)
+    dp = data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    for fmt, (code, _) in FORMATS.items():
+        ours, theirs = str(tmp_path / ("gpu_%s.ovf" % fmt)), str(tmp_path / ("gpu_ref_%s.ovf" % fmt))
+        io.image_write(p, ours, code, "after a gpu run")
+        f = L.ovf_open(theirs.encode())
+        assert L.ovf_write_segment_8(f, ref_segment(L, p, b"after a gpu run"), dp, code) == -1
+        L.ovf_close(f)
+        assert open(ours, "rb").read() == open(theirs, "rb").read(), fmt
+        f = L.ovf_open(ours.encode())
+        seg = L.ovf_segment_create()
+        assert L.ovf_read_segment_header(f, 0, seg) == -1
+        out = np.zeros((126, 3))
+        assert L.ovf_read_segment_data_8(f, 0, seg, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == -1
+        L.ovf_close(f)
+        assert np.abs(out - s1).max() <= FORMATS[fmt][1], fmt
+
+
+@pytest.mark.gpu
 def test_llg_configuration_output_files(tmp_path):
     """Method_LLG::Save_Current (Method_LLG.cpp:310-495) as Method::Iterate calls it
     (Method.cpp:57-113): <tag>_Image-00_Spins-initial.ovf, -final.ovf, _<iteration>.ovf per
