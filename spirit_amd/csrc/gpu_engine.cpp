@@ -407,6 +407,9 @@ void DdiPlan::choose_pass_c( hipStream_t st )
             HIPCHECK( hipEventElapsedTime( &ms, e0, e1 ) );
             t = std::min( t, (double)ms );
         }
+        char msg[96];
+        std::snprintf( msg, sizeof( msg ), "DDI pass C candidate %d/%d: %.4f ms", c.first, c.second, t );
+        log_message( 6, msg );
         if( t < best )
         {
             best = t;
