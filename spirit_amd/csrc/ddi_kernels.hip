@@ -586,18 +586,62 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_inv( DdiDev d, double * __restri
     const double2 * row0  = d.X + ( ( (long long)arr * d.Nc + c ) * d.Pb + b0 ) * d.Hs;
     const double2 * row1  = row0 + d.Hs;
     double2 v[8];
-#pragma unroll
-    for( int k = 0; k < 8; ++k )
+    if constexpr( T <= 64 )
     {
-        const int kk = t + T * k;
-        double2 X1 = make_double2( 0, 0 ), X2 = make_double2( 0, 0 );
-        if( act )
+        // Only the stored half of each row is read (k < 4, and the Nyquist entry N/2 on t = 0); the
+        // Hermitian half X[N - kk] = conj X[kk] comes from the lanes holding it.  A line lives in
+        // T consecutive lanes of one wavefront (T <= 64): index N - (t + T k) is value 7 - k of
+        // lane T - t for t >= 1, and value 8 - k of lane 0 itself for t = 0.
+        double2 r0[4], r1[4];
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
         {
-            X1 = kk < d.Ha ? row0[kk] : cconj( row0[N - kk] );
-            if( has1 )
-                X2 = kk < d.Ha ? row1[kk] : cconj( row1[N - kk] );
+            r0[k] = act ? row0[t + T * k] : make_double2( 0, 0 );
+            r1[k] = ( act && has1 ) ? row1[t + T * k] : make_double2( 0, 0 );
         }
-        v[k] = make_double2( X1.x - X2.y, X1.y + X2.x );
+        const double2 ny0 = ( act && t == 0 ) ? row0[N / 2] : make_double2( 0, 0 );
+        const double2 ny1 = ( act && has1 && t == 0 ) ? row1[N / 2] : make_double2( 0, 0 );
+        const int lane    = threadIdx.x % 64;
+        const int src     = lane - t + ( ( T - t ) & ( T - 1 ) ); // lane T - t of this line (t = 0: itself)
+        double2 m0[4], m1[4];
+#pragma unroll
+        for( int k = 0; k < 4; ++k )
+        {
+            m0[k] = make_double2( __shfl( r0[k].x, src, 64 ), __shfl( r0[k].y, src, 64 ) );
+            m1[k] = make_double2( __shfl( r1[k].x, src, 64 ), __shfl( r1[k].y, src, 64 ) );
+        }
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+        {
+            double2 X1, X2;
+            if( k < 4 )
+                X1 = r0[k], X2 = r1[k];
+            else if( t == 0 )
+            {
+                X1 = k == 4 ? ny0 : cconj( r0[( 8 - k ) & 3] );
+                X2 = k == 4 ? ny1 : cconj( r1[( 8 - k ) & 3] );
+            }
+            else
+                X1 = cconj( m0[7 - k] ), X2 = cconj( m1[7 - k] );
+            v[k] = make_double2( X1.x - X2.y, X1.y + X2.x );
+        }
+    }
+    else
+    {
+        // a line spans several wavefronts: the Hermitian half is read again (from the caches)
+#pragma unroll
+        for( int k = 0; k < 8; ++k )
+        {
+            const int kk = t + T * k;
+            double2 X1 = make_double2( 0, 0 ), X2 = make_double2( 0, 0 );
+            if( act )
+            {
+                X1 = kk < d.Ha ? row0[kk] : cconj( row0[N - kk] );
+                if( has1 )
+                    X2 = kk < d.Ha ? row1[kk] : cconj( row1[N - kk] );
+            }
+            v[k] = make_double2( X1.x - X2.y, X1.y + X2.x );
+        }
     }
     double * Ld = reinterpret_cast<double *>( lds ) + l * LS;
     constexpr bool WAVE = T <= 64 && 64 % T == 0;
