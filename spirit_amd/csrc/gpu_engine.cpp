@@ -1025,21 +1025,43 @@ void LLGRunner::refresh_hamiltonian()
     upload_images();
 }
 
+// The host mirrors are AoS, the device fields SoA: each image's mirror is copied as it is into
+// a device staging buffer and transposed there (a host transpose of the whole batch costs
+// more than the copies).
 void LLGRunner::upload_spins()
 {
-    std::vector<double> soa( 3 * (size_t)n_total );
+    if( b_stage.size() < sizeof( double ) * 3 * (size_t)n_total )
+        b_stage.resize( sizeof( double ) * 3 * (size_t)n_total );
+    double * stage = b_stage.as<double>();
+    HIPCHECK( hipStreamSynchronize( stream ) ); // no kernel of an earlier call still reads the stage
     for( size_t k = 0; k < specs.size(); ++k )
-        aos_to_soa( specs[k].spins_aos, soa.data() + 3 * imgs_host[k].off, specs[k].geometry->nos );
-    HIPCHECK( hipMemcpy( b_s.as<void>(), soa.data(), sizeof( double ) * soa.size(), hipMemcpyHostToDevice ) );
+    {
+        const long long base = 3 * imgs_host[k].off;
+        const int nos        = specs[k].geometry->nos;
+        HIPCHECK( hipMemcpyAsync( stage + base, specs[k].spins_aos, sizeof( double ) * 3 * (size_t)nos,
+                                  hipMemcpyHostToDevice, stream ) );
+        device_aos_to_soa( stage + base, b_s.as<double>() + base, nos, stream );
+    }
+    HIPCHECK( hipGetLastError() );
+    HIPCHECK( hipStreamSynchronize( stream ) );
 }
 
 void LLGRunner::download_spins()
 {
-    std::vector<double> soa( 3 * (size_t)n_total );
-    HIPCHECK( hipStreamSynchronize( stream ) );
-    HIPCHECK( hipMemcpy( soa.data(), b_s.as<void>(), sizeof( double ) * soa.size(), hipMemcpyDeviceToHost ) );
+    if( b_stage.size() < sizeof( double ) * 3 * (size_t)n_total )
+        b_stage.resize( sizeof( double ) * 3 * (size_t)n_total );
+    double * stage = b_stage.as<double>();
     for( size_t k = 0; k < specs.size(); ++k )
-        soa_to_aos( soa.data() + 3 * imgs_host[k].off, specs[k].spins_aos, specs[k].geometry->nos );
+    {
+        const long long base = 3 * imgs_host[k].off;
+        device_soa_to_aos( b_s.as<double>() + base, stage + base, specs[k].geometry->nos, stream );
+    }
+    HIPCHECK( hipGetLastError() );
+    for( size_t k = 0; k < specs.size(); ++k )
+        HIPCHECK( hipMemcpyAsync( specs[k].spins_aos, stage + 3 * imgs_host[k].off,
+                                  sizeof( double ) * 3 * (size_t)specs[k].geometry->nos, hipMemcpyDeviceToHost,
+                                  stream ) );
+    HIPCHECK( hipStreamSynchronize( stream ) );
 }
 
 void LLGRunner::download_effective_field( int img, double * aos )

@@ -19,6 +19,9 @@ namespace spirit_hip
 // one field of n vectors: AoS [x0,y0,z0,x1,...] <-> SoA [x...][y...][z...]
 void aos_to_soa( const double * aos, double * soa, long long n );
 void soa_to_aos( const double * soa, double * aos, long long n );
+// the same conversions of n spins in device memory, on stream st
+void device_aos_to_soa( const double * aos, double * soa, long long n, hipStream_t st );
+void device_soa_to_aos( const double * soa, double * aos, long long n, hipStream_t st );
 
 void hip_check( hipError_t e, const char * what );
 // division by an invariant d without hardware division (device_types.hpp FastDiv)
@@ -292,6 +295,7 @@ private:
     // image k's LLGDev from its parameters, with its per-site temperatures (gradient) uploaded
     void apply_llg( size_t k );
     std::vector<std::unique_ptr<DeviceBuffer>> b_temp; // per image: T_i of a temperature gradient
+    DeviceBuffer b_stage; // the images' AoS spins on their way to or from the host mirrors
     int stt_gradient_any = 0;                          // some image's STT uses the spin derivative
     void tune_stage_tiles();
     int tuned_row_tile = -1; // row_tile_log2 picked by tune_stage_tiles (-1: the default)
