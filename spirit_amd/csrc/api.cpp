@@ -667,8 +667,10 @@ try
             }
         }
         runner->download_spins();
+        std::vector<double *> eff( n_states );
         for( int k = 0; k < n_states; ++k )
-            runner->download_effective_field( k, E[k].image->eff_field.data() );
+            eff[k] = E[k].image->eff_field.data();
+        runner->download_effective_fields( eff );
     }
     catch( ... )
     {

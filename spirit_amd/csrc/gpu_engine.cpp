@@ -1066,20 +1066,37 @@ void LLGRunner::download_spins()
 
 void LLGRunner::download_effective_field( int img, double * aos )
 {
-    const int nos        = specs[img].geometry->nos;
-    const long long base = 3 * imgs_host[img].off;
-    const double * src   = b_eff.as<double>() + base;
-    if( eff_raw )
+    std::vector<double *> dst( specs.size(), nullptr );
+    dst[img] = aos;
+    download_effective_fields( dst );
+}
+
+// The effective fields of the images with a destination (AoS host arrays; nullptr: skipped),
+// projected, transposed to AoS on the device and copied, with one synchronisation.
+void LLGRunner::download_effective_fields( const std::vector<double *> & aos )
+{
+    if( b_stage.size() < sizeof( double ) * 3 * (size_t)n_total )
+        b_stage.resize( sizeof( double ) * 3 * (size_t)n_total );
+    double * stage = b_stage.as<double>();
+    for( size_t k = 0; k < specs.size() && k < aos.size(); ++k )
     {
-        // P_perp(s) F1 with the spins of the last observed step (every run ends on one)
-        llg_launch_project_field( src, b_s.as<double>() + base, b_efft.as<double>() + base, nos, stream );
+        if( !aos[k] )
+            continue;
+        const int nos        = specs[k].geometry->nos;
+        const long long base = 3 * imgs_host[k].off;
+        const double * src   = b_eff.as<double>() + base;
+        if( eff_raw )
+        {
+            // P_perp(s) F1 with the spins of the last observed step (every run ends on one)
+            llg_launch_project_field( src, b_s.as<double>() + base, b_efft.as<double>() + base, nos, stream );
+            src = b_efft.as<double>() + base;
+        }
+        device_soa_to_aos( src, stage + base, nos, stream );
         HIPCHECK( hipGetLastError() );
-        src = b_efft.as<double>() + base;
+        HIPCHECK( hipMemcpyAsync( aos[k], stage + base, sizeof( double ) * 3 * (size_t)nos, hipMemcpyDeviceToHost,
+                                  stream ) );
     }
-    std::vector<double> soa( 3 * (size_t)nos );
     HIPCHECK( hipStreamSynchronize( stream ) );
-    HIPCHECK( hipMemcpy( soa.data(), src, sizeof( double ) * soa.size(), hipMemcpyDeviceToHost ) );
-    soa_to_aos( soa.data(), aos, nos );
 }
 
 void LLGRunner::set_injected_noise( const double * noise, long n_steps )

@@ -187,6 +187,8 @@ public:
         return obs_host[2 * img + 1];
     }
     void download_effective_field( int img, double * aos );
+    // the fields of several images at once (aos[k]: image k's AoS destination or nullptr)
+    void download_effective_fields( const std::vector<double *> & aos );
     // gradient (AoS) and energy of the current device spins of image img
     double gradient( int img, double * grad_aos );
     // per-spin energy contributions of image img's current spins, out[t * nos + i] for the
@@ -400,6 +402,8 @@ public:
     ChainData chain_data();
     void download_spins();                      // to the images' host mirrors
     void download_effective_field( int img, double * aos );
+    // the fields of several images at once (aos[k]: image k's AoS destination or nullptr)
+    void download_effective_fields( const std::vector<double *> & aos );
     void upload_spins();
     void set_types( const std::vector<int> & types );
     void set_parameters( const GNEBParameters & par );
