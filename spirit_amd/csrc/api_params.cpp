@@ -1256,21 +1256,19 @@ catch( ... )
     handle_exception( idx_image, idx_chain );
 }
 
-// Topological charge of a 2D lattice (one basis atom) as the sum of solid angles of the
-// triangles (a,b)-(a+1,b)-(a,b+1) and (a+1,b)-(a+1,b+1)-(a,b+1), Oosterom-Strackee formula
-// (Vectormath.cpp solid_angle_2), divided by 4 pi; periodic directions wrap.
-extern "C" float Quantity_Get_Topological_Charge( State * state, int idx_image, int idx_chain ) noexcept
-try
+// Topological charge density of a 2D lattice (one basis atom): per triangle
+// (a,b)-(a+1,b)-(a,b+1) and (a+1,b)-(a+1,b+1)-(a,b+1), its solid angle (Oosterom-Strackee,
+// Vectormath.cpp solid_angle_2) over 4 pi, oriented so that a counter-clockwise triangle in the
+// xy plane counts positive; periodic directions wrap.  Returns the number of triangles; the
+// densities and the triangles' spin indices (3 per triangle) go to the non-null outputs.
+static int topological_density( Image & image, std::vector<double> * density, std::vector<int> * triangles )
 {
-    API_IMAGE();
-    sync_image_host( state, idx_image );
-    gneb_sync_host( state );
-    auto & g = *image->geometry;
-    auto & h = *image->hamiltonian;
+    auto & g = *image.geometry;
+    auto & h = *image.hamiltonian;
     if( g.nb != 1 || g.n_cells[2] != 1 )
-        return 0.f;
+        return 0;
     const int Na = g.n_cells[0], Nb = g.n_cells[1];
-    auto S = [&]( int a, int b ) -> const double * { return &image->spins[3 * ( a + Na * b )]; };
+    auto S = [&]( int idx ) -> const double * { return &image.spins[3 * (size_t)idx]; };
     auto solid = []( const double * v1, const double * v2, const double * v3 ) {
         double c[3] = { v2[1] * v3[2] - v2[2] * v3[1], v2[2] * v3[0] - v2[0] * v3[2], v2[0] * v3[1] - v2[1] * v3[0] };
         double x    = v1[0] * c[0] + v1[1] * c[1] + v1[2] * c[2];
@@ -1279,19 +1277,65 @@ try
                    + ( v2[0] * v3[0] + v2[1] * v3[1] + v2[2] * v3[2] );
         return 2 * std::atan2( x, y );
     };
-    double q = 0;
-    int amax = h.bc[0] ? Na : Na - 1, bmax = h.bc[1] ? Nb : Nb - 1;
+    const int amax = h.bc[0] ? Na : Na - 1, bmax = h.bc[1] ? Nb : Nb - 1;
     // orientation: the triangle (a,b),(a+1,b),(a,b+1) is counter-clockwise when a x b points along +z
-    double cz   = g.bravais[0][0] * g.bravais[1][1] - g.bravais[0][1] * g.bravais[1][0];
-    double sign = cz >= 0 ? 1.0 : -1.0;
+    const double cz   = g.bravais[0][0] * g.bravais[1][1] - g.bravais[0][1] * g.bravais[1][0];
+    const double sign = cz >= 0 ? 1.0 : -1.0;
+    int n             = 0;
     for( int b = 0; b < bmax; ++b )
         for( int a = 0; a < amax; ++a )
         {
-            int a1 = ( a + 1 ) % Na, b1 = ( b + 1 ) % Nb;
-            q += solid( S( a, b ), S( a1, b ), S( a, b1 ) );
-            q += solid( S( a1, b ), S( a1, b1 ), S( a, b1 ) );
+            const int a1 = ( a + 1 ) % Na, b1 = ( b + 1 ) % Nb;
+            const int tri[2][3] = { { a + Na * b, a1 + Na * b, a + Na * b1 }, { a1 + Na * b, a1 + Na * b1, a + Na * b1 } };
+            for( auto & t : tri )
+            {
+                if( density )
+                    density->push_back( sign * solid( S( t[0] ), S( t[1] ), S( t[2] ) ) / ( 4 * Pi ) );
+                if( triangles )
+                    triangles->insert( triangles->end(), t, t + 3 );
+                ++n;
+            }
         }
-    return (float)( sign * q / ( 4 * Pi ) );
+    return n;
+}
+
+extern "C" float Quantity_Get_Topological_Charge( State * state, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    sync_image_host( state, idx_image );
+    gneb_sync_host( state );
+    std::vector<double> density;
+    topological_density( *image, &density, nullptr );
+    double q = 0;
+    for( double d : density )
+        q += d;
+    return (float)q;
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+    return 0;
+}
+
+// Quantities.h: the number of triangles; with non-null arrays, each triangle's charge and its
+// three spin indices
+extern "C" int Quantity_Get_Topological_Charge_Density( State * state, float * charge_density,
+                                                        int * triangle_indices, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    sync_image_host( state, idx_image );
+    gneb_sync_host( state );
+    std::vector<double> density;
+    std::vector<int> tri;
+    const int n = topological_density( *image, &density, &tri );
+    if( charge_density )
+        for( int k = 0; k < n; ++k )
+            charge_density[k] = (float)density[k];
+    if( triangle_indices )
+        std::copy( tri.begin(), tri.end(), triangle_indices );
+    return n;
 }
 catch( ... )
 {

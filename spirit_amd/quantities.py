@@ -32,3 +32,19 @@ def get_average_spin(p_state, idx_image=-1, idx_chain=-1):
 
 def get_topological_charge(p_state, idx_image=-1, idx_chain=-1):
     return float(_Get_Topological_Charge(ctypes.c_void_p(p_state), _i(idx_image), _i(idx_chain)))
+
+
+_Get_Topological_Charge_Density = _spirit.Quantity_Get_Topological_Charge_Density
+_Get_Topological_Charge_Density.argtypes = [_vp, ctypes.POINTER(_f), ctypes.POINTER(_i), _i, _i]
+_Get_Topological_Charge_Density.restype = _i
+
+
+def get_topological_charge_density(p_state, idx_image=-1, idx_chain=-1):
+    """(charge of each triangle, the triangles' three spin indices) of a 2D lattice with one basis
+    atom; empty lists otherwise."""
+    n = _Get_Topological_Charge_Density(ctypes.c_void_p(p_state), None, None, _i(idx_image), _i(idx_chain))
+    dens = (_f * max(n, 1))()
+    tri = (_i * max(3 * n, 1))()
+    _Get_Topological_Charge_Density(ctypes.c_void_p(p_state), dens, tri, _i(idx_image), _i(idx_chain))
+    return [float(dens[k]) for k in range(n)], [[int(tri[3 * k]), int(tri[3 * k + 1]), int(tri[3 * k + 2])]
+                                                for k in range(n)]

@@ -74,6 +74,14 @@ def get_energy_array_names(p_state, idx_image=-1, idx_chain=-1):
     return names.split("|") if names else []
 
 
+def get_energy_contributions(p_state, divide_by_nspins=True, idx_image=-1, idx_chain=-1):
+    """{name: energy} of the contributions of the last update_data, as the reference's
+    system.get_energy_contributions."""
+    names = get_energy_array_names(p_state, idx_image, idx_chain)
+    values = get_energy_array(p_state, divide_by_nspins, idx_image, idx_chain)
+    return dict(zip(names, values))
+
+
 def get_energy_array(p_state, divide_by_nspins=True, idx_image=-1, idx_chain=-1):
     """Energy contributions of the last update_data (System_Get_Energy_Array)."""
     n = _Get_Energy_Array(ctypes.c_void_p(p_state), None, _b(divide_by_nspins), _i(idx_image), _i(idx_chain))
