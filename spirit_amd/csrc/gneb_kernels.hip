@@ -26,6 +26,12 @@ __device__ __forceinline__ void partial_store( double * p, double v )
 {
     __hip_atomic_store( p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
 }
+// ... and read back with device-scope loads, which bypass the reading XCD's cache, so no
+// acquire fence (an invalidation of that XCD's whole L2) is needed before them
+__device__ __forceinline__ double partial_load( const double * p )
+{
+    return __hip_atomic_load( p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+}
 
 template<int NV>
 __device__ __forceinline__ void block_sums( double ( &v )[NV], double * out, const int ( &slot )[NV] )
@@ -600,7 +606,10 @@ __device__ __forceinline__ void image_sums( const GnebArgs & A, int img )
         const double * p = A.part + (long long)( img * A.bpi + b ) * GNEB_NPART;
 #pragma unroll
         for( int k = 0; k < PS::NS; ++k )
-            s[k] = PS::MAXRED ? fmax( s[k], p[PS::slot( k )] ) : s[k] + p[PS::slot( k )];
+        {
+            const double x = partial_load( p + PS::slot( k ) );
+            s[k]           = PS::MAXRED ? fmax( s[k], x ) : s[k] + x;
+        }
     }
 #pragma unroll
     for( int k = 0; k < PS::NS; ++k )
@@ -633,7 +642,7 @@ __device__ void chain_tail( const GnebArgs & A, int n_red_images )
     for( int img = threadIdx.x; img < n_red_images; img += blockDim.x )
 #pragma unroll
         for( int k = 0; k < PS::NS; ++k )
-            red[k][img] = A.isum[img * GNEB_ISUM + k];
+            red[k][img] = partial_load( A.isum + img * GNEB_ISUM + k );
     __syncthreads();
     chain_phase<PHASE>( A, staged ? ch_lds : A.ch, red );
     __syncthreads();
@@ -644,12 +653,13 @@ __device__ void chain_tail( const GnebArgs & A, int n_red_images )
 
 // Arrival of a block at the end of a pass, in two levels so that no counter takes more than
 // one image's worth of atomics: thread 0 waits until its device-scope partial stores are
-// performed and counts the block in its image; the image's last block drops its XCD's stale
-// cache lines, sums the image's partials and counts the image in the chain; the chain's last
-// image runs the chain phase.  Counters are rearmed by the blocks that completed them.  A
-// device-wide release fence in every block instead would write back each XCD's whole L2 once
-// per block.  Every thread of the block calls it, after the block's partial slots were stored
-// by thread 0 (partial_store).
+// performed and counts the block in its image; the image's last block sums the image's
+// partials and counts the image in the chain; the chain's last image runs the chain phase.
+// Counters are rearmed by the blocks that completed them.  A device-wide release fence in
+// every block instead would write back each XCD's whole L2 once per block; the partials and
+// image sums are read with device-scope loads (partial_load), so no acquire fence invalidates
+// the reading block's L2 either.  Every thread of the block calls it, after the block's
+// partial slots were stored by thread 0 (partial_store).
 template<int PHASE>
 __device__ __forceinline__ void pass_tail( const GnebArgs & A )
 {
@@ -666,7 +676,6 @@ __device__ __forceinline__ void pass_tail( const GnebArgs & A )
     __syncthreads();
     if( !last )
         return;
-    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
     image_sums<PHASE>( A, img );
     __syncthreads();
     if( threadIdx.x == 0 )
@@ -680,7 +689,6 @@ __device__ __forceinline__ void pass_tail( const GnebArgs & A )
     __syncthreads();
     if( !last )
         return;
-    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "agent" );
     chain_tail<PHASE>( A, n_images );
     if( threadIdx.x == 0 )
         __hip_atomic_store( A.count, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
