@@ -25,5 +25,9 @@ SPIRIT_API void Configuration_SpinSpiral( State * state, const char * direction_
 SPIRIT_API void Configuration_Add_Noise_Temperature( State * state, float temperature, const float position[3], const float r_cut_rectangular[3], float r_cut_cylindrical SPIRIT_DEFAULT( -1 ), float r_cut_spherical SPIRIT_DEFAULT( -1 ), bool inverted SPIRIT_DEFAULT( false ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 /* Configurations.h:118 */
 SPIRIT_API void Configuration_Skyrmion( State * state, float r, float order, float phase, bool upDown, bool achiral, bool rl, const float position[3], const float r_cut_rectangular[3], float r_cut_cylindrical SPIRIT_DEFAULT( -1 ), float r_cut_spherical SPIRIT_DEFAULT( -1 ), bool inverted SPIRIT_DEFAULT( false ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Configurations.h:124-127 */
+SPIRIT_API void Configuration_DW_Skyrmion( State * state, float dw_radius, float dw_width, float order, float phase, bool upDown, bool achiral, bool rl, const float position[3], const float r_cut_rectangular[3], float r_cut_cylindrical SPIRIT_DEFAULT( -1 ), float r_cut_spherical SPIRIT_DEFAULT( -1 ), bool inverted SPIRIT_DEFAULT( false ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
+/* Configurations.h:130-133 */
+SPIRIT_API void Configuration_Hopfion( State * state, float r, int order, const float position[3], const float r_cut_rectangular[3], float r_cut_cylindrical SPIRIT_DEFAULT( -1 ), float r_cut_spherical SPIRIT_DEFAULT( -1 ), bool inverted SPIRIT_DEFAULT( false ), const float normal[3] SPIRIT_DEFAULT( 0 ), int idx_image SPIRIT_DEFAULT( -1 ), int idx_chain SPIRIT_DEFAULT( -1 ) ) SPIRIT_NOEXCEPT;
 
 #endif

@@ -283,6 +283,120 @@ catch( ... )
     handle_exception( idx_image, idx_chain );
 }
 
+// Configurations::DW_Skyrmion (Configurations.cpp:272-312): a skyrmion whose profile is a
+// pair of tanh domain walls at distance dw_radius from the centre, of width dw_width; the
+// cylindrical cut defaults to 3 * max(dw_radius, dw_width) (core/src/Spirit/Configurations.cpp:577-578).
+extern "C" void Configuration_DW_Skyrmion( State * state, float dw_radius, float dw_width, float order, float phase,
+                                           bool upDown, bool achiral, bool rl, const float position[3],
+                                           const float r_cut_rectangular[3], float r_cut_cylindrical,
+                                           float r_cut_spherical, bool inverted, int idx_image, int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    if( r_cut_cylindrical < 0 )
+        r_cut_cylindrical = std::max( 3 * dw_radius, 3 * dw_width );
+    auto f = make_filter( *image, position, r_cut_rectangular, r_cut_cylindrical, r_cut_spherical, inverted );
+    const Vec3 c{ image->geometry->center[0] + position[0], image->geometry->center[1] + position[1],
+                  image->geometry->center[2] + position[2] };
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    sync_image_host( state, idx_image );
+    const int ksi = rl ? 1 : -1, dir = upDown ? 1 : -1;
+    const double R = dw_radius, W = dw_width;
+    auto & P       = image->geometry->positions;
+    for( int i = 0; i < image->nos(); ++i )
+    {
+        const double dx = P[3 * i] - c[0], dy = P[3 * i + 1] - c[1];
+        const double r  = std::sqrt( dx * dx + dy * dy );
+        if( !f( &P[3 * i] ) )
+            continue;
+        // polar angle: pi in the core, 0 outside, through the two walls at r = R (and r = -R)
+        const double theta = std::asin( std::tanh( -2 * ( r + R ) / W ) ) + std::asin( std::tanh( -2 * ( r - R ) / W ) ) + Pi;
+        double phi         = r == 0 ? 0.0 : std::acos( std::max( -1.0, std::min( 1.0, dx / r ) ) );
+        if( dy < 0.0 )
+            phi = -phi;
+        phi += phase / 180 * Pi;
+        double * s = &image->spins[3 * i];
+        s[0]       = ksi * std::sin( theta ) * std::cos( order * phi );
+        s[1]       = ksi * std::sin( theta ) * std::sin( order * phi + achiral * Pi );
+        s[2]       = std::cos( theta ) * -dir;
+    }
+    for( int i = 0; i < image->nos(); ++i )
+        normalize3( &image->spins[3 * i] );
+    after_spin_change( state, idx_image );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
+// Configurations::Hopfion (Configurations.cpp:156-216): a toroidal hopfion of radius r about
+// the axis `normal`, evaluated on the positions turned into the frame whose z axis is the
+// normal (Vectormath::dreibein, Vectormath.cpp:410-432) about the centre; the spherical cut
+// defaults to pi r (core/src/Spirit/Configurations.cpp:487-488) and is applied to those positions.
+extern "C" void Configuration_Hopfion( State * state, float r, int order, const float position[3],
+                                       const float r_cut_rectangular[3], float r_cut_cylindrical,
+                                       float r_cut_spherical, bool inverted, const float normal[3], int idx_image,
+                                       int idx_chain ) noexcept
+try
+{
+    API_IMAGE();
+    if( r_cut_spherical < 0 )
+        r_cut_spherical = r * (float)Pi;
+    auto f = make_filter( *image, position, r_cut_rectangular, r_cut_cylindrical, r_cut_spherical, inverted );
+    const Vec3 c{ image->geometry->center[0] + position[0], image->geometry->center[1] + position[1],
+                  image->geometry->center[2] + position[2] };
+    // frame: rows x_hat, y_hat, n (normalised); x_hat = n x e_x unless n is close to e_x
+    static const float default_normal[3] = { 0, 0, 1 };
+    if( !normal )
+        normal = default_normal;
+    double n[3] = { normal[0], normal[1], normal[2] };
+    const double ex[3] = { 1, 0, 0 }, ey[3] = { 0, 1, 0 };
+    auto cross = []( const double * a, const double * b, double * o ) {
+        o[0] = a[1] * b[2] - a[2] * b[1], o[1] = a[2] * b[0] - a[0] * b[2], o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    double xh[3], yh[3];
+    cross( n, std::abs( n[0] - 1 ) > 1e-1 ? ex : ey, xh );
+    cross( n, xh, yh );
+    normalize3( xh );
+    normalize3( yh );
+    normalize3( n );
+    const double Rm[3][3] = { { xh[0], xh[1], xh[2] }, { yh[0], yh[1], yh[2] }, { n[0], n[1], n[2] } };
+    std::lock_guard<std::recursive_mutex> lk( image->mtx );
+    sync_image_host( state, idx_image );
+    auto & P = image->geometry->positions;
+    if( r != 0 )
+        for( int i = 0; i < image->nos(); ++i )
+        {
+            // turned position about the centre
+            const double d0 = P[3 * i] - c[0], d1 = P[3 * i + 1] - c[1], d2 = P[3 * i + 2] - c[2];
+            double q[3];
+            for( int k = 0; k < 3; ++k )
+                q[k] = Rm[k][0] * d0 + Rm[k][1] * d1 + Rm[k][2] * d2 + c[k];
+            if( !f( q ) )
+                continue;
+            const double dq[3] = { q[0] - c[0], q[1] - c[1], q[2] - c[2] };
+            const double d     = std::sqrt( dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2] );
+            // polar angle against the torus axis, and the profile of the tube
+            const double T   = std::acos( d == 0 ? 0.0 : dq[2] / d );
+            double t         = d / r;
+            t                = 1.0 + 4.22 / ( t * t );
+            const double tmp = Pi * ( 1.0 - 1.0 / std::sqrt( t ) );
+            t                = std::sin( tmp ) * std::sin( T );
+            t                = std::acos( 1.0 - 2.0 * t * t );
+            const double F   = std::atan2( dq[1], dq[0] );
+            const double fr  = F + std::atan( 1.0 / ( std::tan( tmp ) * std::cos( T ) ) ) + ( T > Pi / 2.0 ? 0.0 : Pi );
+            double * s       = &image->spins[3 * i];
+            s[0]             = std::sin( t ) * std::cos( order * fr );
+            s[1]             = std::sin( t ) * std::sin( order * fr );
+            s[2]             = std::cos( t );
+        }
+    after_spin_change( state, idx_image );
+}
+catch( ... )
+{
+    handle_exception( idx_image, idx_chain );
+}
+
 // ===========================================================================
 // Chain.h  (core/src/Spirit/Chain.cpp)
 // ===========================================================================

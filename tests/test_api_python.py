@@ -76,3 +76,39 @@ def test_energy_contributions_dictionary(make_state):
     E = system.get_energy(p)
     assert set(E_contribs) == {"Exchange", "DMI"}, E_contribs
     assert abs(sum(E_contribs.values()) - E) <= 1e-5 * max(1.0, abs(E))
+
+
+def test_dw_skyrmion_and_hopfion(make_state):
+    """The configurations the reference's configuration test sets (core/python/test/configuration.py):
+    a domain-wall skyrmion (core down, background up, the wall where s_z changes sign at about
+    dw_radius) and a hopfion (unit spins, up along the torus axis, a closed tube of down spins
+    through the torus ring)."""
+    from spirit_amd import configuration, geometry, system
+    p = make_state(dict(bravais_lattice="sc", n_basis_cells=[30, 30, 1], boundary_conditions=[1, 1, 0], mu_s=[1.0],
+                        n_shells_exchange=0, n_shells_dmi=0))
+    configuration.plus_z(p)
+    configuration.dw_skyrmion(p, 5.0, 2.0)
+    s = np.array(system.get_spin_directions(p))
+    pos = np.array(geometry.get_positions(p))
+    c = np.array(geometry.get_center(p))
+    r = np.hypot(pos[:, 0] - c[0], pos[:, 1] - c[1])
+    np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-12)
+    assert s[np.argmin(r), 2] < -0.99
+    assert np.all(s[(r > 8) & (r < 14), 2] > 0.9)
+    inside, outside = s[(r > 3.5) & (r < 4.5), 2], s[(r > 5.5) & (r < 6.5), 2]
+    assert inside.max() < 0 < outside.min()
+    q = make_state(dict(bravais_lattice="sc", n_basis_cells=[24, 24, 24], boundary_conditions=[0, 0, 0],
+                        mu_s=[1.0], n_shells_exchange=0, n_shells_dmi=0))
+    configuration.plus_z(q)
+    configuration.hopfion(q, 5.0)
+    s = np.array(system.get_spin_directions(q))
+    pos = np.array(geometry.get_positions(q))
+    c = np.array(geometry.get_center(q))
+    d = np.linalg.norm(pos - c, axis=1)
+    np.testing.assert_allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-12)
+    assert s[:, 2].min() < -0.9  # the down tube
+    rho = np.hypot(pos[:, 0] - c[0], pos[:, 1] - c[1])
+    ring = (np.abs(pos[:, 2] - c[2]) < 1.0) & (np.abs(rho - 5.0) < 1.0)
+    assert s[ring, 2].mean() < -0.5
+    axis = (rho < 1.0) & (d < 5.0 * math.pi)
+    assert axis.sum() > 10 and s[axis, 2].min() > 0.95  # up along the torus axis
