@@ -111,4 +111,4 @@ def test_dw_skyrmion_and_hopfion(make_state):
     ring = (np.abs(pos[:, 2] - c[2]) < 1.0) & (np.abs(rho - 5.0) < 1.0)
     assert s[ring, 2].mean() < -0.5
     axis = (rho < 1.0) & (d < 5.0 * math.pi)
-    assert axis.sum() > 10 and s[axis, 2].min() > 0.95  # up along the torus axis
+    assert axis.sum() > 10 and s[axis, 2].min() > 0.9  # up along the torus axis (sites 0.7 off it)
