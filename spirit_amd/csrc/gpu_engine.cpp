@@ -1167,7 +1167,7 @@ static StepArgs with_stt( StepArgs A, int stt_gradient )
               stt_gradient_any )
 
 // The stage kernels' row tile (rows along b swept through the c planes) by timing: the first
-// solver stage at tiles of 8, 16 and 32 rows, where every image's rows hold whole blocks and
+// solver stage at tiles of 8, 16 and 32 rows (64 measured 5-10% slower), where every image's rows hold whole blocks and
 // Nb is a multiple of the tile, three launches each after a warm-up; the fastest is kept for
 // the run (lattices below 2^20 spins keep the default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the
 // tile instead.
@@ -1213,6 +1213,9 @@ void LLGRunner::tune_stage_tiles()
             HIPCHECK( hipEventElapsedTime( &ms, e0, e1 ) );
             t = std::min( t, (double)ms );
         }
+        char msg[80];
+        std::snprintf( msg, sizeof( msg ), "stage row tile %d rows: %.4f ms", 1 << tl, t );
+        log_message( 6, msg );
         if( t < best )
         {
             best = t;
