@@ -1167,9 +1167,9 @@ static StepArgs with_stt( StepArgs A, int stt_gradient )
               stt_gradient_any )
 
 // The stage kernels' row tile (rows along b swept through the c planes) by timing: the first
-// solver stage at tiles of 8, 16 and 32 rows (64 measured 5-10% slower), where every image's rows hold whole blocks and
-// Nb is a multiple of the tile, three launches each after a warm-up; the fastest is kept for
-// the run (lattices below 2^20 spins keep the default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the
+// solver stage at tiles of 8, 16 and 32 rows (64 measured 5-8% slower than 32), where every image's
+// rows hold whole blocks and Nb is a multiple of the tile; the tile with the lowest median of
+// seven timed launches is kept for the run (lattices below 2^20 spins keep the default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the
 // tile instead.
 void LLGRunner::tune_stage_tiles()
 {
@@ -1194,32 +1194,38 @@ void LLGRunner::tune_stage_tiles()
     hipEvent_t e0, e1;
     HIPCHECK( hipEventCreate( &e0 ) );
     HIPCHECK( hipEventCreate( &e1 ) );
-    double best = 1e30;
-    int pick    = cand[0];
-    for( int tl : cand )
-    {
-        for( auto & I : imgs_host )
-            I.ham.row_tile_log2 = tl;
-        upload_images();
-        llg_launch_stage( solver, 0, A, n_blocks, stream );
-        double t = 1e30;
-        for( int rep = 0; rep < 3; ++rep )
+    // rounds over the candidates in turn (a warm-up launch after each switch, then a timed
+    // one), so that drifts of the clock hit every candidate alike; the median of each decides
+    constexpr int ROUNDS = 7;
+    std::vector<std::vector<double>> times( cand.size() );
+    for( int round = 0; round < ROUNDS; ++round )
+        for( size_t c = 0; c < cand.size(); ++c )
         {
+            for( auto & I : imgs_host )
+                I.ham.row_tile_log2 = cand[c];
+            upload_images();
+            llg_launch_stage( solver, 0, A, n_blocks, stream );
             HIPCHECK( hipEventRecord( e0, stream ) );
             llg_launch_stage( solver, 0, A, n_blocks, stream );
             HIPCHECK( hipEventRecord( e1, stream ) );
             HIPCHECK( hipEventSynchronize( e1 ) );
             float ms = 0;
             HIPCHECK( hipEventElapsedTime( &ms, e0, e1 ) );
-            t = std::min( t, (double)ms );
+            times[c].push_back( ms );
         }
+    double best = 1e30;
+    int pick    = cand[0];
+    for( size_t c = 0; c < cand.size(); ++c )
+    {
+        std::sort( times[c].begin(), times[c].end() );
+        const double t = times[c][ROUNDS / 2];
         char msg[80];
-        std::snprintf( msg, sizeof( msg ), "stage row tile %d rows: %.4f ms", 1 << tl, t );
+        std::snprintf( msg, sizeof( msg ), "stage row tile %d rows: %.4f ms (median)", 1 << cand[c], t );
         log_message( 6, msg );
         if( t < best )
         {
             best = t;
-            pick = tl;
+            pick = cand[c];
         }
     }
     (void)hipEventDestroy( e0 );
