@@ -1167,10 +1167,10 @@ static StepArgs with_stt( StepArgs A, int stt_gradient )
               stt_gradient_any )
 
 // The stage kernels' row tile (rows along b swept through the c planes) by timing: the first
-// solver stage at tiles of 8, 16 and 32 rows (64 measured 5-8% slower than 32), where every image's
-// rows hold whole blocks and Nb is a multiple of the tile; the tile with the lowest median of
-// seven timed launches is kept for the run (lattices below 2^20 spins keep the default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the
-// tile instead.
+// solver stage at tiles of 8, 16 and 32 rows (64 measured 5-8% slower than 32), where every
+// image's rows hold whole blocks and Nb is a multiple of the tile; the tile with the lowest
+// median of seven timed launches is kept for the run (lattices below 2^20 spins keep the
+// default).  SPIRIT_HIP_STAGE_ROW_TILE fixes the tile instead.
 void LLGRunner::tune_stage_tiles()
 {
     if( std::getenv( "SPIRIT_HIP_STAGE_ROW_TILE" ) || n_total < ( 1LL << 20 ) )
