@@ -274,3 +274,49 @@ def test_batch_with_stt_and_gradient_replicas(make_state, oracle):
         err = float(np.abs(spins_of(ps[k]) - r).max())
         print("  replica %d: max |s - s_oracle| = %.3e" % (k, err), flush=True)
         assert err <= 1e-9
+
+
+def test_stt_gradient_full_size_cube(make_state, oracle):
+    """The gradient torque at the full 256x256x128 size (without the dipolar field, whose
+    oracle run would take minutes): the stage kernels' STT instantiation over 32-row tiles of
+    8.4M spins, 3 Depondt steps against the oracle."""
+    cfg = dict(configs.CUBE_256, ddi_method="none", llg_beta=0.1, llg_stt_magnitude=0.5, llg_stt_use_gradient=1,
+               llg_stt_polarisation_normal=[0.6, 0.8, 0.0])
+    print_case("stt gradient 256x256x128")
+    p = make_state(cfg)
+    S = oracle.System(cfg)
+    s0 = oracle.random_spins(S.nos, 46)
+    set_spins(p, s0)
+    s, _, _ = run_n_shot(p, SOLVERS["depondt"], 3)
+    s_ref, _, _, _ = S.llg(s0, SOLVERS["depondt"], 3, dt=cfg["llg_dt"], damping=cfg["llg_damping"], beta=0.1,
+                           stt=(0.5, [0.6, 0.8, 0.0], True))
+    err = float(np.abs(s - s_ref).max())
+    print("  max |s - s_oracle| = %.3e" % err, flush=True)
+    assert err <= 1e-9
+
+
+def test_temperature_gradient_rk4_with_stt(make_state, oracle):
+    """RK4 (the thermal field drawn once per step and read by the later stages) with a
+    temperature gradient and the monolayer torque together, injected noise, 20 steps."""
+    from spirit_amd import hip, simulation
+    cfg = film([12, 10, 2], [1, 1, 0], llg_temperature=4.0, llg_temperature_gradient_inclination=1.5,
+               llg_temperature_gradient_direction=[1.0, 0.0, 0.0], **STT_MONO)
+    print_case("temperature gradient + stt, rk4")
+    p = make_state(cfg)
+    S = oracle.System(cfg)
+    s0 = oracle.random_spins(S.nos, 47)
+    set_spins(p, s0)
+    n = 20
+    noise = np.random.default_rng(79).normal(size=(n, S.nos, 3))
+    simulation.start(p, simulation.METHOD_LLG, SOLVERS["rk4"], single_shot=True)
+    hip.set_thermal_noise(p, noise)
+    simulation.n_shot(p, n)
+    s = spins_of(p)
+    simulation.stop(p)
+    s_ref, _, _, _ = S.llg(s0, SOLVERS["rk4"], n, dt=cfg["llg_dt"], damping=cfg["llg_damping"], temperature=4.0,
+                           noise=noise, beta=STT_MONO["llg_beta"],
+                           stt=(STT_MONO["llg_stt_magnitude"], STT_MONO["llg_stt_polarisation_normal"], False),
+                           temperature_gradient=(1.5, [1.0, 0.0, 0.0]))
+    err = float(np.abs(s - s_ref).max())
+    print("  max |s - s_oracle| = %.3e" % err, flush=True)
+    assert err <= 1e-9
