@@ -31,6 +31,17 @@ struct DdiDev
     int lds_max; // dynamic LDS bytes a block may use on this device (host-side launch choice)
     int c_kind;  // pass C kernel (DdiPassC), chosen by timing when the plan is built
     int c_tk;    // column tile of DDI_C_P2_LDS
+    int rev;     // DDI_REV_* bits: passes that visit their blocks in reverse order
+};
+
+// Passes whose blocks run in reverse order, so that they first read what the pass before
+// them wrote last (still held in the memory-side cache).
+enum DdiReverse
+{
+    DDI_REV_A_FWD = 1,
+    DDI_REV_B_FWD = 2,
+    DDI_REV_B_INV = 4,
+    DDI_REV_A_INV = 8,
 };
 
 enum DdiPassC

@@ -483,7 +483,8 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_fwd( DdiDev d, const double * __
     const int t             = threadIdx.x % T;
     const int npairs        = ( d.Nb + 1 ) / 2;
     const long long n_lines = 3LL * d.nb * npairs * d.Nc;
-    const long long line    = (long long)blockIdx.x * LPB + l;
+    const long long blk     = ( d.rev & DDI_REV_A_FWD ) ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const long long line    = blk * LPB + l;
     const bool act          = line < n_lines;
     int bp = 0, c = 0, arr = 0;
     if( act )
@@ -571,7 +572,8 @@ __global__ void __launch_bounds__( 512 ) k_p2_a_inv( DdiDev d, double * __restri
     const int t             = threadIdx.x % T;
     const int npairs        = ( d.Nb + 1 ) / 2;
     const long long n_lines = 3LL * d.nb * npairs * d.Nc;
-    const long long line    = (long long)blockIdx.x * LPB + l;
+    const long long blk     = ( d.rev & DDI_REV_A_INV ) ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const long long line    = blk * LPB + l;
     const bool act          = line < n_lines;
     int bp = 0, c = 0, arr = 0;
     if( act )
@@ -687,7 +689,7 @@ __global__ void __launch_bounds__( 512 ) k_p2_b( DdiDev d, double2 * __restrict_
     extern __shared__ double2 lds[];
     constexpr int T = P2<N>::T, CW = P2Col<N>::CWB, LS = P2<N>::LS;
     const int n_tiles = ( d.Ha + CW - 1 ) / CW;
-    long long bid     = blockIdx.x;
+    long long bid     = ( d.rev & ( INV ? DDI_REV_B_INV : DDI_REV_B_FWD ) ) ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
     const int tile    = (int)( bid % n_tiles );
     bid /= n_tiles;
     const int c    = (int)( bid % c_ext );

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <cstdlib>
 #include <string>
+#include <atomic>
+#include <thread>
 
 namespace spirit_hip
 {
@@ -238,6 +240,13 @@ void DdiPlan::build( const Geometry & g, const Hamiltonian & h, hipStream_t st )
         HIPCHECK( hipDeviceGetAttribute( &lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id ) );
         d.lds_max = lds;
     }
+    // Pass B reads what pass A wrote and pass A^-1 what pass B^-1 wrote, in the same
+    // order: run B and A^-1 backwards so that they start on the rows written last, which
+    // the 256 MB memory-side cache still holds (3.20 -> 3.15 ms per Depondt step at
+    // 256x256x128).  SPIRIT_HIP_DDI_REVERSE = sum of DdiReverse bits overrides it (tuning).
+    d.rev = DDI_REV_B_FWD | DDI_REV_A_INV;
+    if( const char * r = std::getenv( "SPIRIT_HIP_DDI_REVERSE" ) )
+        d.rev = std::atoi( r ) & 15;
     d.La  = La.dev;
     d.Lb  = Lb.dev;
     d.Lc  = Lc.dev;
@@ -1025,15 +1034,100 @@ void LLGRunner::refresh_hamiltonian()
     upload_images();
 }
 
+// Transfers of whole batches go through page-locked host memory: the runtime copies
+// to and from pageable memory in pieces through its own staging (about 5 GB/s here), and
+// page-locked memory moves at the link's rate.  One buffer per process, grown on demand
+// and kept to the end (page-locking costs about as much as one pageable transfer of the
+// same size); a transfer holds its lock from the first copy into it to the last copy out.
+// Below PINNED_MIN_BYTES the images' mirrors are copied directly.
+static constexpr size_t PINNED_MIN_BYTES = size_t( 8 ) << 20;
+static std::mutex pinned_mutex;
+static double * pinned_buf    = nullptr;
+static size_t pinned_capacity = 0; // bytes
+
+static double * pinned_reserve( size_t bytes )
+{
+    if( bytes > pinned_capacity )
+    {
+        if( pinned_buf )
+            HIPCHECK( hipHostFree( pinned_buf ) );
+        pinned_buf      = nullptr;
+        pinned_capacity = 0;
+        HIPCHECK( hipHostMalloc( (void **)&pinned_buf, bytes, hipHostMallocDefault ) );
+        pinned_capacity = bytes;
+    }
+    return pinned_buf;
+}
+
+struct HostCopy
+{
+    void * dst;
+    const void * src;
+    size_t bytes;
+};
+
+// Host copies on up to 8 threads, in pieces of at most 4 MiB taken in turn (one thread
+// copies about 10 GB/s; the batch's 256 mirrors would otherwise cost more than the link).
+static void parallel_host_copies( const std::vector<HostCopy> & copies )
+{
+    constexpr size_t PIECE = size_t( 4 ) << 20;
+    std::vector<HostCopy> pieces;
+    size_t total = 0;
+    for( const auto & c : copies )
+    {
+        total += c.bytes;
+        for( size_t o = 0; o < c.bytes; o += PIECE )
+            pieces.push_back( { (char *)c.dst + o, (const char *)c.src + o, std::min( PIECE, c.bytes - o ) } );
+    }
+    const unsigned hw = std::max( 1u, std::thread::hardware_concurrency() );
+    const int n_thr   = (int)std::min<size_t>( { (size_t)8, (size_t)hw, pieces.size() } );
+    std::atomic<size_t> next( 0 );
+    auto work = [&]() {
+        for( size_t k = next++; k < pieces.size(); k = next++ )
+            std::memcpy( pieces[k].dst, pieces[k].src, pieces[k].bytes );
+    };
+    if( n_thr <= 1 || total < PINNED_MIN_BYTES )
+    {
+        work();
+        return;
+    }
+    std::vector<std::thread> pool;
+    for( int t = 1; t < n_thr; ++t )
+        pool.emplace_back( work );
+    work();
+    for( auto & t : pool )
+        t.join();
+}
+
 // The host mirrors are AoS, the device fields SoA: each image's mirror is copied as it is into
 // a device staging buffer and transposed there (a host transpose of the whole batch costs
 // more than the copies).
 void LLGRunner::upload_spins()
 {
-    if( b_stage.size() < sizeof( double ) * 3 * (size_t)n_total )
-        b_stage.resize( sizeof( double ) * 3 * (size_t)n_total );
+    const size_t bytes = sizeof( double ) * 3 * (size_t)n_total;
+    if( b_stage.size() < bytes )
+        b_stage.resize( bytes );
     double * stage = b_stage.as<double>();
     HIPCHECK( hipStreamSynchronize( stream ) ); // no kernel of an earlier call still reads the stage
+    if( bytes >= PINNED_MIN_BYTES )
+    {
+        std::lock_guard<std::mutex> guard( pinned_mutex );
+        double * pin = pinned_reserve( bytes );
+        std::vector<HostCopy> copies;
+        for( size_t k = 0; k < specs.size(); ++k )
+            copies.push_back( { pin + 3 * imgs_host[k].off, specs[k].spins_aos,
+                                sizeof( double ) * 3 * (size_t)specs[k].geometry->nos } );
+        parallel_host_copies( copies );
+        HIPCHECK( hipMemcpyAsync( stage, pin, bytes, hipMemcpyHostToDevice, stream ) );
+        for( size_t k = 0; k < specs.size(); ++k )
+        {
+            const long long base = 3 * imgs_host[k].off;
+            device_aos_to_soa( stage + base, b_s.as<double>() + base, specs[k].geometry->nos, stream );
+        }
+        HIPCHECK( hipGetLastError() );
+        HIPCHECK( hipStreamSynchronize( stream ) );
+        return;
+    }
     for( size_t k = 0; k < specs.size(); ++k )
     {
         const long long base = 3 * imgs_host[k].off;
@@ -1046,22 +1140,64 @@ void LLGRunner::upload_spins()
     HIPCHECK( hipStreamSynchronize( stream ) );
 }
 
+// Copies the AoS device stage of the images with a destination (nullptr: skipped) to their
+// host arrays, with one synchronisation.  Runs of consecutive images move in one copy.
+void LLGRunner::stage_to_host( const std::vector<double *> & dst )
+{
+    const double * stage = b_stage.as<double>();
+    size_t bytes         = 0;
+    for( size_t k = 0; k < specs.size() && k < dst.size(); ++k )
+        if( dst[k] )
+            bytes += sizeof( double ) * 3 * (size_t)specs[k].geometry->nos;
+    if( bytes < PINNED_MIN_BYTES )
+    {
+        for( size_t k = 0; k < specs.size() && k < dst.size(); ++k )
+            if( dst[k] )
+                HIPCHECK( hipMemcpyAsync( dst[k], stage + 3 * imgs_host[k].off,
+                                          sizeof( double ) * 3 * (size_t)specs[k].geometry->nos,
+                                          hipMemcpyDeviceToHost, stream ) );
+        HIPCHECK( hipStreamSynchronize( stream ) );
+        return;
+    }
+    std::lock_guard<std::mutex> guard( pinned_mutex );
+    double * pin = pinned_reserve( sizeof( double ) * 3 * (size_t)n_total );
+    std::vector<HostCopy> copies;
+    for( size_t k = 0; k < specs.size() && k < dst.size(); )
+    {
+        if( !dst[k] )
+        {
+            ++k;
+            continue;
+        }
+        size_t e = k;
+        while( e < specs.size() && e < dst.size() && dst[e] )
+            ++e;
+        const long long b0 = 3 * imgs_host[k].off;
+        const long long b1 = 3 * ( imgs_host[e - 1].off + specs[e - 1].geometry->nos );
+        HIPCHECK( hipMemcpyAsync( pin + b0, stage + b0, sizeof( double ) * (size_t)( b1 - b0 ), hipMemcpyDeviceToHost,
+                                  stream ) );
+        for( ; k < e; ++k )
+            copies.push_back( { dst[k], pin + 3 * imgs_host[k].off,
+                                sizeof( double ) * 3 * (size_t)specs[k].geometry->nos } );
+    }
+    HIPCHECK( hipStreamSynchronize( stream ) );
+    parallel_host_copies( copies );
+}
+
 void LLGRunner::download_spins()
 {
     if( b_stage.size() < sizeof( double ) * 3 * (size_t)n_total )
         b_stage.resize( sizeof( double ) * 3 * (size_t)n_total );
     double * stage = b_stage.as<double>();
+    std::vector<double *> dst( specs.size() );
     for( size_t k = 0; k < specs.size(); ++k )
     {
         const long long base = 3 * imgs_host[k].off;
         device_soa_to_aos( b_s.as<double>() + base, stage + base, specs[k].geometry->nos, stream );
+        dst[k] = specs[k].spins_aos;
     }
     HIPCHECK( hipGetLastError() );
-    for( size_t k = 0; k < specs.size(); ++k )
-        HIPCHECK( hipMemcpyAsync( specs[k].spins_aos, stage + 3 * imgs_host[k].off,
-                                  sizeof( double ) * 3 * (size_t)specs[k].geometry->nos, hipMemcpyDeviceToHost,
-                                  stream ) );
-    HIPCHECK( hipStreamSynchronize( stream ) );
+    stage_to_host( dst );
 }
 
 void LLGRunner::download_effective_field( int img, double * aos )
@@ -1092,11 +1228,9 @@ void LLGRunner::download_effective_fields( const std::vector<double *> & aos )
             src = b_efft.as<double>() + base;
         }
         device_soa_to_aos( src, stage + base, nos, stream );
-        HIPCHECK( hipGetLastError() );
-        HIPCHECK( hipMemcpyAsync( aos[k], stage + base, sizeof( double ) * 3 * (size_t)nos, hipMemcpyDeviceToHost,
-                                  stream ) );
     }
-    HIPCHECK( hipStreamSynchronize( stream ) );
+    HIPCHECK( hipGetLastError() );
+    stage_to_host( aos );
 }
 
 void LLGRunner::set_injected_noise( const double * noise, long n_steps )

@@ -259,6 +259,8 @@ public:
                     double * fv2 );
 
 private:
+    // the AoS device stage of the images with a host destination (nullptr: skipped) to those arrays
+    void stage_to_host( const std::vector<double *> & dst );
     hipEvent_t next_event();
     void collect_profile();
     bool profiling = false;
