@@ -164,8 +164,10 @@ __device__ __forceinline__ void normal3( uint64_t seed, uint64_t step, uint64_t 
     double u4 = u53( c1[2], c1[3] );
     double r1 = sqrt( -2.0 * log( u1 ) );
     double r2 = sqrt( -2.0 * log( u3 ) );
-    z[0]      = r1 * cos( 2.0 * Pi * u2 );
-    z[1]      = r1 * sin( 2.0 * Pi * u2 );
+    double sn, cs;
+    sincos( 2.0 * Pi * u2, &sn, &cs ); // one argument reduction for both
+    z[0] = r1 * cs;
+    z[1] = r1 * sn;
     z[2]      = r2 * cos( 2.0 * Pi * u4 );
 }
 
