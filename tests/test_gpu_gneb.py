@@ -251,3 +251,17 @@ def test_gneb_hexagonal_climbing_vp(state, oracle):
     types[4] = 1
     _, cg, cr, Eg, Rg, data = run_both(state, oracle, cfg, SOLVERS["vp"], 40, noi=noi, types=types)
     check(cg, cr, Eg, Rg, data)
+
+
+def test_gneb_full_size_hexagonal_chain(state, oracle):
+    """The GNEB workload at its full size: a 128x128 hexagonal monolayer (J=10, Neel DMI 6,
+    K=0.4, B=3 T), 32 images from a Neel skyrmion to the ferromagnet, climbing image in the
+    middle, VP, 20 iterations against the oracle (VP amplifies rounding differences to about
+    1e-12 over 20 iterations)."""
+    print_case("gneb", "hex 128x128 32 images climbing vp")
+    cfg = dict(configs.HEX_128, gneb_force_convergence=1e-14)
+    noi = 32
+    types = [0] * noi
+    types[16] = 1
+    _, cg, cr, Eg, Rg, data = run_both(state, oracle, cfg, SOLVERS["vp"], 20, noi=noi, types=types)
+    check(cg, cr, Eg, Rg, data)
