@@ -388,8 +388,9 @@ def test_full_size_gradient_film(engine, oracle):
 
 def test_headline_cube_against_oracle(engine, oracle):
     """The measured workload itself: 256x256x128 with the dipolar field (padded 512x512x256)
-    and per-site K_i, the gradient and energy of a random configuration, then 2 Depondt steps,
-    against the oracle (whose set-up and steps take about a minute on the host cores)."""
+    and per-site K_i, the gradient and energy of a random configuration, then 2 Depondt steps
+    (and 2 Heun, 2 SIB and 1 RK4 step from the same start), against the oracle (whose set-up
+    and steps take about a minute on the host cores)."""
     cfg = configs.CUBE_256
     print_case("headline cube 256x256x128 against the oracle")
     K = configs.site_anisotropy(256 * 256 * 128, 20006)
@@ -401,11 +402,14 @@ def test_headline_cube_against_oracle(engine, oracle):
     g0, E0 = S.gradient_and_energy(s0)
     assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max()
     assert abs(E - E0) <= 1e-11 * abs(E0)
-    s, _, _ = _run_llg(e, SOLVERS["depondt"], 2)
-    s_ref, _, _, _ = S.llg(s0, SOLVERS["depondt"], 2, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
-    err = float(np.abs(s - s_ref).max())
-    print("  max |s - s_oracle| after 2 steps = %.3e" % err, flush=True)
-    assert err <= 1e-9
+    # the measured solver first, then the other three on the same workload (RK4: one step)
+    for name, n in (("depondt", 2), ("heun", 2), ("sib", 2), ("rk4", 1)):
+        e.set_spins(s0)
+        s, _, _ = _run_llg(e, SOLVERS[name], n)
+        s_ref, _, _, _ = S.llg(s0, SOLVERS[name], n, dt=cfg["llg_dt"], damping=cfg["llg_damping"])
+        err = float(np.abs(s - s_ref).max())
+        print("  %s: max |s - s_oracle| after %d steps = %.3e" % (name, n, err), flush=True)
+        assert err <= 1e-9, name
 
 
 def test_full_size_properties_cube(engine):
