@@ -1093,7 +1093,16 @@ static void parallel_host_copies( const std::vector<HostCopy> & copies )
     }
     std::vector<std::thread> pool;
     for( int t = 1; t < n_thr; ++t )
-        pool.emplace_back( work );
+    {
+        try
+        {
+            pool.emplace_back( work );
+        }
+        catch( ... )
+        {
+            break; // fewer threads: the pieces are taken from the shared counter either way
+        }
+    }
     work();
     for( auto & t : pool )
         t.join();
