@@ -730,20 +730,34 @@ __global__ void __launch_bounds__( 256, 3 ) k_p2_c1( DdiDev d, double2 * __restr
     constexpr int T = P2<N>::T, CW = P2Col<N>::CWC, LS = P2<N>::LS;
     const int n_tiles = ( d.Ha + CW - 1 ) / CW;
     // Blocks are dealt to the 8 XCDs round-robin; renumber so that each XCD runs a
-    // contiguous range of logical blocks.  With order_pairs the planes kb and Pb-kb,
-    // which read the same stored kernel-spectrum plane, follow each other.
+    // contiguous range of logical blocks.  The planes kb and Pb-kb read the same stored
+    // kernel-spectrum plane.
     const int nblk = gridDim.x;
     const int full = ( nblk / 8 ) * 8;
     int lb         = blockIdx.x;
     if( lb < full )
         lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
-    const int tile = lb % n_tiles;
-    const int o    = lb / n_tiles;
-    int kb         = o;
-    if( order_pairs && o > 0 )
+    // With order_pairs: plane 0, then for p = 1 .. Pb/2-1 the tiles of the planes p and Pb-p
+    // interleaved (tile j of p, tile j of Pb-p, ...), then plane Pb/2, so the second read of
+    // a stored spectrum tile follows the first by one block and hits in L2.
+    int tile = lb % n_tiles;
+    int kb   = lb / n_tiles;
+    if( order_pairs && lb >= n_tiles )
     {
-        const int p = ( o + 1 ) / 2;
-        kb          = ( o & 1 ) ? p : d.Pb - p;
+        const int r     = lb - n_tiles;
+        const int pairs = d.Pb / 2 - 1;
+        if( r < 2 * n_tiles * pairs )
+        {
+            const int p = 1 + r / ( 2 * n_tiles );
+            const int w = r % ( 2 * n_tiles );
+            tile        = w >> 1;
+            kb          = ( w & 1 ) ? d.Pb - p : p;
+        }
+        else
+        {
+            tile = r - 2 * n_tiles * pairs;
+            kb   = d.Pb / 2;
+        }
     }
     const int l           = threadIdx.x % CW;
     const int t           = threadIdx.x / CW;
