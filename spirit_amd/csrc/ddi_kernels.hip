@@ -719,6 +719,33 @@ __global__ void __launch_bounds__( 512 ) k_p2_b( DdiDev d, double2 * __restrict_
     }
 }
 
+// Logical block lb of a pass C launch (n_tiles column tiles per kb plane, Pb planes) to its
+// (tile, kb).  With order_pairs: plane 0, then for 0 < p < Pb-p the tiles of the planes p and
+// Pb-p interleaved (tile j of p, tile j of Pb-p, ...), then plane Pb/2 if Pb is even, so the
+// second read of a stored spectrum tile (the two planes share it) follows the first by one
+// block and hits in L2.  A bijection on [0, n_tiles * Pb).
+__device__ __forceinline__ void pass_c_block( int lb, int n_tiles, int Pb, int order_pairs, int & tile, int & kb )
+{
+    tile = lb % n_tiles;
+    kb   = lb / n_tiles;
+    if( !order_pairs || lb < n_tiles )
+        return;
+    const int r     = lb - n_tiles;
+    const int pairs = ( Pb - 1 ) / 2; // planes p and Pb-p with 0 < p < Pb-p
+    if( r < 2 * n_tiles * pairs )
+    {
+        const int p = 1 + r / ( 2 * n_tiles );
+        const int w = r % ( 2 * n_tiles );
+        tile        = w >> 1;
+        kb          = ( w & 1 ) ? Pb - p : p;
+    }
+    else
+    {
+        tile = r - 2 * n_tiles * pairs;
+        kb   = Pb / 2;
+    }
+}
+
 // Pass C for one basis atom: CW columns (ka) of one kb plane; each thread holds the values
 // c = t + (N/8) k of all three components of its column, so after the three forward
 // transforms the spectrum multiply (3x3, symmetric) runs in registers on the same index set
@@ -737,28 +764,8 @@ __global__ void __launch_bounds__( 256, 3 ) k_p2_c1( DdiDev d, double2 * __restr
     int lb         = blockIdx.x;
     if( lb < full )
         lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
-    // With order_pairs: plane 0, then for p = 1 .. Pb/2-1 the tiles of the planes p and Pb-p
-    // interleaved (tile j of p, tile j of Pb-p, ...), then plane Pb/2, so the second read of
-    // a stored spectrum tile follows the first by one block and hits in L2.
-    int tile = lb % n_tiles;
-    int kb   = lb / n_tiles;
-    if( order_pairs && lb >= n_tiles )
-    {
-        const int r     = lb - n_tiles;
-        const int pairs = d.Pb / 2 - 1;
-        if( r < 2 * n_tiles * pairs )
-        {
-            const int p = 1 + r / ( 2 * n_tiles );
-            const int w = r % ( 2 * n_tiles );
-            tile        = w >> 1;
-            kb          = ( w & 1 ) ? d.Pb - p : p;
-        }
-        else
-        {
-            tile = r - 2 * n_tiles * pairs;
-            kb   = d.Pb / 2;
-        }
-    }
+    int tile, kb;
+    pass_c_block( lb, n_tiles, d.Pb, order_pairs, tile, kb );
     const int l           = threadIdx.x % CW;
     const int t           = threadIdx.x / CW;
     const int ka          = tile * CW + l;
@@ -879,20 +886,14 @@ __global__ void __launch_bounds__( 512 ) k_p4_c1( DdiDev d, double2 * __restrict
     extern __shared__ double2 lds[];
     constexpr int T = P4<N>::T, CW = P4Col<N>::CW, LS = P4<N>::LS;
     const int n_tiles = ( d.Ha + CW - 1 ) / CW;
-    // XCD-contiguous block order; planes kb and Pb-kb (one stored spectrum plane) back to back
+    // XCD-contiguous block order; planes kb and Pb-kb (one stored spectrum plane) interleaved
     const int nblk = gridDim.x;
     const int full = ( nblk / 8 ) * 8;
     int lb         = blockIdx.x;
     if( lb < full )
         lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
-    const int tile = lb % n_tiles;
-    const int o    = lb / n_tiles;
-    int kb         = o;
-    if( order_pairs && o > 0 )
-    {
-        const int p = ( o + 1 ) / 2;
-        kb          = ( o & 1 ) ? p : d.Pb - p;
-    }
+    int tile, kb;
+    pass_c_block( lb, n_tiles, d.Pb, order_pairs, tile, kb );
     const int l           = threadIdx.x % CW;
     const int t           = threadIdx.x / CW;
     const int ka          = tile * CW + l;
@@ -989,21 +990,15 @@ __global__ void __launch_bounds__( 1024 ) k_p2_c( DdiDev d, double2 * __restrict
     const int nthr    = blockDim.x;
     const int n_tiles = ( d.Ha + tk - 1 ) / tk;
     // Blocks are dealt to the 8 XCDs round-robin; renumber so that each XCD runs a
-    // contiguous range of logical blocks.  With order_pairs the planes kb and Pb-kb,
-    // which read the same stored kernel-spectrum plane, follow each other.
+    // contiguous range of logical blocks.  The planes kb and Pb-kb read the same stored
+    // kernel-spectrum plane.
     const int nblk = gridDim.x;
     const int full = ( nblk / 8 ) * 8;
     int lb         = blockIdx.x;
     if( lb < full )
         lb = ( lb % 8 ) * ( full / 8 ) + lb / 8;
-    const int tile = lb % n_tiles;
-    const int o    = lb / n_tiles;
-    int kb         = o;
-    if( order_pairs && o > 0 )
-    {
-        const int p = ( o + 1 ) / 2;
-        kb          = ( o & 1 ) ? p : d.Pb - p;
-    }
+    int tile, kb;
+    pass_c_block( lb, n_tiles, d.Pb, order_pairs, tile, kb );
     const int ka0         = tile * tk;
     const long long plane = (long long)d.Pb * d.Hs;
     const int Nc          = d.Nc;
